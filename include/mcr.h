@@ -1,0 +1,245 @@
+/*
+ * mcr.h — C-ABI of the MI355X trace path for MeerCADRaytracer scenes.
+ *
+ * Plain pointers and sizes only: no torch or HIP types cross this boundary. Every entry point
+ * returns MCR_OK (0) or a negative MCR_E_* code; the message for the last failure on the
+ * calling thread is returned by mcr_last_error().
+ *
+ * The data structs restate the reference's plain-data scene format field for field
+ * (src/types.ts:43-211). Arrays are caller-owned; the library copies what it keeps.
+ */
+#ifndef MCR_H
+#define MCR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------ */
+#define MCR_OK 0
+#define MCR_E_INVALID (-1)     /* a null pointer, a size out of range, an unknown enum value */
+#define MCR_E_UNSUPPORTED (-2) /* scene data this build does not trace (textures, environments) */
+#define MCR_E_DEVICE (-3)      /* a HIP call failed */
+#define MCR_E_NO_SCENE (-4)    /* protocol.ts:77-79: the tile refers to a scene not received */
+#define MCR_E_OVERFLOW (-5)    /* a fixed device capacity was exceeded */
+#define MCR_E_CANCELLED (-6)   /* the job's generation was cancelled before it ran */
+
+/* ---- enums restated from src/constants.ts and src/types.ts ------------------------------ */
+enum { MCR_FRONT_SIDE = 0, MCR_BACK_SIDE = 1, MCR_DOUBLE_SIDE = 2 };           /* constants.ts:5-7 */
+enum { MCR_MULTIPLY = 0, MCR_MIX = 1, MCR_ADD = 2 };                           /* constants.ts:9-11 */
+enum {                                                                         /* constants.ts:13-20 */
+  MCR_NO_TONE_MAPPING = 0, MCR_LINEAR_TONE_MAPPING = 1, MCR_REINHARD_TONE_MAPPING = 2,
+  MCR_CINEON_TONE_MAPPING = 3, MCR_ACES_FILMIC_TONE_MAPPING = 4, MCR_CUSTOM_TONE_MAPPING = 5,
+  MCR_AGX_TONE_MAPPING = 6, MCR_NEUTRAL_TONE_MAPPING = 7
+};
+enum { MCR_TRI_OPAQUE = 1, MCR_TRI_CAST_SHADOW = 2, MCR_TRI_RECEIVE_SHADOW = 4 }; /* types.ts:133-135 */
+enum {                                                                          /* types.ts:41 */
+  MCR_MODEL_PHYSICAL = 0, MCR_MODEL_PHONG = 1, MCR_MODEL_LAMBERT = 2, MCR_MODEL_TOON = 3,
+  MCR_MODEL_BASIC = 4
+};
+enum {                                                                          /* types.ts:91-113 */
+  MCR_LIGHT_AMBIENT = 0, MCR_LIGHT_HEMISPHERE = 1, MCR_LIGHT_DIRECTIONAL = 2, MCR_LIGHT_POINT = 3,
+  MCR_LIGHT_SPOT = 4
+};
+enum { MCR_SHADOWS_ALL = 0, MCR_SHADOWS_CAST = 1, MCR_SHADOWS_NONE = 2 };         /* types.ts:195 */
+enum { MCR_FOG_NONE = 0, MCR_FOG_LINEAR = 1, MCR_FOG_EXP2 = 2 };                 /* types.ts:115-121 */
+enum { MCR_BACKGROUND_CLEAR = 0, MCR_BACKGROUND_COLOR = 1, MCR_BACKGROUND_SCREEN = 2,
+       MCR_BACKGROUND_ENVIRONMENT = 3 };                                         /* types.ts:123-127 */
+
+/* ---- plain-data scene (types.ts) -------------------------------------------------------- */
+
+/* CompiledGeometry (types.ts:137-156) with its CompiledBvh (types.ts:158-167). Per-triangle
+ * arrays are in BVH leaf order (compile.ts:363-387). colors may be NULL (no vertex colours). */
+typedef struct mcr_geometry {
+  uint32_t triangle_count;
+  uint32_t node_count;
+  const float *triangles;   /* n*9: v0, e1 = v1-v0, e2 = v2-v0 */
+  const float *normals;     /* n*9 */
+  const float *uvs;         /* n*6 */
+  const float *tangents;    /* n*4 */
+  const float *colors;      /* n*12 or NULL */
+  const uint32_t *materials;/* n */
+  const uint8_t *flags;     /* n, MCR_TRI_* bits */
+  const uint8_t *sides;     /* n, MCR_*_SIDE */
+  const float *bvh_bounds;  /* node_count*6: min xyz, max xyz */
+  const int32_t *bvh_nodes; /* node_count*3: first|left, count, axis */
+  double epsilon;
+} mcr_geometry;
+
+/* CompiledMaterial (types.ts:43-89). Texture and environment indices are -1 when absent. */
+typedef struct mcr_material {
+  int32_t model, side;
+  double color[3];
+  double opacity;
+  int32_t transparent;
+  int32_t vertex_colors;
+  double alpha_test;
+  int32_t flat_shading, tone_mapped;
+  int32_t map, alpha_map, emissive_map, normal_map, ao_map, roughness_map, metalness_map;
+  int32_t env_map;
+  double emissive[3];
+  double normal_scale[2];
+  double ao_map_intensity;
+  double roughness, metalness;
+  int32_t physical, combine;
+  double ior, specular_intensity;
+  double specular_color[3];
+  double transmission, thickness;
+  double attenuation[3];
+  double shininess;
+  double specular[3];
+  double env_map_intensity, reflectivity;
+} mcr_material;
+
+/* CompiledLight (types.ts:91-113): one struct for all kinds; fields a kind lacks are ignored. */
+typedef struct mcr_light {
+  int32_t type, cast_shadow;
+  double color[3];
+  double ground_color[3];
+  double direction[3];
+  double position[3];
+  double distance, decay, cone_cos, penumbra_cos;
+} mcr_light;
+
+/* Everything of CompiledScene but the geometry (protocol.ts:19-27 ShadingPayload). This build
+ * traces no textures or environments: texture_count and environment_count must be 0, every
+ * texture index -1, and background_kind CLEAR or COLOR; otherwise MCR_E_UNSUPPORTED. */
+typedef struct mcr_shading {
+  uint32_t material_count, light_count;
+  const mcr_material *materials;
+  const mcr_light *lights;
+  uint32_t texture_count, environment_count;
+  int32_t background_kind;
+  int32_t environment;       /* scene.environment index or -1 */
+  double background_color[3];
+  int32_t fog_kind, pad0;    /* MCR_FOG_* */
+  double fog_color[3];
+  double fog_near, fog_far, fog_density;
+} mcr_shading;
+
+/* CompiledCamera (types.ts:181-190), matrices column-major. */
+typedef struct mcr_camera {
+  double projection_inverse[16];
+  double world[16];
+  double forward[3];
+  double position[3];
+  int32_t orthographic, pad0;
+} mcr_camera;
+
+/* TraceSettings (types.ts:193-204). seed is ToInt32(settings.seed) as random.ts:7 mixes it. */
+typedef struct mcr_settings {
+  int32_t max_bounces, shadows;
+  int32_t global_illumination, tone_mapping;
+  double tone_mapping_exposure;
+  double clear_color[3];
+  double clear_alpha;
+  uint32_t seed, pad0;
+} mcr_settings;
+
+/* Tile (types.ts:206-211). */
+typedef struct mcr_tile {
+  int32_t x, y, width, height;
+} mcr_tile;
+
+/* One item of a batched render: a tile of an image for one pass. */
+typedef struct mcr_tile_job {
+  mcr_tile tile;
+  uint32_t width, height; /* image size */
+  uint32_t pass, block;
+  uint64_t out_offset;    /* float offset of this tile's RGBA in the batch output buffer */
+} mcr_tile_job;
+
+/* Per-call ray and work counters (kernel counter mode). Mrays/s counts closest + any queries. */
+typedef struct mcr_counters {
+  uint64_t closest_queries;  /* bvh.ts:202 intersectClosest calls */
+  uint64_t any_queries;      /* bvh.ts:289 intersectAny calls */
+  uint64_t node_visits;      /* inner-node visits (closest: 64-byte child-pair reads) */
+  uint64_t leaf_visits;
+  uint64_t triangle_tests;   /* triangle records read */
+  uint64_t paths;            /* camera samples traced */
+} mcr_counters;
+
+typedef struct mcr_ctx mcr_ctx;
+typedef struct mcr_scene mcr_scene;
+
+/* ---- context ------------------------------------------------------------------------------ */
+int mcr_create(int device, mcr_ctx **out);
+void mcr_destroy(mcr_ctx *ctx);
+const char *mcr_last_error(void);
+const char *mcr_version(void);
+
+/* ---- builder: replaces buildBvh (src/bvh.ts:16-178), called at src/compile.ts:361 -------- */
+/* Builds the binned-SAH BVH of n triangles (v0, e1, e2; n*9 floats) on the GPU. The caller
+ * allocates bounds_out[(2n-1)*6], nodes_out[(2n-1)*3] and order_out[n]; *node_count_out gets
+ * the node count. Arrays are bit-identical to buildBvh's. n = 0 gives 0 nodes (bvh.ts:18). */
+int mcr_build_bvh(mcr_ctx *ctx, const float *triangles, uint32_t n, float *bounds_out,
+                  int32_t *nodes_out, uint32_t *order_out, uint32_t *node_count_out);
+/* As mcr_build_bvh, with triangles and outputs already in device memory (no host copies). */
+int mcr_build_bvh_device(mcr_ctx *ctx, const float *d_triangles, uint32_t n, float *d_bounds,
+                         int32_t *d_nodes, uint32_t *d_order, uint32_t *node_count_out);
+
+/* ---- scene residency: the geometry / shading messages of src/protocol.ts:30-32,64-74 ------ */
+/* Uploads a compiled geometry (BVH order, with its BVH) once; it stays in HBM until released.
+ * version is the geometryVersion tile requests must name (protocol.ts:77). */
+int mcr_scene_create(mcr_ctx *ctx, mcr_scene **out);
+int mcr_scene_set_geometry(mcr_scene *scene, const mcr_geometry *geometry, uint32_t version);
+int mcr_scene_set_shading(mcr_scene *scene, const mcr_shading *shading, uint32_t version);
+void mcr_scene_release(mcr_scene *scene);
+/* Device bytes the scene holds (geometry + BVH + shading), for admission by memory budget. */
+uint64_t mcr_scene_bytes(const mcr_scene *scene);
+
+/* ---- tile render: replaces Tracer.renderTile (src/tracer.ts:62-89) ------------------------ */
+/* Traces one sample per pixel (or per block) of the tile and writes premultiplied RGBA floats,
+ * row by row over the tile (tile.width*tile.height*4), exactly as renderTile returns them.
+ * geometry_version/shading_version must match what the scene holds (protocol.ts:77-79). */
+int mcr_render_tile(mcr_ctx *ctx, mcr_scene *scene, uint32_t geometry_version,
+                    uint32_t shading_version, const mcr_camera *camera,
+                    const mcr_settings *settings, mcr_tile tile, uint32_t width, uint32_t height,
+                    uint32_t pass, uint32_t block, float *rgba_out, mcr_counters *counters);
+/* Many tile jobs of one scene, camera and settings in one device launch sequence. out receives
+ * each job's RGBA at job.out_offset. out_device != 0 means out is a device pointer. */
+int mcr_render_tiles(mcr_ctx *ctx, mcr_scene *scene, const mcr_camera *camera,
+                     const mcr_settings *settings, const mcr_tile_job *jobs, uint32_t job_count,
+                     float *out, int out_device, mcr_counters *counters);
+
+/* ---- accumulate + encode: replaces writeTile/encodeTile (src/progressive.ts:288-313) ------- */
+/* sum may be NULL (preview tiles). image is width*height*4 bytes, straight alpha. */
+int mcr_encode_tile(mcr_ctx *ctx, uint8_t *image, uint32_t width, mcr_tile tile,
+                    const float *data, float *sum, uint32_t count, int srgb, int device_ptrs);
+
+/* ---- ray queries: intersectClosest / intersectAny (src/bvh.ts:202-332) over ray batches --- */
+/* rays: n*6 doubles (origin, direction). Outputs per ray: triangle (-1 = none), t, u, v, front.
+ * Any output pointer may be NULL. device_ptrs != 0: every pointer is device memory. */
+int mcr_intersect_closest(mcr_ctx *ctx, mcr_scene *scene, const double *rays, uint32_t n,
+                          double t_min, double t_max, int cull, int32_t *tri_out, double *t_out,
+                          double *u_out, double *v_out, uint8_t *front_out, int device_ptrs,
+                          mcr_counters *counters);
+/* Number of hits intersectAny visits per ray with a visitor that always continues
+ * (tests/bvh.test.ts:100-103 counts them so). */
+int mcr_intersect_any_count(mcr_ctx *ctx, mcr_scene *scene, const double *rays, uint32_t n,
+                            double t_min, double t_max, uint32_t *count_out, int device_ptrs,
+                            mcr_counters *counters);
+
+/* ---- device buffers, for hosts without a GPU allocator of their own ----------------------- */
+int mcr_device_alloc(mcr_ctx *ctx, uint64_t bytes, void **out);
+int mcr_device_free(mcr_ctx *ctx, void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device. Synchronous. */
+int mcr_copy(mcr_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind);
+int mcr_memset(mcr_ctx *ctx, void *dst, int value, uint64_t bytes);
+
+/* ---- timing: device time of the last call's kernels, by kernel family (milliseconds) ------ */
+typedef struct mcr_timing {
+  double closest_ms, any_ms, shade_ms, other_ms, total_ms;
+  uint64_t closest_launches, any_launches;
+} mcr_timing;
+int mcr_last_timing(mcr_ctx *ctx, mcr_timing *out);
+int mcr_set_timing(mcr_ctx *ctx, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCR_H */

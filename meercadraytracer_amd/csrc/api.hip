@@ -1,0 +1,828 @@
+// api.hip — the C-ABI of include/mcr.h: context, resident scenes, batched tile rendering, ray
+// queries, tile encoding and timing. Host orchestration of the wavefront loop lives here; the
+// kernels are in trace.hip and bvh_build.hip.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mcr.h"
+#include "builder.h"
+#include "device.h"
+#include "dfg_lut.h"
+#include "kernels.h"
+#include "shading.h"
+#include "wavefront.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(MCR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+
+// Grid of persistent traversal blocks: enough waves to fill 256 CUs several times over.
+const uint32_t TRAVERSE_GRID = 256 * 8;
+const uint32_t TRAVERSE_THREADS = TRAVERSE_GRID * 256;
+const uint32_t DEFAULT_CAPACITY = 1u << 21;  // path slots per wavefront chunk
+
+}  // namespace
+
+struct mcr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // wavefront
+  Wavefront w = {};
+  void *w_block = nullptr;
+  uint32_t capacity = 0;
+  // traversal scratch
+  int32_t *spill = nullptr;
+  double *seen = nullptr;
+  DevCounters *counters = nullptr;
+  JobDev *jobs = nullptr;
+  uint32_t jobs_cap = 0;
+  float *out_tmp = nullptr;
+  size_t out_tmp_cap = 0;
+  uint32_t *counts_host = nullptr;  // pinned
+  // builder workspace
+  BuilderWorkspace builder;
+  // timing
+  bool timing = false;
+  mcr_timing last_timing = {};
+  hipEvent_t ev[8] = {};
+  std::mutex mu;
+};
+
+struct mcr_scene {
+  mcr_ctx *ctx = nullptr;
+  DevScene d = {};
+  uint32_t geometry_version = 0, shading_version = 0;
+  bool has_geometry = false, has_shading = false;
+  float4 *tri = nullptr;
+  float4 *inner = nullptr;
+  int2 *longleaf = nullptr;
+  float *normals = nullptr;
+  float *colors = nullptr;
+  DevMaterial *materials = nullptr;
+  DevLight *lights = nullptr;
+  uint64_t bytes_geometry = 0, bytes_shading = 0;
+};
+
+namespace {
+
+void free_geometry(mcr_scene *s) {
+  hipFree(s->tri);
+  hipFree(s->inner);
+  hipFree(s->longleaf);
+  hipFree(s->normals);
+  hipFree(s->colors);
+  s->tri = nullptr;
+  s->inner = nullptr;
+  s->longleaf = nullptr;
+  s->normals = nullptr;
+  s->colors = nullptr;
+  s->bytes_geometry = 0;
+  s->has_geometry = false;
+}
+
+int ensure_wavefront(mcr_ctx *c, uint32_t capacity) {
+  if (c->capacity >= capacity) return MCR_OK;
+  hipFree(c->w_block);
+  c->w_block = nullptr;
+  c->capacity = 0;
+  size_t P = capacity;
+  size_t L = (size_t)MCR_MAX_LIGHTS * P;
+  size_t bytes = 0;
+  auto take = [&](size_t n, size_t align = 256) {
+    size_t off = (bytes + align - 1) / align * align;
+    bytes = off + n;
+    return off;
+  };
+  size_t o_f = take(sizeof(double) * F_COUNT * P);
+  size_t o_i = take(sizeof(int32_t) * I_COUNT * P);
+  size_t o_mode = take(P);
+  size_t o_contrib = take(sizeof(double) * 3 * L);
+  size_t o_shadow = take(sizeof(double) * 3 * L);
+  size_t o_sray = take(sizeof(double) * 7 * L);
+  size_t o_status = take(L);
+  size_t o_q0 = take(sizeof(int32_t) * P);
+  size_t o_q1 = take(sizeof(int32_t) * P);
+  size_t o_counts = take(sizeof(uint32_t) * 8);
+  size_t o_sref = take(sizeof(int32_t) * L);
+  HIP_TRY(hipMalloc(&c->w_block, bytes));
+  char *b = (char *)c->w_block;
+  Wavefront &w = c->w;
+  w.capacity = capacity;
+  w.f = (double *)(b + o_f);
+  w.i = (int32_t *)(b + o_i);
+  w.mode = (uint8_t *)(b + o_mode);
+  w.contrib = (double *)(b + o_contrib);
+  w.shadow = (double *)(b + o_shadow);
+  w.sray = (double *)(b + o_sray);
+  w.status = (uint8_t *)(b + o_status);
+  w.queue[0] = (int32_t *)(b + o_q0);
+  w.queue[1] = (int32_t *)(b + o_q1);
+  w.counts = (uint32_t *)(b + o_counts);
+  w.sref = (int32_t *)(b + o_sref);
+  c->capacity = capacity;
+  return MCR_OK;
+}
+
+int ensure_jobs(mcr_ctx *c, uint32_t n) {
+  if (c->jobs_cap >= n) return MCR_OK;
+  hipFree(c->jobs);
+  c->jobs = nullptr;
+  c->jobs_cap = 0;
+  HIP_TRY(dalloc(&c->jobs, n));
+  c->jobs_cap = n;
+  return MCR_OK;
+}
+
+int ensure_out_tmp(mcr_ctx *c, size_t floats) {
+  if (c->out_tmp_cap >= floats) return MCR_OK;
+  hipFree(c->out_tmp);
+  c->out_tmp = nullptr;
+  c->out_tmp_cap = 0;
+  HIP_TRY(dalloc(&c->out_tmp, floats));
+  c->out_tmp_cap = floats;
+  return MCR_OK;
+}
+
+bool shading_supported(const mcr_shading *s, std::string &why) {
+  if (s->texture_count || s->environment_count || s->environment >= 0) {
+    why = "textures and environments are not traced by this build";
+    return false;
+  }
+  if (s->background_kind != MCR_BACKGROUND_CLEAR && s->background_kind != MCR_BACKGROUND_COLOR) {
+    why = "screen and environment backgrounds are not traced by this build";
+    return false;
+  }
+  for (uint32_t i = 0; i < s->material_count; i++) {
+    const mcr_material &m = s->materials[i];
+    if (m.map >= 0 || m.alpha_map >= 0 || m.emissive_map >= 0 || m.normal_map >= 0 || m.ao_map >= 0 ||
+        m.roughness_map >= 0 || m.metalness_map >= 0 || m.env_map >= 0) {
+      why = "material " + std::to_string(i) + " uses a texture or environment map";
+      return false;
+    }
+    if (m.model < 0 || m.model > 4) {
+      why = "material " + std::to_string(i) + " has an unknown model";
+      return false;
+    }
+  }
+  uint32_t punctual = 0;
+  for (uint32_t i = 0; i < s->light_count; i++) {
+    int t = s->lights[i].type;
+    if (t < 0 || t > 4) {
+      why = "light " + std::to_string(i) + " has an unknown type";
+      return false;
+    }
+    punctual++;
+  }
+  if (s->light_count > MCR_MAX_LIGHTS) {
+    why = "more than " + std::to_string(MCR_MAX_LIGHTS) + " lights";
+    return false;
+  }
+  (void)punctual;
+  return true;
+}
+
+// Child reference of node `node` in the device layout.
+int32_t child_ref(const int32_t *nodes, uint32_t node, const std::vector<int32_t> &inner_index,
+                  std::vector<int2> &longleaf) {
+  int32_t first = nodes[node * 3], count = nodes[node * 3 + 1];
+  if (count == 0) return inner_index[node];
+  if (count <= 31 && (uint32_t)first < MCR_MAX_TRIANGLES)
+    return (int32_t)(MCR_LEAF_BIT | ((uint32_t)count << MCR_LEAF_FIRST_BITS) | (uint32_t)first);
+  uint32_t id = (uint32_t)longleaf.size();
+  longleaf.push_back(make_int2(first, count));
+  return (int32_t)(MCR_LEAF_BIT | id);
+}
+
+}  // namespace
+
+// =================================================================================================
+extern "C" {
+
+const char *mcr_last_error(void) { return g_error.c_str(); }
+const char *mcr_version(void) { return "meercadraytracer_amd 0.1 (gfx950)"; }
+
+int mcr_create(int device, mcr_ctx **out) {
+  if (!out) return fail(MCR_E_INVALID, "out is null");
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(MCR_E_INVALID, "no such device");
+  HIP_TRY(hipSetDevice(device));
+  mcr_ctx *c = new mcr_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(MCR_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  float lut[512];
+  for (int i = 0; i < 512; i++) lut[i] = half_to_float_js(MCR_DFG_LUT_HALF[i]);
+  e = mcr_upload_dfg(lut);
+  if (e == hipSuccess) e = hipMalloc(&c->spill, sizeof(int32_t) * (size_t)(MCR_STACK_MAX - MCR_STACK_LDS) * TRAVERSE_THREADS);
+  if (e == hipSuccess) e = hipMalloc(&c->seen, sizeof(double) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS);
+  if (e == hipSuccess) e = hipMalloc(&c->counters, sizeof(DevCounters));
+  if (e == hipSuccess) e = hipHostMalloc(&c->counts_host, sizeof(uint32_t) * 8, hipHostMallocDefault);
+  for (int i = 0; i < 8 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+  if (e != hipSuccess) {
+    mcr_destroy(c);
+    return fail(MCR_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return MCR_OK;
+}
+
+void mcr_destroy(mcr_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  hipFree(c->w_block);
+  hipFree(c->spill);
+  hipFree(c->seen);
+  hipFree(c->counters);
+  hipFree(c->jobs);
+  hipFree(c->out_tmp);
+  if (c->counts_host) hipHostFree(c->counts_host);
+  builder_free(c->builder);
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int mcr_set_timing(mcr_ctx *c, int enabled) {
+  if (!c) return fail(MCR_E_INVALID, "ctx is null");
+  c->timing = enabled != 0;
+  return MCR_OK;
+}
+
+int mcr_last_timing(mcr_ctx *c, mcr_timing *out) {
+  if (!c || !out) return fail(MCR_E_INVALID, "null argument");
+  *out = c->last_timing;
+  return MCR_OK;
+}
+
+// ---- scenes -----------------------------------------------------------------------------------
+int mcr_scene_create(mcr_ctx *c, mcr_scene **out) {
+  if (!c || !out) return fail(MCR_E_INVALID, "null argument");
+  mcr_scene *s = new mcr_scene();
+  s->ctx = c;
+  s->d.root_ref = -1;
+  *out = s;
+  return MCR_OK;
+}
+
+void mcr_scene_release(mcr_scene *s) {
+  if (!s) return;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  free_geometry(s);
+  hipFree(s->materials);
+  hipFree(s->lights);
+  delete s;
+}
+
+uint64_t mcr_scene_bytes(const mcr_scene *s) { return s ? s->bytes_geometry + s->bytes_shading : 0; }
+
+int mcr_scene_set_geometry(mcr_scene *s, const mcr_geometry *g, uint32_t version) {
+  if (!s || !g) return fail(MCR_E_INVALID, "null argument");
+  uint32_t n = g->triangle_count, nodes = g->node_count;
+  if (n >= MCR_MAX_TRIANGLES) return fail(MCR_E_OVERFLOW, "too many triangles for this build (2^26 max)");
+  if (n > 0 && (!g->triangles || !g->normals || !g->materials || !g->flags || !g->sides || !g->bvh_bounds ||
+                !g->bvh_nodes || nodes == 0))
+    return fail(MCR_E_INVALID, "geometry arrays missing");
+  if (nodes > 0 && n == 0) return fail(MCR_E_INVALID, "nodes without triangles");
+  mcr_ctx *c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_geometry(s);
+  // Triangle records.
+  std::vector<float4> tri((size_t)n * 3);
+  for (uint32_t k = 0; k < n; k++) {
+    const float *t = g->triangles + (size_t)k * 9;
+    uint32_t word = (uint32_t)g->sides[k] | ((uint32_t)g->flags[k] << 8);
+    uint32_t mat = g->materials[k];
+    float wf, mf;
+    memcpy(&wf, &word, 4);
+    memcpy(&mf, &mat, 4);
+    tri[k * 3 + 0] = make_float4(t[0], t[1], t[2], t[3]);
+    tri[k * 3 + 1] = make_float4(t[4], t[5], t[6], t[7]);
+    tri[k * 3 + 2] = make_float4(t[8], wf, mf, 0.0f);
+  }
+  // Inner records: inner nodes numbered in node order.
+  std::vector<int32_t> inner_index(nodes, -1);
+  uint32_t ninner = 0;
+  for (uint32_t i = 0; i < nodes; i++) {
+    int32_t cnt = g->bvh_nodes[i * 3 + 1];
+    if (cnt < 0) return fail(MCR_E_INVALID, "negative leaf count");
+    if (cnt == 0) {
+      int32_t left = g->bvh_nodes[i * 3];
+      if (left <= (int32_t)i || (uint32_t)left + 1 >= nodes) return fail(MCR_E_INVALID, "bad child index");
+      int32_t axis = g->bvh_nodes[i * 3 + 2];
+      if (axis < 0 || axis > 2) return fail(MCR_E_INVALID, "bad split axis");
+      inner_index[i] = (int32_t)ninner++;
+    } else if ((uint32_t)g->bvh_nodes[i * 3] + (uint32_t)cnt > n) {
+      return fail(MCR_E_INVALID, "leaf range out of bounds");
+    }
+  }
+  // Every node reached exactly once from the root, and no deeper than the traversal stack holds
+  // (a tree of depth d needs at most d stack entries; buildBvh stops at depth 100).
+  {
+    std::vector<int32_t> depth(nodes, -1);
+    if (nodes) depth[0] = 0;
+    for (uint32_t i = 0; i < nodes; i++) {
+      if (depth[i] < 0) return fail(MCR_E_INVALID, "BVH node " + std::to_string(i) + " is unreachable");
+      if (g->bvh_nodes[i * 3 + 1] != 0) continue;
+      for (int c = 0; c < 2; c++) {
+        uint32_t child = (uint32_t)g->bvh_nodes[i * 3] + c;
+        if (depth[child] >= 0) return fail(MCR_E_INVALID, "BVH node " + std::to_string(child) + " has two parents");
+        depth[child] = depth[i] + 1;
+        if (depth[child] >= MCR_STACK_MAX - 1) return fail(MCR_E_OVERFLOW, "BVH deeper than the traversal stack");
+      }
+    }
+  }
+  std::vector<float4> inner((size_t)ninner * 4);
+  std::vector<int2> longleaf;
+  for (uint32_t i = 0; i < nodes; i++) {
+    if (inner_index[i] < 0) continue;
+    int32_t left = g->bvh_nodes[i * 3];
+    const float *bl = g->bvh_bounds + (size_t)left * 6, *br = g->bvh_bounds + (size_t)(left + 1) * 6;
+    size_t r = (size_t)inner_index[i] * 4;
+    inner[r + 0] = make_float4(bl[0], bl[1], bl[2], bl[3]);
+    inner[r + 1] = make_float4(bl[4], bl[5], br[0], br[1]);
+    inner[r + 2] = make_float4(br[2], br[3], br[4], br[5]);
+    int32_t refs[4] = {child_ref(g->bvh_nodes, left, inner_index, longleaf),
+                       child_ref(g->bvh_nodes, left + 1, inner_index, longleaf), g->bvh_nodes[i * 3 + 2], 0};
+    memcpy(&inner[r + 3], refs, 16);
+  }
+  DevScene &d = s->d;
+  d.triangle_count = n;
+  d.inner_count = ninner;
+  d.epsilon = g->epsilon;
+  d.root_ref = nodes ? child_ref(g->bvh_nodes, 0, inner_index, longleaf) : -1;
+  for (int k = 0; k < 6; k++) d.root_box[k] = nodes ? g->bvh_bounds[k] : 0.0f;
+  HIP_TRY(dalloc(&s->tri, tri.size()));
+  HIP_TRY(dalloc(&s->inner, inner.size()));
+  HIP_TRY(dalloc(&s->longleaf, longleaf.size()));
+  HIP_TRY(dalloc(&s->normals, (size_t)n * 9));
+  HIP_TRY(hipMemcpy(s->tri, tri.data(), tri.size() * sizeof(float4), hipMemcpyHostToDevice));
+  if (!inner.empty()) HIP_TRY(hipMemcpy(s->inner, inner.data(), inner.size() * sizeof(float4), hipMemcpyHostToDevice));
+  if (!longleaf.empty())
+    HIP_TRY(hipMemcpy(s->longleaf, longleaf.data(), longleaf.size() * sizeof(int2), hipMemcpyHostToDevice));
+  if (n) HIP_TRY(hipMemcpy(s->normals, g->normals, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice));
+  s->colors = nullptr;
+  if (g->colors && n) {
+    HIP_TRY(dalloc(&s->colors, (size_t)n * 12));
+    HIP_TRY(hipMemcpy(s->colors, g->colors, (size_t)n * 12 * sizeof(float), hipMemcpyHostToDevice));
+  }
+  d.tri = s->tri;
+  d.inner = s->inner;
+  d.longleaf = s->longleaf;
+  d.normals = s->normals;
+  d.colors = s->colors;
+  s->bytes_geometry = tri.size() * sizeof(float4) + inner.size() * sizeof(float4) +
+                      longleaf.size() * sizeof(int2) + (size_t)n * 9 * 4 + (s->colors ? (size_t)n * 48 : 0);
+  s->geometry_version = version;
+  s->has_geometry = true;
+  return MCR_OK;
+}
+
+int mcr_scene_set_shading(mcr_scene *s, const mcr_shading *sh, uint32_t version) {
+  if (!s || !sh) return fail(MCR_E_INVALID, "null argument");
+  std::string why;
+  if (!shading_supported(sh, why)) return fail(MCR_E_UNSUPPORTED, why);
+  mcr_ctx *c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<DevMaterial> mats(std::max<uint32_t>(1, sh->material_count));
+  memset(mats.data(), 0, mats.size() * sizeof(DevMaterial));
+  for (uint32_t i = 0; i < sh->material_count; i++) {
+    const mcr_material &m = sh->materials[i];
+    DevMaterial &d = mats[i];
+    d.model = m.model;
+    d.side = m.side;
+    d.transparent = m.transparent;
+    d.vertex_colors = m.vertex_colors;
+    d.tone_mapped = m.tone_mapped;
+    d.physical = m.physical;
+    for (int k = 0; k < 3; k++) {
+      d.color[k] = m.color[k];
+      d.emissive[k] = m.emissive[k];
+      d.specular_color[k] = m.specular_color[k];
+      d.attenuation[k] = m.attenuation[k];
+      d.specular[k] = m.specular[k];
+    }
+    d.opacity = m.opacity;
+    d.alpha_test = m.alpha_test;
+    d.roughness = m.roughness;
+    d.metalness = m.metalness;
+    d.ior = m.ior;
+    d.specular_intensity = m.specular_intensity;
+    d.transmission = m.transmission;
+    d.thickness = m.thickness;
+    d.shininess = m.shininess;
+  }
+  std::vector<DevLight> lights(std::max<uint32_t>(1, sh->light_count));
+  memset(lights.data(), 0, lights.size() * sizeof(DevLight));
+  for (uint32_t i = 0; i < sh->light_count; i++) {
+    const mcr_light &l = sh->lights[i];
+    DevLight &d = lights[i];
+    d.type = l.type;
+    d.cast_shadow = l.cast_shadow;
+    for (int k = 0; k < 3; k++) {
+      d.color[k] = l.color[k];
+      d.ground_color[k] = l.ground_color[k];
+      d.direction[k] = l.direction[k];
+      d.position[k] = l.position[k];
+    }
+    d.distance = l.distance;
+    d.decay = l.decay;
+    d.cone_cos = l.cone_cos;
+    d.penumbra_cos = l.penumbra_cos;
+  }
+  hipFree(s->materials);
+  hipFree(s->lights);
+  s->materials = nullptr;
+  s->lights = nullptr;
+  HIP_TRY(dalloc(&s->materials, mats.size()));
+  HIP_TRY(dalloc(&s->lights, lights.size()));
+  HIP_TRY(hipMemcpy(s->materials, mats.data(), mats.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->lights, lights.data(), lights.size() * sizeof(DevLight), hipMemcpyHostToDevice));
+  DevScene &d = s->d;
+  d.materials = s->materials;
+  d.lights = s->lights;
+  d.material_count = sh->material_count;
+  d.light_count = sh->light_count;
+  d.background_kind = sh->background_kind;
+  d.fog_kind = sh->fog_kind;
+  for (int k = 0; k < 3; k++) {
+    d.background_color[k] = sh->background_color[k];
+    d.fog_color[k] = sh->fog_color[k];
+  }
+  d.fog_near = sh->fog_near;
+  d.fog_far = sh->fog_far;
+  d.fog_density = sh->fog_density;
+  s->bytes_shading = mats.size() * sizeof(DevMaterial) + lights.size() * sizeof(DevLight);
+  s->shading_version = version;
+  s->has_shading = true;
+  return MCR_OK;
+}
+
+// ---- rendering --------------------------------------------------------------------------------
+static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const mcr_settings *st,
+                       const mcr_tile_job *jobs, uint32_t njobs, float *dout, mcr_counters *counters) {
+  // Validate and lay out the jobs: sample ranges and per-job columns.
+  std::vector<JobDev> jd(njobs);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < njobs; i++) {
+    const mcr_tile_job &j = jobs[i];
+    uint32_t block = j.block < 1 ? 1 : j.block;
+    if (j.tile.width <= 0 || j.tile.height <= 0 || j.tile.x < 0 || j.tile.y < 0 ||
+        (uint64_t)j.tile.x + j.tile.width > j.width || (uint64_t)j.tile.y + j.tile.height > j.height)
+      return fail(MCR_E_INVALID, "tile " + std::to_string(i) + " lies outside its image");
+    if (j.out_offset % 4) return fail(MCR_E_INVALID, "out_offset must be a multiple of 4 floats");
+    uint32_t cols = (j.tile.width + block - 1) / block, rows = (j.tile.height + block - 1) / block;
+    jd[i] = JobDev{j.tile.x, j.tile.y, j.tile.width, j.tile.height, j.width, j.height, j.pass, block,
+                   j.out_offset, (uint32_t)total, cols};
+    total += (uint64_t)cols * rows;
+  }
+  if (total >= (1ull << 31)) return fail(MCR_E_OVERFLOW, "too many samples in one call");
+  if (njobs == 0 || total == 0) return MCR_OK;
+  int rc = ensure_jobs(c, njobs);
+  if (rc) return rc;
+  uint32_t cap = (uint32_t)std::min<uint64_t>(total, DEFAULT_CAPACITY);
+  cap = std::max<uint32_t>(cap, 256);
+  rc = ensure_wavefront(c, std::max(cap, c->capacity));
+  if (rc) return rc;
+  hipStream_t sm = c->stream;
+  HIP_TRY(hipMemcpyAsync(c->jobs, jd.data(), njobs * sizeof(JobDev), hipMemcpyHostToDevice, sm));
+  HIP_TRY(hipMemsetAsync(c->counters, 0, sizeof(DevCounters), sm));
+  const Wavefront &w = c->w;
+  int cast_only = st->shadows == MCR_SHADOWS_CAST;
+  uint32_t spill_stride = TRAVERSE_THREADS;
+  double t_closest = 0, t_any = 0, t_shade = 0;
+  uint64_t n_closest = 0, n_any = 0;
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev[6], sm));
+  for (uint64_t s0 = 0; s0 < total; s0 += c->capacity) {
+    uint32_t count = (uint32_t)std::min<uint64_t>(c->capacity, total - s0);
+    c->counts_host[0] = count;
+    HIP_TRY(hipMemsetAsync(w.counts, 0, sizeof(uint32_t) * 4, sm));
+    HIP_TRY(hipMemcpyAsync(w.counts, c->counts_host, sizeof(uint32_t), hipMemcpyHostToDevice, sm));
+    launch_raygen(sm, c->jobs, njobs, (uint32_t)s0, count, w, *camera, st->seed);
+    int q = 0;
+    uint32_t live = count;
+    while (live > 0) {
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], sm));
+      launch_extend(sm, s->d, w, q, std::min<uint32_t>(TRAVERSE_GRID, (live + 255) / 256), c->spill,
+                    spill_stride, c->counters);
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], sm));
+      HIP_TRY(hipMemsetAsync(w.counts + 2, 0, sizeof(uint32_t), sm));
+      launch_shade(sm, s->d, w, q, live, *st, *camera, c->jobs, njobs, dout);
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], sm));
+      uint32_t max_shadow = live * std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
+      if (max_shadow)
+        launch_shadow(sm, s->d, w, cast_only, std::min<uint32_t>(TRAVERSE_GRID, (max_shadow + 255) / 256),
+                      c->spill, spill_stride, c->seen, c->counters);
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], sm));
+      HIP_TRY(hipMemsetAsync(w.counts + (q ^ 1), 0, sizeof(uint32_t), sm));
+      launch_resolve(sm, w, q, live, *st, s->d, c->jobs, njobs, dout);
+      HIP_TRY(hipMemcpyAsync(c->counts_host, w.counts, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, sm));
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], sm));
+      HIP_TRY(hipStreamSynchronize(sm));
+      if (c->timing) {
+        float a, b, d;
+        hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+        hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+        hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
+        t_closest += a;
+        t_shade += b;
+        t_any += d;
+        n_closest++;
+        if (max_shadow) n_any++;
+      }
+      live = c->counts_host[q ^ 1];
+      if (c->counts_host[3]) return fail(MCR_E_OVERFLOW, "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
+      q ^= 1;
+    }
+  }
+  if (c->timing) {
+    HIP_TRY(hipEventRecord(c->ev[7], sm));
+    HIP_TRY(hipEventSynchronize(c->ev[7]));
+    float tot;
+    hipEventElapsedTime(&tot, c->ev[6], c->ev[7]);
+    c->last_timing = mcr_timing{t_closest, t_any, t_shade, tot - t_closest - t_any - t_shade, tot, n_closest, n_any};
+  }
+  if (counters) {
+    DevCounters hc;
+    HIP_TRY(hipMemcpy(&hc, c->counters, sizeof(hc), hipMemcpyDeviceToHost));
+    counters->closest_queries += hc.closest_queries;
+    counters->any_queries += hc.any_queries;
+    counters->node_visits += hc.node_visits;
+    counters->leaf_visits += hc.leaf_visits;
+    counters->triangle_tests += hc.triangle_tests;
+    counters->paths += total;
+  }
+  return MCR_OK;
+}
+
+int mcr_render_tiles(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const mcr_settings *st,
+                     const mcr_tile_job *jobs, uint32_t njobs, float *out, int out_device,
+                     mcr_counters *counters) {
+  if (!c || !s || !camera || !st || (!jobs && njobs) || !out) return fail(MCR_E_INVALID, "null argument");
+  if (!s->has_geometry || !s->has_shading)
+    return fail(MCR_E_NO_SCENE, "the tile refers to a scene this worker has not received");
+  std::lock_guard<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  uint64_t floats = 0;
+  for (uint32_t i = 0; i < njobs; i++)
+    floats = std::max<uint64_t>(floats, jobs[i].out_offset + (uint64_t)jobs[i].tile.width * jobs[i].tile.height * 4);
+  float *dout = out;
+  if (!out_device) {
+    int rc = ensure_out_tmp(c, floats);
+    if (rc) return rc;
+    dout = c->out_tmp;
+  }
+  int rc = render_jobs(c, s, camera, st, jobs, njobs, dout, counters);
+  if (rc) return rc;
+  if (!out_device && floats)
+    HIP_TRY(hipMemcpyAsync(out, dout, floats * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCR_OK;
+}
+
+int mcr_render_tile(mcr_ctx *c, mcr_scene *s, uint32_t geometry_version, uint32_t shading_version,
+                    const mcr_camera *camera, const mcr_settings *settings, mcr_tile tile,
+                    uint32_t width, uint32_t height, uint32_t pass, uint32_t block, float *rgba_out,
+                    mcr_counters *counters) {
+  if (!c || !s) return fail(MCR_E_INVALID, "null argument");
+  if (!s->has_geometry || !s->has_shading || s->geometry_version != geometry_version ||
+      s->shading_version != shading_version)
+    return fail(MCR_E_NO_SCENE, "the tile refers to a scene this worker has not received");
+  mcr_tile_job j{tile, width, height, pass, block, 0};
+  return mcr_render_tiles(c, s, camera, settings, &j, 1, rgba_out, 0, counters);
+}
+
+int mcr_encode_tile(mcr_ctx *c, uint8_t *image, uint32_t width, mcr_tile tile, const float *data,
+                    float *sum, uint32_t count, int srgb, int device_ptrs) {
+  if (!c || !image || !data) return fail(MCR_E_INVALID, "null argument");
+  if (!device_ptrs) return fail(MCR_E_INVALID, "mcr_encode_tile works on device buffers");
+  if (tile.width <= 0 || tile.height <= 0) return MCR_OK;
+  if (count < 1) return fail(MCR_E_INVALID, "count must be at least 1");
+  std::lock_guard<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  launch_encode(c->stream, image, width, tile, data, sum, count, srgb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCR_OK;
+}
+
+// ---- ray queries ------------------------------------------------------------------------------
+int mcr_intersect_closest(mcr_ctx *c, mcr_scene *s, const double *rays, uint32_t n, double t_min,
+                          double t_max, int cull, int32_t *tri_out, double *t_out, double *u_out,
+                          double *v_out, uint8_t *front_out, int device_ptrs, mcr_counters *counters) {
+  if (!c || !s || (!rays && n)) return fail(MCR_E_INVALID, "null argument");
+  if (!s->has_geometry) return fail(MCR_E_NO_SCENE, "the scene has no geometry");
+  if (n == 0) return MCR_OK;
+  std::lock_guard<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t sm = c->stream;
+  const double *drays = rays;
+  int32_t *dtri = tri_out;
+  double *dt = t_out, *du = u_out, *dv = v_out;
+  uint8_t *dfront = front_out;
+  void *tmp = nullptr;
+  if (!device_ptrs) {
+    size_t bytes = (size_t)n * (48 + 4 + 24 + 1) + 1024;
+    HIP_TRY(hipMalloc(&tmp, bytes));
+    char *b = (char *)tmp;
+    double *r = (double *)b;
+    b += (size_t)n * 48;
+    dt = t_out ? (double *)b : nullptr;
+    b += (size_t)n * 8;
+    du = u_out ? (double *)b : nullptr;
+    b += (size_t)n * 8;
+    dv = v_out ? (double *)b : nullptr;
+    b += (size_t)n * 8;
+    dtri = tri_out ? (int32_t *)b : nullptr;
+    b += (size_t)n * 4;
+    dfront = front_out ? (uint8_t *)b : nullptr;
+    HIP_TRY(hipMemcpyAsync(r, rays, (size_t)n * 48, hipMemcpyHostToDevice, sm));
+    drays = r;
+  }
+  HIP_TRY(hipMemsetAsync(c->counters, 0, sizeof(DevCounters), sm));
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], sm));
+  launch_probe_closest(sm, s->d, drays, n, t_min, t_max, cull, dtri, dt, du, dv, dfront,
+                       std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->spill, TRAVERSE_THREADS,
+                       c->counters);
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], sm));
+  HIP_TRY(hipGetLastError());
+  if (!device_ptrs) {
+    if (t_out) HIP_TRY(hipMemcpyAsync(t_out, dt, (size_t)n * 8, hipMemcpyDeviceToHost, sm));
+    if (u_out) HIP_TRY(hipMemcpyAsync(u_out, du, (size_t)n * 8, hipMemcpyDeviceToHost, sm));
+    if (v_out) HIP_TRY(hipMemcpyAsync(v_out, dv, (size_t)n * 8, hipMemcpyDeviceToHost, sm));
+    if (tri_out) HIP_TRY(hipMemcpyAsync(tri_out, dtri, (size_t)n * 4, hipMemcpyDeviceToHost, sm));
+    if (front_out) HIP_TRY(hipMemcpyAsync(front_out, dfront, n, hipMemcpyDeviceToHost, sm));
+  }
+  HIP_TRY(hipStreamSynchronize(sm));
+  if (c->timing) {
+    float a;
+    hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+    c->last_timing = mcr_timing{a, 0, 0, 0, a, 1, 0};
+  }
+  if (tmp) hipFree(tmp);
+  if (counters) {
+    DevCounters hc;
+    HIP_TRY(hipMemcpy(&hc, c->counters, sizeof(hc), hipMemcpyDeviceToHost));
+    counters->closest_queries += hc.closest_queries;
+    counters->node_visits += hc.node_visits;
+    counters->leaf_visits += hc.leaf_visits;
+    counters->triangle_tests += hc.triangle_tests;
+  }
+  return MCR_OK;
+}
+
+int mcr_intersect_any_count(mcr_ctx *c, mcr_scene *s, const double *rays, uint32_t n, double t_min,
+                            double t_max, uint32_t *count_out, int device_ptrs, mcr_counters *counters) {
+  if (!c || !s || (!rays && n) || (!count_out && n)) return fail(MCR_E_INVALID, "null argument");
+  if (!s->has_geometry) return fail(MCR_E_NO_SCENE, "the scene has no geometry");
+  if (n == 0) return MCR_OK;
+  std::lock_guard<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t sm = c->stream;
+  const double *drays = rays;
+  uint32_t *dcount = count_out;
+  void *tmp = nullptr;
+  if (!device_ptrs) {
+    HIP_TRY(hipMalloc(&tmp, (size_t)n * 52 + 256));
+    double *r = (double *)tmp;
+    dcount = (uint32_t *)((char *)tmp + (size_t)n * 48);
+    HIP_TRY(hipMemcpyAsync(r, rays, (size_t)n * 48, hipMemcpyHostToDevice, sm));
+    drays = r;
+  }
+  HIP_TRY(hipMemsetAsync(c->counters, 0, sizeof(DevCounters), sm));
+  launch_probe_any_count(sm, s->d, drays, n, t_min, t_max, dcount,
+                         std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->spill, TRAVERSE_THREADS,
+                         c->counters);
+  HIP_TRY(hipGetLastError());
+  if (!device_ptrs) HIP_TRY(hipMemcpyAsync(count_out, dcount, (size_t)n * 4, hipMemcpyDeviceToHost, sm));
+  HIP_TRY(hipStreamSynchronize(sm));
+  if (tmp) hipFree(tmp);
+  if (counters) {
+    DevCounters hc;
+    HIP_TRY(hipMemcpy(&hc, c->counters, sizeof(hc), hipMemcpyDeviceToHost));
+    counters->any_queries += hc.any_queries;
+    counters->node_visits += hc.node_visits;
+    counters->leaf_visits += hc.leaf_visits;
+    counters->triangle_tests += hc.triangle_tests;
+  }
+  return MCR_OK;
+}
+
+// ---- device buffers ---------------------------------------------------------------------------
+int mcr_device_alloc(mcr_ctx *c, uint64_t bytes, void **out) {
+  if (!c || !out) return fail(MCR_E_INVALID, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMalloc(out, bytes ? bytes : 1));
+  return MCR_OK;
+}
+
+int mcr_device_free(mcr_ctx *c, void *ptr) {
+  if (!c) return fail(MCR_E_INVALID, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(ptr));
+  return MCR_OK;
+}
+
+int mcr_copy(mcr_ctx *c, void *dst, const void *src, uint64_t bytes, int kind) {
+  if (!c || (!bytes)) return c ? MCR_OK : fail(MCR_E_INVALID, "null argument");
+  if (!dst || !src) return fail(MCR_E_INVALID, "null argument");
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCR_OK;
+}
+
+int mcr_memset(mcr_ctx *c, void *dst, int value, uint64_t bytes) {
+  if (!c || !dst) return fail(MCR_E_INVALID, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemsetAsync(dst, value, bytes, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCR_OK;
+}
+
+// ---- builder ----------------------------------------------------------------------------------
+int mcr_build_bvh_device(mcr_ctx *c, const float *d_tris, uint32_t n, float *d_bounds,
+                         int32_t *d_nodes, uint32_t *d_order, uint32_t *node_count_out) {
+  if (!c || !node_count_out) return fail(MCR_E_INVALID, "null argument");
+  if (n == 0) {
+    *node_count_out = 0;
+    return MCR_OK;
+  }
+  if (!d_tris || !d_bounds || !d_nodes || !d_order) return fail(MCR_E_INVALID, "null argument");
+  if (n >= (1u << 30)) return fail(MCR_E_OVERFLOW, "too many triangles");
+  std::lock_guard<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  std::string err;
+  int rc = builder_run(c->builder, c->stream, d_tris, n, d_bounds, d_nodes, d_order, node_count_out, err);
+  if (rc) return fail(rc, err);
+  return MCR_OK;
+}
+
+int mcr_build_bvh(mcr_ctx *c, const float *tris, uint32_t n, float *bounds_out, int32_t *nodes_out,
+                  uint32_t *order_out, uint32_t *node_count_out) {
+  if (!c || !node_count_out) return fail(MCR_E_INVALID, "null argument");
+  if (n == 0) {
+    *node_count_out = 0;
+    return MCR_OK;
+  }
+  if (!tris || !bounds_out || !nodes_out || !order_out) return fail(MCR_E_INVALID, "null argument");
+  if (n >= (1u << 30)) return fail(MCR_E_OVERFLOW, "too many triangles");
+  HIP_TRY(hipSetDevice(c->device));
+  size_t maxn = 2 * (size_t)n - 1;
+  float *dt = nullptr, *db = nullptr;
+  int32_t *dn = nullptr;
+  uint32_t *dord = nullptr;
+  HIP_TRY(hipMalloc(&dt, (size_t)n * 9 * sizeof(float)));
+  HIP_TRY(hipMalloc(&db, maxn * 6 * sizeof(float)));
+  HIP_TRY(hipMalloc(&dn, maxn * 3 * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&dord, (size_t)n * sizeof(uint32_t)));
+  HIP_TRY(hipMemcpy(dt, tris, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice));
+  uint32_t nc = 0;
+  int rc = mcr_build_bvh_device(c, dt, n, db, dn, dord, &nc);
+  if (rc == MCR_OK) {
+    HIP_TRY(hipMemcpy(bounds_out, db, (size_t)nc * 6 * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(nodes_out, dn, (size_t)nc * 3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(order_out, dord, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    *node_count_out = nc;
+  }
+  hipFree(dt);
+  hipFree(db);
+  hipFree(dn);
+  hipFree(dord);
+  return rc;
+}
+
+}  // extern "C"

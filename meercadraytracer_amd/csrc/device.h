@@ -1,0 +1,299 @@
+// device.h — HBM layout of a resident scene, and the traversal routines over it.
+//
+// Layout (all arrays in BVH leaf order, uploaded once per geometry version):
+//   tri      3 x float4 per triangle (48 B, one 16-byte-aligned record per lane access):
+//            {v0.x v0.y v0.z e1.x} {e1.y e1.z e2.x e2.y} {e2.z, side|flags<<8, material, 0}
+//   inner    one 64-byte record per inner node of the reference's binary BVH, holding BOTH
+//            children's boxes, so one record read serves the two slab tests an inner visit does
+//            (bvh.ts:259-260): {L.min L.max.x} {L.max.yz R.min.xy} {R.min.z R.max} {refL refR axis 0}
+//   A child ref is an inner-record index (>= 0) or a leaf (bit 31 set): bits 26..30 hold the count
+//   (1..31; 0 = a long leaf whose (first, count) sits in `longleaf`), bits 0..25 the first triangle.
+//   Node numbering, child order and the split axis are the reference's (bvh.ts:165-171), so the
+//   traversals below visit nodes in exactly the reference's order.
+#ifndef MCR_DEVICE_H
+#define MCR_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jsmath.h"
+
+#define MCR_LEAF_BIT 0x80000000u
+#define MCR_LEAF_FIRST_BITS 26
+#define MCR_LEAF_FIRST_MASK ((1u << MCR_LEAF_FIRST_BITS) - 1u)
+#define MCR_MAX_TRIANGLES (1u << MCR_LEAF_FIRST_BITS)
+#define MCR_STACK_LDS 24      // per-lane traversal stack entries kept in LDS
+#define MCR_STACK_MAX 128     // total per-lane stack (tree depth <= 100, bvh.ts:5)
+
+struct DevMaterial {
+  int model, side, transparent, vertex_colors;
+  int tone_mapped, physical, pad0, pad1;
+  double color[3];
+  double opacity, alpha_test;
+  double emissive[3];
+  double roughness, metalness, ior, specular_intensity;
+  double specular_color[3];
+  double transmission, thickness;
+  double attenuation[3];
+  double shininess;
+  double specular[3];
+};
+
+struct DevLight {
+  int type, cast_shadow, pad0, pad1;
+  double color[3], ground_color[3], direction[3], position[3];
+  double distance, decay, cone_cos, penumbra_cos;
+};
+
+struct DevScene {
+  const float4 *tri;        // 3 per triangle
+  const float4 *inner;      // 4 per inner record
+  const int2 *longleaf;     // (first, count) of leaves that do not pack into a ref
+  const float *normals;     // 9 per triangle
+  const float *colors;      // 12 per triangle or null
+  const DevMaterial *materials;
+  const DevLight *lights;
+  uint32_t triangle_count, inner_count, material_count, light_count;
+  int32_t root_ref;         // child ref of node 0, or -1 for an empty BVH
+  float root_box[6];
+  double epsilon;
+  // shading payload scalars
+  int background_kind, fog_kind;
+  double background_color[3];
+  double fog_color[3];
+  double fog_near, fog_far, fog_density;
+};
+
+struct DevCounters {  // accumulated per wave, flushed with one atomic per field
+  unsigned long long closest_queries, any_queries, node_visits, leaf_visits, triangle_tests, paths;
+};
+
+__device__ inline bool ref_is_leaf(int32_t ref) { return (uint32_t)ref & MCR_LEAF_BIT; }
+
+__device__ inline void leaf_range(const DevScene &s, int32_t ref, int &first, int &count) {
+  uint32_t r = (uint32_t)ref;
+  uint32_t c = (r >> MCR_LEAF_FIRST_BITS) & 31u;
+  if (c != 0) {
+    first = (int)(r & MCR_LEAF_FIRST_MASK);
+    count = (int)c;
+  } else {
+    int2 l = s.longleaf[r & MCR_LEAF_FIRST_MASK];
+    first = l.x;
+    count = l.y;
+  }
+}
+
+// bvh.ts:336-338
+__device__ inline double safe_inverse(double d) { return d != 0 ? 1.0 / d : 1e30; }
+
+// bvh.ts:340-356 on one child box given as 6 floats.
+__device__ inline double box_entry(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
+                                   double ox, double oy, double oz, double ix, double iy, double iz,
+                                   double tmin, double tmax) {
+  double t0 = ((double)bx0 - ox) * ix, t1 = ((double)bx1 - ox) * ix;
+  double lo = js_min(t0, t1), hi = js_max(t0, t1);
+  t0 = ((double)by0 - oy) * iy;
+  t1 = ((double)by1 - oy) * iy;
+  lo = js_max(lo, js_min(t0, t1));
+  hi = js_min(hi, js_max(t0, t1));
+  t0 = ((double)bz0 - oz) * iz;
+  t1 = ((double)bz1 - oz) * iz;
+  lo = js_max(lo, js_min(t0, t1));
+  hi = js_min(hi, js_max(t0, t1));
+  if (hi >= lo && hi >= tmin && lo <= tmax) return js_max(lo, tmin);
+  return JM_INF;
+}
+
+// Per-lane traversal stack: the first MCR_STACK_LDS entries in LDS (lane-interleaved, so a
+// wave's pushes hit distinct banks), the rest in a global spill area owned by this thread.
+struct LaneStack {
+  int32_t *lds;      // base for this lane: lds[i * stride]
+  int32_t *spill;    // base for this lane: spill[(i - LDS) * spill_stride]
+  int stride, spill_stride;
+  int sp;
+  __device__ void push(int32_t v) {
+    if (sp < MCR_STACK_LDS) lds[sp * stride] = v;
+    else spill[(size_t)(sp - MCR_STACK_LDS) * spill_stride] = v;
+    sp++;
+  }
+  __device__ int32_t pop() {
+    sp--;
+    if (sp < MCR_STACK_LDS) return lds[sp * stride];
+    return spill[(size_t)(sp - MCR_STACK_LDS) * spill_stride];
+  }
+};
+
+struct TriHit {
+  int tri;
+  double t, u, v;
+  bool front;
+};
+
+struct TravCount {
+  uint32_t nodes, leaves, tris;
+};
+
+// Möller–Trumbore in FP64 on one triangle record (bvh.ts:228-252). Returns false if missed;
+// otherwise t, u, v and det.
+__device__ inline bool mt_test(const float4 a, const float4 b, const float4 c, double ox, double oy,
+                               double oz, double dx, double dy, double dz, int cull_side,
+                               double &t, double &u, double &v, double &det_out) {
+  double e1x = a.w, e1y = b.x, e1z = b.y;
+  double e2x = b.z, e2y = b.w, e2z = c.x;
+  double px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
+  double det = e1x * px + e1y * py + e1z * pz;
+  if (det == 0) return false;
+  if (cull_side == 0 && det < 0) return false;
+  if (cull_side == 1 && det > 0) return false;
+  double inv = 1.0 / det;
+  double sx = ox - (double)a.x, sy = oy - (double)a.y, sz = oz - (double)a.z;
+  u = (sx * px + sy * py + sz * pz) * inv;
+  if (u < 0 || u > 1) return false;
+  double qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+  v = (dx * qx + dy * qy + dz * qz) * inv;
+  if (v < 0 || u + v > 1) return false;
+  t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  det_out = det;
+  return true;
+}
+
+// bvh.ts:202-282 over the device layout. cull: drop back faces of FrontSide and front faces of
+// BackSide triangles.
+__device__ inline bool intersect_closest(const DevScene &s, double ox, double oy, double oz,
+                                         double dx, double dy, double dz, double tmin, double tmax,
+                                         bool cull, TriHit &hit, LaneStack &st, TravCount &cnt) {
+  hit.tri = -1;
+  hit.t = tmax;
+  hit.u = 0;
+  hit.v = 0;
+  hit.front = true;
+  if (s.root_ref == -1) return false;
+  double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+  bool neg[3] = {ix < 0, iy < 0, iz < 0};
+  st.sp = 0;
+  int32_t ref = s.root_ref;
+  for (;;) {
+    if (ref_is_leaf(ref)) {
+      int first, count;
+      leaf_range(s, ref, first, count);
+      cnt.leaves++;
+      for (int k = first; k < first + count; k++) {
+        cnt.tris++;
+        const float4 *r = s.tri + (size_t)k * 3;
+        float4 a = r[0], b = r[1], c = r[2];
+        int side = cull ? (int)(__float_as_uint(c.y) & 0xffu) : 2;
+        double t, u, v, det;
+        if (!mt_test(a, b, c, ox, oy, oz, dx, dy, dz, side, t, u, v, det)) continue;
+        if (t > tmin && t < hit.t) {
+          hit.t = t;
+          hit.u = u;
+          hit.v = v;
+          hit.tri = k;
+          hit.front = det > 0;
+        }
+      }
+    } else {
+      cnt.nodes++;
+      const float4 *r = s.inner + (size_t)ref * 4;
+      float4 p = r[0], q = r[1], w = r[2];
+      int4 m = *reinterpret_cast<const int4 *>(r + 3);
+      bool right_first = neg[m.z];
+      double tl = box_entry(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, hit.t);
+      double tr = box_entry(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, hit.t);
+      int32_t near_ref = right_first ? m.y : m.x, far_ref = right_first ? m.x : m.y;
+      double tn = right_first ? tr : tl, tf = right_first ? tl : tr;
+      if (tn < JM_INF) {
+        if (tf < JM_INF) {
+          if (tf < tn) {
+            st.push(near_ref);
+            ref = far_ref;
+          } else {
+            st.push(far_ref);
+            ref = near_ref;
+          }
+        } else {
+          ref = near_ref;
+        }
+        continue;
+      }
+      if (tf < JM_INF) {
+        ref = far_ref;
+        continue;
+      }
+    }
+    if (st.sp == 0) break;
+    ref = st.pop();
+  }
+  return hit.tri >= 0;
+}
+
+// bvh.ts:289-332 over the device layout: every hit in (tmin, tmax), no culling, in the
+// reference's depth-first order (right child first). Both child boxes are tested when their parent
+// is visited; a box test does not depend on when it runs, so the visit order is unchanged.
+// visit(k, t, u, v, front) returns false to stop.
+template <class Visit>
+__device__ inline void intersect_any(const DevScene &s, double ox, double oy, double oz, double dx,
+                                     double dy, double dz, double tmin, double tmax, LaneStack &st,
+                                     TravCount &cnt, Visit &visit) {
+  if (s.root_ref == -1) return;
+  double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+  const float *rb = s.root_box;
+  if (!(box_entry(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF))
+    return;
+  st.sp = 0;
+  int32_t ref = s.root_ref;
+  for (;;) {
+    if (ref_is_leaf(ref)) {
+      int first, count;
+      leaf_range(s, ref, first, count);
+      cnt.leaves++;
+      for (int k = first; k < first + count; k++) {
+        cnt.tris++;
+        const float4 *r = s.tri + (size_t)k * 3;
+        float4 a = r[0], b = r[1], c = r[2];
+        double t, u, v, det;
+        if (!mt_test(a, b, c, ox, oy, oz, dx, dy, dz, 2, t, u, v, det)) continue;
+        if (t > tmin && t < tmax && !visit(k, t, u, v, det > 0)) return;
+      }
+    } else {
+      cnt.nodes++;
+      const float4 *r = s.inner + (size_t)ref * 4;
+      float4 p = r[0], q = r[1], w = r[2];
+      int4 m = *reinterpret_cast<const int4 *>(r + 3);
+      bool hl = box_entry(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
+      bool hr = box_entry(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
+      if (hr) {
+        if (hl) st.push(m.x);
+        ref = m.y;
+        continue;
+      }
+      if (hl) {
+        ref = m.x;
+        continue;
+      }
+    }
+    if (st.sp == 0) break;
+    ref = st.pop();
+  }
+}
+
+// Sum over the 64 lanes of a wave.
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline void flush_counts(DevCounters *c, const TravCount &t, unsigned long long closest,
+                                    unsigned long long any) {
+  unsigned long long n = wave_sum(t.nodes), l = wave_sum(t.leaves), k = wave_sum(t.tris);
+  unsigned long long q = wave_sum(closest), a = wave_sum(any);
+  if ((threadIdx.x & 63) == 0) {
+    if (n) atomicAdd(&c->node_visits, n);
+    if (l) atomicAdd(&c->leaf_visits, l);
+    if (k) atomicAdd(&c->triangle_tests, k);
+    if (q) atomicAdd(&c->closest_queries, q);
+    if (a) atomicAdd(&c->any_queries, a);
+  }
+}
+
+#endif  // MCR_DEVICE_H

@@ -1,0 +1,931 @@
+// trace.hip — wavefront path-tracing kernels for gfx950: ray generation, extend (closest hit),
+// shade, shadow (any hit) and resolve, plus the ray-query and tile-encode kernels.
+//
+// The kernels restate src/tracer.ts (samplePixel, lightAt, visibleLight, transmittance,
+// attenuateShadow, cosineDirection, reflection, refract, fog, background), src/bvh.ts
+// (traversals) and src/progressive.ts:288-313 (writeTile) in FP64 with -ffp-contract=off, so
+// each sample reproduces the reference's arithmetic; the order in which light terms are summed
+// is kept by deferring the sum to the resolve kernel, after the shadow rays have run.
+#include <hip/hip_runtime.h>
+
+#include "../../include/mcr.h"
+#include "device.h"
+#include "kernels.h"
+#include "shading.h"
+#include "wavefront.h"
+
+__constant__ float g_dfg[512];
+
+// ---- helpers ----------------------------------------------------------------------------------
+// Exclusive prefix of v over the active lanes of the wave, plus the wave total.
+__device__ inline uint32_t wave_exclusive(uint32_t v, uint32_t &total) {
+  uint32_t lane = threadIdx.x & 63;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+// Appends `want` (0/1/...) entries for this lane to a queue whose length is *counter, with one
+// atomic per wave. Every lane of the wave must call it.
+__device__ inline uint32_t wave_append(uint32_t *counter, uint32_t want) {
+  uint32_t total;
+  uint32_t pre = wave_exclusive(want, total);
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 63 && total) base = atomicAdd(counter, total);
+  base = __shfl(base, 63, 64);
+  return base + pre;
+}
+
+__device__ inline int find_job(const JobDev *jobs, uint32_t njobs, uint32_t s) {
+  int lo = 0, hi = (int)njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].sample_start <= s) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// tracer.ts:715-727
+__device__ inline void unproject(const mcr_camera &c, double x, double y, double z, double *out) {
+  const double *p = c.projection_inverse;
+  double w = p[3] * x + p[7] * y + p[11] * z + p[15];
+  double vx = (p[0] * x + p[4] * y + p[8] * z + p[12]) / w;
+  double vy = (p[1] * x + p[5] * y + p[9] * z + p[13]) / w;
+  double vz = (p[2] * x + p[6] * y + p[10] * z + p[14]) / w;
+  const double *m = c.world;
+  out[0] = m[0] * vx + m[4] * vy + m[8] * vz + m[12];
+  out[1] = m[1] * vx + m[5] * vy + m[9] * vz + m[13];
+  out[2] = m[2] * vx + m[6] * vy + m[10] * vz + m[14];
+}
+
+// ---- ray generation: tracer.ts:62-80 + 705-713 --------------------------------------------------
+__global__ __launch_bounds__(256) void k_raygen(const JobDev *jobs, uint32_t njobs, uint32_t s0,
+                                                uint32_t count, Wavefront w, mcr_camera cam,
+                                                uint32_t seed) {
+  uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= count) return;
+  uint32_t s = s0 + slot;
+  const JobDev &j = jobs[find_job(jobs, njobs, s)];
+  uint32_t li = s - j.sample_start;
+  uint32_t x = (li % j.cols) * j.block, y = (li / j.cols) * j.block;
+  int px = j.tx + (int)x, py = j.ty + (int)y;
+  uint32_t rng = rng_seed((uint32_t)px, (uint32_t)py, j.pass, seed);
+  double jx = 0.5, jy = 0.5;
+  if (j.block > 1) {
+    jx = js_min((double)j.block, (double)j.width - px) * 0.5;
+    jy = js_min((double)j.block, (double)j.height - py) * 0.5;
+  } else if (j.pass > 0) {
+    jx = rng_next(rng);
+    jy = rng_next(rng);
+  }
+  double sx = (px + jx) / j.width, sy = (py + jy) / j.height;
+  double nx = sx * 2 - 1, ny = 1 - sy * 2;
+  double a[3], b[3];
+  unproject(cam, nx, ny, -1, a);
+  unproject(cam, nx, ny, 1, b);
+  double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+  double len = or_one(js_hypot3(dx, dy, dz));
+  dx /= len;
+  dy /= len;
+  dz /= len;
+  size_t P = w.capacity;
+  double *f = w.f;
+  f[F_OX * P + slot] = a[0];
+  f[F_OY * P + slot] = a[1];
+  f[F_OZ * P + slot] = a[2];
+  f[F_DX * P + slot] = dx;
+  f[F_DY * P + slot] = dy;
+  f[F_DZ * P + slot] = dz;
+  f[F_TR * P + slot] = 1;
+  f[F_TG * P + slot] = 1;
+  f[F_TB * P + slot] = 1;
+  f[F_LR * P + slot] = 0;
+  f[F_LG * P + slot] = 0;
+  f[F_LB * P + slot] = 0;
+  f[F_MR * P + slot] = 0;
+  f[F_MG * P + slot] = 0;
+  f[F_MB * P + slot] = 0;
+  f[F_FOG * P + slot] = 0;
+  int32_t *iv = w.i;
+  iv[I_RNG * P + slot] = (int32_t)rng;
+  iv[I_FLAGS * P + slot] = PF_TONEMAP | PF_CULL;
+  iv[I_BOUNCE * P + slot] = 0;
+  iv[I_PASSES * P + slot] = 0;
+  iv[I_SAMPLE * P + slot] = (int32_t)s;
+  w.queue[0][slot] = (int32_t)slot;
+}
+
+// ---- extend: closest hit for every live slot (bvh.ts:202-282) ------------------------------------
+__global__ __launch_bounds__(256) void k_extend(DevScene sc, Wavefront w, int q, int32_t *spill,
+                                                uint32_t spill_stride, DevCounters *cnt) {
+  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+  uint32_t n = w.counts[q];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nthreads = gridDim.x * blockDim.x;
+  LaneStack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.stride = 256;
+  st.spill = spill + gid;
+  st.spill_stride = (int)spill_stride;
+  TravCount tc = {0, 0, 0};
+  unsigned long long queries = 0;
+  size_t P = w.capacity;
+  const double *f = w.f;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    int slot = w.queue[q][i];
+    double ox = f[F_OX * P + slot], oy = f[F_OY * P + slot], oz = f[F_OZ * P + slot];
+    double dx = f[F_DX * P + slot], dy = f[F_DY * P + slot], dz = f[F_DZ * P + slot];
+    int flags = w.i[I_FLAGS * P + slot];
+    TriHit h;
+    intersect_closest(sc, ox, oy, oz, dx, dy, dz, 0, JM_INF, (flags & PF_CULL) != 0, h, st, tc);
+    queries++;
+    w.i[I_HTRI * P + slot] = h.tri;
+    w.f[F_HT * P + slot] = h.t;
+    w.f[F_HU * P + slot] = h.u;
+    w.f[F_HV * P + slot] = h.v;
+    w.i[I_FLAGS * P + slot] = h.front ? (flags | PF_FRONT) : (flags & ~PF_FRONT);
+  }
+  flush_counts(cnt, tc, queries, 0);
+}
+
+// Writes one finished sample (RGBA, doubles rounded to float as Float32Array.set does) over its
+// block of the tile (tracer.ts:81-85).
+__device__ inline void write_sample(const JobDev *jobs, uint32_t njobs, float *out, uint32_t s,
+                                    const double *c) {
+  const JobDev &j = jobs[find_job(jobs, njobs, s)];
+  uint32_t li = s - j.sample_start;
+  uint32_t x = (li % j.cols) * j.block, y = (li / j.cols) * j.block;
+  uint32_t ymax = min(y + j.block, (uint32_t)j.th), xmax = min(x + j.block, (uint32_t)j.tw);
+  float4 v = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]);
+  for (uint32_t by = y; by < ymax; by++)
+    for (uint32_t bx = x; bx < xmax; bx++)
+      *reinterpret_cast<float4 *>(out + j.out_offset + ((size_t)by * j.tw + bx) * 4) = v;
+}
+
+struct LightSample {
+  double lx, ly, lz, lr, lg, lb, dist;
+};
+
+// tracer.ts:486-512
+__device__ inline bool light_at(const DevLight &l, double px, double py, double pz, LightSample &o) {
+  if (l.type == MCR_LIGHT_DIRECTIONAL) {
+    o.lx = l.direction[0];
+    o.ly = l.direction[1];
+    o.lz = l.direction[2];
+    o.lr = l.color[0];
+    o.lg = l.color[1];
+    o.lb = l.color[2];
+    o.dist = JM_INF;
+    return true;
+  }
+  if (l.type == MCR_LIGHT_POINT || l.type == MCR_LIGHT_SPOT) {
+    double x = l.position[0] - px, y = l.position[1] - py, z = l.position[2] - pz;
+    double distance = js_hypot3(x, y, z);
+    if (distance == 0) return false;
+    o.lx = x / distance;
+    o.ly = y / distance;
+    o.lz = z / distance;
+    double scale = distance_attenuation(distance, l.distance, l.decay);
+    if (l.type == MCR_LIGHT_SPOT) {
+      double ac = o.lx * l.direction[0] + o.ly * l.direction[1] + o.lz * l.direction[2];
+      scale *= smoothstep_js(l.cone_cos, l.penumbra_cos, ac);
+    }
+    if (scale <= 0) return false;
+    o.lr = l.color[0] * scale;
+    o.lg = l.color[1] * scale;
+    o.lb = l.color[2] * scale;
+    o.dist = distance;
+    return true;
+  }
+  return false;
+}
+
+// ---- shade: one path vertex of samplePixel (tracer.ts:113-469) ----------------------------------
+// Writes the vertex's unshadowed light terms and shadow rays into the light slots, the values
+// resolve needs to finish `lr += tr * (outR + er)`, and the continuation ray (the continuation
+// uses only the random stream and the material, never the shadow results, so it runs here).
+__global__ __launch_bounds__(256) void k_shade(DevScene sc, Wavefront w, int q, mcr_settings st,
+                                               mcr_camera cam, const JobDev *jobs, uint32_t njobs,
+                                               float *out) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = w.counts[q];
+  // Every lane reaches the wave_append at the end, so the bound check guards the work only.
+  bool live = gid < n;
+  size_t P = w.capacity;
+  double *f = w.f;
+  int32_t *iv = w.i;
+  int slot = live ? w.queue[q][gid] : 0;
+  uint32_t nshadow = 0;
+  const uint32_t nl = min(sc.light_count, (uint32_t)MCR_MAX_LIGHTS);
+  if (live) {
+    double eps = sc.epsilon;
+    int flags = iv[I_FLAGS * P + slot];
+    uint32_t rng = (uint32_t)iv[I_RNG * P + slot];
+    int bounce = iv[I_BOUNCE * P + slot];
+    int passes = iv[I_PASSES * P + slot];
+    double ox = f[F_OX * P + slot], oy = f[F_OY * P + slot], oz = f[F_OZ * P + slot];
+    double dx = f[F_DX * P + slot], dy = f[F_DY * P + slot], dz = f[F_DZ * P + slot];
+    double tr = f[F_TR * P + slot], tg = f[F_TG * P + slot], tb = f[F_TB * P + slot];
+    int k = iv[I_HTRI * P + slot];
+    int mode;
+    for (uint32_t l = 0; l < nl; l++) w.status[(size_t)l * P + slot] = LS_NONE;
+    if (k < 0) {
+      if (!(flags & PF_SURFACE)) {
+        double c[4];  // background (tracer.ts:659-691)
+        if (sc.background_kind == MCR_BACKGROUND_COLOR) {
+          c[0] = sc.background_color[0];
+          c[1] = sc.background_color[1];
+          c[2] = sc.background_color[2];
+          c[3] = 1;
+        } else {
+          double a = st.clear_alpha;
+          c[0] = st.clear_color[0] * a;
+          c[1] = st.clear_color[1] * a;
+          c[2] = st.clear_color[2] * a;
+          c[3] = a;
+        }
+        write_sample(jobs, njobs, out, (uint32_t)iv[I_SAMPLE * P + slot], c);
+        mode = MODE_DONE;
+      } else {
+        mode = MODE_FINISH;  // no environment to add (tracer.ts:119-125)
+      }
+    } else {
+      double t = f[F_HT * P + slot];
+      double mr = f[F_MR * P + slot], mg = f[F_MG * P + slot], mb = f[F_MB * P + slot];
+      if (mr > 0 || mg > 0 || mb > 0) {  // Beer-Lambert (tracer.ts:128-131)
+        tr *= jm_exp(-mr * t);
+        tg *= jm_exp(-mg * t);
+        tb *= jm_exp(-mb * t);
+      }
+      bool front = (flags & PF_FRONT) != 0;
+      const float4 *rec = sc.tri + (size_t)k * 3;
+      float4 ra = rec[0], rb = rec[1], rc = rec[2];
+      int tflags = (int)((__float_as_uint(rc.y) >> 8) & 0xffu);
+      const DevMaterial &mat = sc.materials[__float_as_uint(rc.z)];
+      double px = ox + dx * t, py = oy + dy * t, pz = oz + dz * t;
+      double w1 = f[F_HU * P + slot], w2 = f[F_HV * P + slot], w0 = 1 - w1 - w2;
+      double e1x = ra.w, e1y = rb.x, e1z = rb.y, e2x = rb.z, e2y = rb.w, e2z = rc.x;
+      double gx = e1y * e2z - e1z * e2y;
+      double gy = e1z * e2x - e1x * e2z;
+      double gz = e1x * e2y - e1y * e2x;
+      double gl = or_one(js_hypot3(gx, gy, gz));
+      double face = front ? 1 : -1;
+      gx = (gx / gl) * face;
+      gy = (gy / gl) * face;
+      gz = (gz / gl) * face;
+      double cr = mat.color[0], cg = mat.color[1], cb = mat.color[2];
+      double alpha = mat.opacity;
+      if (mat.vertex_colors && sc.colors) {
+        const float *col = sc.colors + (size_t)k * 12;
+        cr *= col[0] * w0 + col[4] * w1 + col[8] * w2;
+        cg *= col[1] * w0 + col[5] * w1 + col[9] * w2;
+        cb *= col[2] * w0 + col[6] * w1 + col[10] * w2;
+        alpha *= col[3] * w0 + col[7] * w1 + col[11] * w2;
+      }
+      bool cutOut = mat.alpha_test > 0 && alpha < mat.alpha_test;
+      if (cutOut || (mat.transparent && rng_next(rng) >= alpha)) {
+        // pass straight through (tracer.ts:170-176)
+        if (++passes > 64) {
+          mode = MODE_FINISH;
+        } else {
+          double side = gx * dx + gy * dy + gz * dz > 0 ? eps : -eps;
+          f[F_OX * P + slot] = px + gx * side;
+          f[F_OY * P + slot] = py + gy * side;
+          f[F_OZ * P + slot] = pz + gz * side;
+          mode = MODE_EXTEND;
+        }
+      } else {
+        if (!(flags & PF_SURFACE)) {
+          flags |= PF_SURFACE;
+          flags = mat.tone_mapped ? (flags | PF_TONEMAP) : (flags & ~PF_TONEMAP);
+          double fogf = 0;
+          if (sc.fog_kind != MCR_FOG_NONE) {  // tracer.ts:649-656
+            const double *fw = cam.forward, *cp = cam.position;
+            double depth = (px - cp[0]) * fw[0] + (py - cp[1]) * fw[1] + (pz - cp[2]) * fw[2];
+            if (sc.fog_kind == MCR_FOG_LINEAR) fogf = smoothstep_js(sc.fog_near, sc.fog_far, depth);
+            else fogf = 1 - jm_exp(-sc.fog_density * sc.fog_density * depth * depth);
+          }
+          f[F_FOG * P + slot] = fogf;
+        }
+        const float *nrm = sc.normals + (size_t)k * 9;
+        double nx = (nrm[0] * w0 + nrm[3] * w1 + nrm[6] * w2) * face;
+        double ny = (nrm[1] * w0 + nrm[4] * w1 + nrm[7] * w2) * face;
+        double nz = (nrm[2] * w0 + nrm[5] * w1 + nrm[8] * w2) * face;
+        double nlen = or_one(js_hypot3(nx, ny, nz));
+        nx /= nlen;
+        ny /= nlen;
+        nz /= nlen;
+        double vx = -dx, vy = -dy, vz = -dz;
+        double dotNV = js_min(js_max(nx * vx + ny * vy + nz * vz, 0), 1);
+        double aR = 0, aG = 0, aB = 0;  // out* before the light sum (basic: the colour)
+        double iR = 0, iG = 0, iB = 0;  // indirect term added after it
+        bool has_ind = false;
+        bool receive = (tflags & MCR_TRI_RECEIVE_SHADOW) != 0;
+        bool canContinue = bounce < st.max_bounces;
+        double specR = 0, specG = 0, specB = 0, diffR = 0, diffG = 0, diffB = 0;
+        double transR = 0, transG = 0, transB = 0;
+        double roughness = 1;
+        bool castOnly = st.shadows == MCR_SHADOWS_CAST;
+        if (mat.model == MCR_MODEL_BASIC) {
+          aR = cr;
+          aG = cg;
+          aB = cb;
+        } else {
+          double ir = 0, ig = 0, ib = 0;
+          for (uint32_t li = 0; li < sc.light_count; li++) {
+            const DevLight &L = sc.lights[li];
+            if (L.type == MCR_LIGHT_AMBIENT) {
+              ir += L.color[0];
+              ig += L.color[1];
+              ib += L.color[2];
+            } else if (L.type == MCR_LIGHT_HEMISPHERE) {
+              double hw = 0.5 * (nx * L.direction[0] + ny * L.direction[1] + nz * L.direction[2]) + 0.5;
+              ir += L.ground_color[0] + (L.color[0] - L.ground_color[0]) * hw;
+              ig += L.ground_color[1] + (L.color[1] - L.ground_color[1]) * hw;
+              ib += L.ground_color[2] + (L.color[2] - L.ground_color[2]) * hw;
+            }
+          }
+          const double ao = 1;
+          bool physicalModel = mat.model == MCR_MODEL_PHYSICAL;
+          double metalness = 0, sr = 0.04, sg = 0.04, sb = 0.04, f90 = 1, br = 0, bg = 0, bbl = 0;
+          double drr = 0, dgg = 0, dbb = 0, fa = 0, fb = 0, ems = 0, compR = 0, compG = 0, compB = 0;
+          double ggxAlpha = 0, keepDiffuse = 1;
+          if (physicalModel) {
+            metalness = mat.metalness;
+            roughness = mat.roughness;
+            double shadeRoughness = js_min(js_max(roughness, 0.045), 1);
+            if (mat.physical) {
+              double fr = jm_pow((mat.ior - 1) / (mat.ior + 1), 2);
+              sr = js_min(fr * mat.specular_color[0], 1) * mat.specular_intensity;
+              sg = js_min(fr * mat.specular_color[1], 1) * mat.specular_intensity;
+              sb = js_min(fr * mat.specular_color[2], 1) * mat.specular_intensity;
+              f90 = mat.specular_intensity + (1 - mat.specular_intensity) * metalness;
+            }
+            br = sr + (cr - sr) * metalness;
+            bg = sg + (cg - sg) * metalness;
+            bbl = sb + (cb - sb) * metalness;
+            drr = cr * (1 - metalness);
+            dgg = cg * (1 - metalness);
+            dbb = cb * (1 - metalness);
+            dfg(g_dfg, shadeRoughness, dotNV, fa, fb);
+            double essMs = fa + fb;
+            ems = 1 - essMs;
+            compR = 1 + br * (1 / essMs - 1);
+            compG = 1 + bg * (1 / essMs - 1);
+            compB = 1 + bbl * (1 / essMs - 1);
+            ggxAlpha = shadeRoughness * shadeRoughness;
+            keepDiffuse = 1 - mat.transmission;
+          }
+          // Direct lights (tracer.ts:286-312 physical, 357-382 lambert / phong / toon).
+          for (uint32_t li = 0; li < nl; li++) {
+            const DevLight &L = sc.lights[li];
+            LightSample ls;
+            if (!light_at(L, px, py, pz, ls)) continue;
+            double r, g, b;
+            if (physicalModel) {
+              double dotNL = nx * ls.lx + ny * ls.ly + nz * ls.lz;
+              if (dotNL <= 0) continue;
+              double hx = ls.lx + vx, hy = ls.ly + vy, hz = ls.lz + vz;
+              double hl = or_one(js_hypot3(hx, hy, hz));
+              hx /= hl;
+              hy /= hl;
+              hz /= hl;
+              double dotNH = js_min(js_max(nx * hx + ny * hy + nz * hz, 0), 1);
+              double dotVH = js_min(js_max(vx * hx + vy * hy + vz * hz, 0), 1);
+              double fw = fresnel_weight(dotVH);
+              double vd = visibility_ggx(ggxAlpha, dotNL, dotNV) * distribution_ggx(ggxAlpha, dotNH);
+              double specularR = (br * (1 - fw) + f90 * fw) * vd * compR;
+              double specularG = (bg * (1 - fw) + f90 * fw) * vd * compG;
+              double specularB = (bbl * (1 - fw) + f90 * fw) * vd * compB;
+              double diffuseR = drr * MCR_INV_PI * (1 - (sr * (1 - fw) + f90 * fw)) * keepDiffuse;
+              double diffuseG = dgg * MCR_INV_PI * (1 - (sg * (1 - fw) + f90 * fw)) * keepDiffuse;
+              double diffuseB = dbb * MCR_INV_PI * (1 - (sb * (1 - fw) + f90 * fw)) * keepDiffuse;
+              double scale = dotNL;
+              r = (specularR + diffuseR) * scale * ls.lr;
+              g = (specularG + diffuseG) * scale * ls.lg;
+              b = (specularB + diffuseB) * scale * ls.lb;
+            } else {
+              double cosine = nx * ls.lx + ny * ls.ly + nz * ls.lz;
+              double irradiance;
+              if (mat.model == MCR_MODEL_TOON) irradiance = cosine * 0.5 + 0.5 < 0.7 ? 0.7 : 1;
+              else if (cosine <= 0) continue;
+              else irradiance = cosine;
+              r = cr * MCR_INV_PI;
+              g = cg * MCR_INV_PI;
+              b = cb * MCR_INV_PI;
+              if (mat.model == MCR_MODEL_PHONG && cosine > 0) {
+                double hx = ls.lx + vx, hy = ls.ly + vy, hz = ls.lz + vz;
+                double hl = or_one(js_hypot3(hx, hy, hz));
+                hx /= hl;
+                hy /= hl;
+                hz /= hl;
+                double dotNH = js_min(js_max(nx * hx + ny * hy + nz * hz, 0), 1);
+                double dotVH = js_min(js_max(vx * hx + vy * hy + vz * hz, 0), 1);
+                double fw = fresnel_weight(dotVH);
+                double d = blinn_phong(mat.shininess, dotNH);
+                r += (mat.specular[0] * (1 - fw) + fw) * d;
+                g += (mat.specular[1] * (1 - fw) + fw) * d;
+                b += (mat.specular[2] * (1 - fw) + fw) * d;
+              }
+              r *= irradiance * ls.lr;
+              g *= irradiance * ls.lg;
+              b *= irradiance * ls.lb;
+            }
+            if (r + g + b <= 0) continue;
+            size_t ls_idx = (size_t)li * P + slot;
+            const size_t RS = (size_t)MCR_MAX_LIGHTS * P;  // one row of light slots
+            w.contrib[0 * RS + ls_idx] = r;
+            w.contrib[1 * RS + ls_idx] = g;
+            w.contrib[2 * RS + ls_idx] = b;
+            // visibleLight (tracer.ts:515-526); 'castShadow' in light holds for these three types.
+            if (st.shadows == MCR_SHADOWS_NONE || (castOnly && (!L.cast_shadow || !receive))) {
+              w.shadow[0 * RS + ls_idx] = 1;
+              w.shadow[1 * RS + ls_idx] = 1;
+              w.shadow[2 * RS + ls_idx] = 1;
+              w.status[ls_idx] = LS_VISIBLE;
+            } else {
+              double side = gx * ls.lx + gy * ls.ly + gz * ls.lz > 0 ? eps : -eps;
+              w.sray[0 * RS + ls_idx] = px + gx * side;
+              w.sray[1 * RS + ls_idx] = py + gy * side;
+              w.sray[2 * RS + ls_idx] = pz + gz * side;
+              w.sray[3 * RS + ls_idx] = ls.lx;
+              w.sray[4 * RS + ls_idx] = ls.ly;
+              w.sray[5 * RS + ls_idx] = ls.lz;
+              w.sray[6 * RS + ls_idx] = ls.dist == JM_INF ? JM_INF : ls.dist - 2 * eps;
+              w.status[ls_idx] = LS_RAY;
+              nshadow++;
+            }
+          }
+          if (physicalModel) {
+            double fssR = sr * fa + f90 * fb, fssG = sg * fa + f90 * fb, fssB = sb * fa + f90 * fb;
+            double scatterR = fssR + multi_scatter(fssR, sr, ems);
+            double scatterG = fssG + multi_scatter(fssG, sg, ems);
+            double scatterB = fssB + multi_scatter(fssB, sb, ems);
+            iR = ir * MCR_INV_PI * drr * (1 - scatterR) * ao * keepDiffuse;
+            iG = ig * MCR_INV_PI * dgg * (1 - scatterG) * ao * keepDiffuse;
+            iB = ib * MCR_INV_PI * dbb * (1 - scatterB) * ao * keepDiffuse;
+            double mfR = cr * fa + f90 * fb, mfG = cg * fa + f90 * fb, mfB = cb * fa + f90 * fb;
+            specR = fssR + (mfR - fssR) * metalness;
+            specG = fssG + (mfG - fssG) * metalness;
+            specB = fssB + (mfB - fssB) * metalness;
+            diffR = drr * (1 - scatterR) * keepDiffuse;
+            diffG = dgg * (1 - scatterG) * keepDiffuse;
+            diffB = dbb * (1 - scatterB) * keepDiffuse;
+            double transmission = mat.transmission;
+            if (transmission > 0) {
+              double fr = sr * fa + f90 * fb, fg = sg * fa + f90 * fb, fbl = sb * fa + f90 * fb;
+              transR = transmission * (1 - fr) * cr;
+              transG = transmission * (1 - fg) * cg;
+              transB = transmission * (1 - fbl) * cb;
+            }
+            if (!st.global_illumination) diffR = diffG = diffB = 0;
+          } else {
+            iR = ir * MCR_INV_PI * cr * ao;
+            iG = ig * MCR_INV_PI * cg * ao;
+            iB = ib * MCR_INV_PI * cb * ao;
+            if (st.global_illumination) {
+              diffR = cr;
+              diffG = cg;
+              diffB = cb;
+            }
+          }
+          has_ind = true;
+        }
+        f[F_QR * P + slot] = tr;
+        f[F_QG * P + slot] = tg;
+        f[F_QB * P + slot] = tb;
+        f[F_AR * P + slot] = aR;
+        f[F_AG * P + slot] = aG;
+        f[F_AB * P + slot] = aB;
+        f[F_IR * P + slot] = iR;
+        f[F_IG * P + slot] = iG;
+        f[F_IB * P + slot] = iB;
+        f[F_ER * P + slot] = mat.emissive[0];
+        f[F_EG * P + slot] = mat.emissive[1];
+        f[F_EB * P + slot] = mat.emissive[2];
+        flags = has_ind ? (flags | PF_INDIRECT) : (flags & ~PF_INDIRECT);
+        // Continuation (tracer.ts:412-469).
+        mode = MODE_ADD_FINISH;
+        if (canContinue) {
+          double ws = luminance(specR, specG, specB);
+          double wt = luminance(transR, transG, transB);
+          double wd = luminance(diffR, diffG, diffB);
+          double total = ws + wt + wd;
+          if (total > 1e-6) {
+            double pick = rng_next(rng) * total;
+            double wr, wg, wb, p, sx = 0, sy = 0, sz = 0;
+            bool stop = false;
+            bool cull = (flags & PF_CULL) != 0;
+            if (pick < ws) {
+              p = ws / total;
+              wr = specR;
+              wg = specG;
+              wb = specB;
+              double hx = nx, hy = ny, hz = nz;  // reflection (tracer.ts:608-621)
+              if (roughness > 0.02) {
+                double a = roughness * roughness;
+                double r1 = rng_next(rng), r2 = rng_next(rng);
+                double phi = 2 * MCR_PI * r1;
+                double cosT = jm_sqrt((1 - r2) / (1 + (a * a - 1) * r2));
+                double sinT = jm_sqrt(js_max(0, 1 - cosT * cosT));
+                to_world(nx, ny, nz, sinT * jm_cos(phi), sinT * jm_sin(phi), cosT, hx, hy, hz);
+              }
+              double dn = dx * hx + dy * hy + dz * hz;
+              sx = dx - 2 * dn * hx;
+              sy = dy - 2 * dn * hy;
+              sz = dz - 2 * dn * hz;
+              if (sx * gx + sy * gy + sz * gz <= 0) stop = true;
+            } else if (pick < ws + wt) {
+              p = wt / total;
+              wr = transR;
+              wg = transG;
+              wb = transB;
+              if (mat.thickness > 0) {
+                double eta = front ? 1 / mat.ior : mat.ior;
+                double cosi = -(dx * nx + dy * ny + dz * nz);  // refract (tracer.ts:624-633)
+                double kk = 1 - eta * eta * (1 - cosi * cosi);
+                if (!(kk < 0)) {
+                  double s = eta * cosi - jm_sqrt(kk);
+                  sx = eta * dx + s * nx;
+                  sy = eta * dy + s * ny;
+                  sz = eta * dz + s * nz;
+                  double length = or_one(js_hypot3(sx, sy, sz));
+                  sx /= length;
+                  sy /= length;
+                  sz /= length;
+                  cull = !front;
+                  f[F_MR * P + slot] = front ? mat.attenuation[0] : 0;
+                  f[F_MG * P + slot] = front ? mat.attenuation[1] : 0;
+                  f[F_MB * P + slot] = front ? mat.attenuation[2] : 0;
+                } else {
+                  double dn = dx * nx + dy * ny + dz * nz;  // total internal reflection
+                  sx = dx - 2 * dn * nx;
+                  sy = dy - 2 * dn * ny;
+                  sz = dz - 2 * dn * nz;
+                }
+              } else {
+                sx = dx;
+                sy = dy;
+                sz = dz;
+              }
+            } else {
+              p = wd / total;
+              wr = diffR;
+              wg = diffG;
+              wb = diffB;
+              double r1 = rng_next(rng), r2 = rng_next(rng);  // cosineDirection (tracer.ts:600-605)
+              double phi = 2 * MCR_PI * r1;
+              double radius = jm_sqrt(r2);
+              to_world(nx, ny, nz, radius * jm_cos(phi), radius * jm_sin(phi), jm_sqrt(js_max(0, 1 - r2)),
+                       sx, sy, sz);
+              if (sx * gx + sy * gy + sz * gz <= 0) stop = true;
+            }
+            if (!stop) {
+              tr *= wr / p;
+              tg *= wg / p;
+              tb *= wb / p;
+              double side = sx * gx + sy * gy + sz * gz > 0 ? eps : -eps;
+              f[F_OX * P + slot] = px + gx * side;
+              f[F_OY * P + slot] = py + gy * side;
+              f[F_OZ * P + slot] = pz + gz * side;
+              f[F_DX * P + slot] = sx;
+              f[F_DY * P + slot] = sy;
+              f[F_DZ * P + slot] = sz;
+              bounce++;
+              bool alive = true;
+              if (bounce >= 3) {  // Russian roulette
+                double qq = js_min(0.95, js_max3(tr, tg, tb));
+                if (rng_next(rng) >= qq) {
+                  alive = false;
+                } else {
+                  tr /= qq;
+                  tg /= qq;
+                  tb /= qq;
+                }
+              }
+              flags = cull ? (flags | PF_CULL) : (flags & ~PF_CULL);
+              if (alive) mode = MODE_ADD_EXTEND;
+            }
+          }
+        }
+      }
+    }
+    f[F_TR * P + slot] = tr;
+    f[F_TG * P + slot] = tg;
+    f[F_TB * P + slot] = tb;
+    iv[I_FLAGS * P + slot] = flags;
+    iv[I_RNG * P + slot] = (int32_t)rng;
+    iv[I_BOUNCE * P + slot] = bounce;
+    iv[I_PASSES * P + slot] = passes;
+    w.mode[slot] = (uint8_t)mode;
+  }
+  // Enqueue this lane's shadow rays (one atomic per wave).
+  uint32_t e = wave_append(&w.counts[2], nshadow);
+  if (live && nshadow) {
+    for (uint32_t li = 0; li < nl; li++)
+      if (w.status[(size_t)li * P + slot] == LS_RAY) w.sref[e++] = slot * MCR_MAX_LIGHTS + (int32_t)li;
+  }
+}
+
+// ---- shadow: transmittance + attenuateShadow for every queued shadow ray (tracer.ts:529-594) ----
+struct ShadowVisitor {
+  const DevScene *sc;
+  double col[3], depth[3];
+  bool castOnly;
+  double *seen;  // this thread's list, stride `stride`
+  size_t stride;
+  int nseen;
+  bool overflow;
+  __device__ bool operator()(int tri, double t, double u, double v, bool front) {
+    (void)u;
+    (void)v;
+    float4 c = sc->tri[(size_t)tri * 3 + 2];
+    int flags = (int)((__float_as_uint(c.y) >> 8) & 0xffu);
+    if (castOnly && (flags & MCR_TRI_CAST_SHADOW) == 0) return true;
+    if (flags & MCR_TRI_OPAQUE) {
+      col[0] = col[1] = col[2] = 0;
+      return false;
+    }
+    const DevMaterial &m = sc->materials[__float_as_uint(c.z)];
+    double alpha = m.opacity;
+    double cr = m.color[0], cg = m.color[1], cb = m.color[2];
+    if (m.alpha_test > 0 && alpha < m.alpha_test) return true;
+    if (!m.transparent) alpha = 1;
+    double eps = sc->epsilon;
+    for (int j = 0; j < nseen; j++)
+      if (jm_fabs(seen[(size_t)j * stride] - t) <= eps) return true;
+    if (nseen < MCR_SHADOW_HIT_CAP) seen[(size_t)(nseen++) * stride] = t;
+    else overflow = true;
+    if (m.thickness > 0) {
+      double sgn = front ? -t : t;
+      depth[0] = depth[0] + m.attenuation[0] * sgn;
+      depth[1] = depth[1] + m.attenuation[1] * sgn;
+      depth[2] = depth[2] + m.attenuation[2] * sgn;
+    }
+    double open = 1 - alpha, through = m.transmission;
+    col[0] = col[0] * (open + alpha * through * cr);
+    col[1] = col[1] * (open + alpha * through * cg);
+    col[2] = col[2] * (open + alpha * through * cb);
+    if (col[0] + col[1] + col[2] < 1e-4) {
+      col[0] = col[1] = col[2] = 0;
+      return false;
+    }
+    return true;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int cast_only,
+                                                int32_t *spill, uint32_t spill_stride,
+                                                double *seen, DevCounters *cnt) {
+  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+  uint32_t n = w.counts[2];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nthreads = gridDim.x * blockDim.x;
+  LaneStack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.stride = 256;
+  st.spill = spill + gid;
+  st.spill_stride = (int)spill_stride;
+  TravCount tc = {0, 0, 0};
+  unsigned long long queries = 0;
+  size_t P = w.capacity;
+  const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
+  bool overflow = false;
+  for (uint32_t e = gid; e < n; e += nthreads) {
+    int32_t ref = w.sref[e];
+    int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
+    size_t idx = (size_t)li * P + slot;
+    double ox = w.sray[0 * RS + idx], oy = w.sray[1 * RS + idx], oz = w.sray[2 * RS + idx];
+    double dx = w.sray[3 * RS + idx], dy = w.sray[4 * RS + idx], dz = w.sray[5 * RS + idx];
+    double maxT = w.sray[6 * RS + idx];
+    ShadowVisitor vis;
+    vis.sc = &sc;
+    vis.col[0] = vis.col[1] = vis.col[2] = 1;
+    vis.depth[0] = vis.depth[1] = vis.depth[2] = 0;
+    vis.castOnly = cast_only != 0;
+    vis.seen = seen + gid;
+    vis.stride = nthreads;
+    vis.nseen = 0;
+    vis.overflow = false;
+    intersect_any(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, vis);
+    queries++;
+    overflow |= vis.overflow;
+    for (int c = 0; c < 3; c++)
+      if (vis.depth[c] > 0) vis.col[c] = vis.col[c] * jm_exp(-vis.depth[c]);
+    bool visible = vis.col[0] + vis.col[1] + vis.col[2] > 0;
+    w.shadow[0 * RS + idx] = vis.col[0];
+    w.shadow[1 * RS + idx] = vis.col[1];
+    w.shadow[2 * RS + idx] = vis.col[2];
+    w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
+  }
+  if (overflow) atomicOr(&w.counts[3], 1u);
+  flush_counts(cnt, tc, 0, queries);
+}
+
+// ---- resolve: finish the vertex's radiance in the reference's summation order ---------------------
+__global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_settings st, DevScene sc,
+                                                 const JobDev *jobs, uint32_t njobs, float *out) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = w.counts[q];
+  bool live = gid < n;
+  size_t P = w.capacity;
+  const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
+  double *f = w.f;
+  int slot = live ? w.queue[q][gid] : 0;
+  int mode = live ? w.mode[slot] : MODE_DONE;
+  if (mode == MODE_ADD_EXTEND || mode == MODE_ADD_FINISH) {
+    double oR = f[F_AR * P + slot], oG = f[F_AG * P + slot], oB = f[F_AB * P + slot];
+    uint32_t nl = min(sc.light_count, (uint32_t)MCR_MAX_LIGHTS);
+    for (uint32_t li = 0; li < nl; li++) {
+      size_t idx = (size_t)li * P + slot;
+      if (w.status[idx] != LS_VISIBLE) continue;
+      oR += w.contrib[0 * RS + idx] * w.shadow[0 * RS + idx];
+      oG += w.contrib[1 * RS + idx] * w.shadow[1 * RS + idx];
+      oB += w.contrib[2 * RS + idx] * w.shadow[2 * RS + idx];
+    }
+    if (w.i[I_FLAGS * P + slot] & PF_INDIRECT) {
+      oR += f[F_IR * P + slot];
+      oG += f[F_IG * P + slot];
+      oB += f[F_IB * P + slot];
+    }
+    f[F_LR * P + slot] += f[F_QR * P + slot] * (oR + f[F_ER * P + slot]);
+    f[F_LG * P + slot] += f[F_QG * P + slot] * (oG + f[F_EG * P + slot]);
+    f[F_LB * P + slot] += f[F_QB * P + slot] * (oB + f[F_EB * P + slot]);
+  }
+  bool requeue = mode == MODE_EXTEND || mode == MODE_ADD_EXTEND;
+  uint32_t pos = wave_append(&w.counts[q ^ 1], requeue ? 1u : 0u);
+  if (requeue) w.queue[q ^ 1][pos] = slot;
+  if (mode == MODE_ADD_FINISH || mode == MODE_FINISH) {
+    // tracer.ts:472-477: tone map, then fog, alpha 1.
+    double c[4] = {f[F_LR * P + slot], f[F_LG * P + slot], f[F_LB * P + slot], 1};
+    int flags = w.i[I_FLAGS * P + slot];
+    if ((flags & PF_TONEMAP) && st.tone_mapping != MCR_NO_TONE_MAPPING)
+      tone_map(c, st.tone_mapping, st.tone_mapping_exposure);
+    double fogf = f[F_FOG * P + slot];
+    if (fogf > 0 && sc.fog_kind != MCR_FOG_NONE)
+      for (int k = 0; k < 3; k++) c[k] = c[k] + (sc.fog_color[k] - c[k]) * fogf;
+    write_sample(jobs, njobs, out, (uint32_t)w.i[I_SAMPLE * P + slot], c);
+  }
+}
+
+// ---- ray queries over caller rays (bvh.ts:202-332) ---------------------------------------------
+__global__ __launch_bounds__(256) void k_probe_closest(DevScene sc, const double *rays, uint32_t n,
+                                                       double tmin, double tmax, int cull,
+                                                       int32_t *tri, double *t, double *u, double *v,
+                                                       uint8_t *front, int32_t *spill,
+                                                       uint32_t spill_stride, DevCounters *cnt) {
+  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nthreads = gridDim.x * blockDim.x;
+  LaneStack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.stride = 256;
+  st.spill = spill + gid;
+  st.spill_stride = (int)spill_stride;
+  TravCount tc = {0, 0, 0};
+  unsigned long long queries = 0;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    const double *r = rays + (size_t)i * 6;
+    TriHit h;
+    intersect_closest(sc, r[0], r[1], r[2], r[3], r[4], r[5], tmin, tmax, cull != 0, h, st, tc);
+    queries++;
+    if (tri) tri[i] = h.tri;
+    if (t) t[i] = h.t;
+    if (u) u[i] = h.u;
+    if (v) v[i] = h.v;
+    if (front) front[i] = h.front ? 1 : 0;
+  }
+  flush_counts(cnt, tc, queries, 0);
+}
+
+struct CountVisitor {
+  uint32_t n;
+  __device__ bool operator()(int, double, double, double, bool) {
+    n++;
+    return true;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_probe_any_count(DevScene sc, const double *rays, uint32_t n,
+                                                         double tmin, double tmax, uint32_t *count,
+                                                         int32_t *spill, uint32_t spill_stride,
+                                                         DevCounters *cnt) {
+  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nthreads = gridDim.x * blockDim.x;
+  LaneStack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.stride = 256;
+  st.spill = spill + gid;
+  st.spill_stride = (int)spill_stride;
+  TravCount tc = {0, 0, 0};
+  unsigned long long queries = 0;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    const double *r = rays + (size_t)i * 6;
+    CountVisitor cv{0};
+    intersect_any(sc, r[0], r[1], r[2], r[3], r[4], r[5], tmin, tmax, st, tc, cv);
+    queries++;
+    count[i] = cv.n;
+  }
+  flush_counts(cnt, tc, 0, queries);
+}
+
+// ---- writeTile (progressive.ts:288-313) --------------------------------------------------------
+// ECMAScript ToUint8Clamp: round half to even, clamp to [0, 255], NaN -> 0.
+__device__ inline uint8_t to_uint8_clamp(double v) {
+  if (!(v > 0)) return 0;
+  if (v >= 255) return 255;
+  double f = jm_floor(v);
+  if (f + 0.5 < v) return (uint8_t)(f + 1);
+  if (v < f + 0.5) return (uint8_t)f;
+  int fi = (int)f;
+  return (uint8_t)((fi & 1) ? fi + 1 : fi);
+}
+
+__global__ __launch_bounds__(256) void k_encode(uint8_t *image, uint32_t width, mcr_tile tile,
+                                                const float *data, float *sum, uint32_t count,
+                                                int srgb) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t npx = (uint32_t)tile.width * (uint32_t)tile.height;
+  if (i >= npx) return;
+  uint32_t x = i % tile.width, y = i / tile.width;
+  size_t s = (size_t)i * 4;
+  size_t d = (((size_t)tile.y + y) * width + tile.x + x) * 4;
+  double c[4] = {data[s], data[s + 1], data[s + 2], data[s + 3]};
+  if (sum) {
+    double scale = 1.0 / count;
+    for (int k = 0; k < 4; k++) {
+      double v = (double)sum[d + k] + c[k];  // the array keeps the float, the frame the double
+      sum[d + k] = (float)v;
+      c[k] = v;
+    }
+    for (int k = 0; k < 4; k++) c[k] *= scale;
+  }
+  if (c[3] > 0) {
+    c[0] /= c[3];
+    c[1] /= c[3];
+    c[2] /= c[3];
+  }
+  if (srgb)
+    for (int k = 0; k < 3; k++) c[k] = linear_to_srgb(js_max(0, c[k]));
+  for (int k = 0; k < 4; k++) image[d + k] = to_uint8_clamp(c[k] * 255 + 0.5);
+}
+
+// ---- launch wrappers ----------------------------------------------------------------------------
+static inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+hipError_t mcr_upload_dfg(const float *lut512) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_dfg), lut512, 512 * sizeof(float));
+}
+
+void launch_raygen(hipStream_t s, const JobDev *jobs, uint32_t njobs, uint32_t s0, uint32_t count,
+                   const Wavefront &w, const mcr_camera &cam, uint32_t seed) {
+  hipLaunchKernelGGL(k_raygen, dim3(blocks_for(count)), dim3(256), 0, s, jobs, njobs, s0, count, w,
+                     cam, seed);
+}
+void launch_extend(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t grid,
+                   int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
+  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, s, sc, w, q, spill, spill_stride, cnt);
+}
+void launch_shade(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t max_live,
+                  const mcr_settings &st, const mcr_camera &cam, const JobDev *jobs, uint32_t njobs,
+                  float *out) {
+  hipLaunchKernelGGL(k_shade, dim3(blocks_for(max_live)), dim3(256), 0, s, sc, w, q, st, cam, jobs,
+                     njobs, out);
+}
+void launch_shadow(hipStream_t s, const DevScene &sc, const Wavefront &w, int cast_only,
+                   uint32_t grid, int32_t *spill, uint32_t spill_stride, double *seen,
+                   DevCounters *cnt) {
+  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, s, sc, w, cast_only, spill, spill_stride,
+                     seen, cnt);
+}
+void launch_resolve(hipStream_t s, const Wavefront &w, int q, uint32_t max_live,
+                    const mcr_settings &st, const DevScene &sc, const JobDev *jobs, uint32_t njobs,
+                    float *out) {
+  hipLaunchKernelGGL(k_resolve, dim3(blocks_for(max_live)), dim3(256), 0, s, w, q, st, sc, jobs,
+                     njobs, out);
+}
+void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
+                          double tmin, double tmax, int cull, int32_t *tri, double *t, double *u,
+                          double *v, uint8_t *front, uint32_t grid, int32_t *spill,
+                          uint32_t spill_stride, DevCounters *cnt) {
+  hipLaunchKernelGGL(k_probe_closest, dim3(grid), dim3(256), 0, s, sc, rays, n, tmin, tmax, cull,
+                     tri, t, u, v, front, spill, spill_stride, cnt);
+}
+void launch_probe_any_count(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
+                            double tmin, double tmax, uint32_t *count, uint32_t grid,
+                            int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
+  hipLaunchKernelGGL(k_probe_any_count, dim3(grid), dim3(256), 0, s, sc, rays, n, tmin, tmax,
+                     count, spill, spill_stride, cnt);
+}
+void launch_encode(hipStream_t s, uint8_t *image, uint32_t width, const mcr_tile &tile,
+                   const float *data, float *sum, uint32_t count, int srgb) {
+  uint64_t npx = (uint64_t)tile.width * (uint64_t)tile.height;
+  hipLaunchKernelGGL(k_encode, dim3(blocks_for(npx)), dim3(256), 0, s, image, width, tile, data, sum,
+                     count, srgb);
+}

@@ -1,0 +1,254 @@
+"""ctypes binding of the native library libmcr.so (include/mcr.h).
+
+This is the product path: every entry point runs on the GPU through the C-ABI. There is no CPU
+fallback: importing works anywhere, but the first call raises NativeError when the library is not
+built or no GPU is visible.
+"""
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libmcr.so')
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f'{message} (code {code})')
+        self.code = code
+        self.message = message
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(abi.MCR_E_DEVICE, f'{LIB_PATH} is not built: run __graft_entry__.build()')
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            'mcr_create': (C.c_int, [C.c_int, C.POINTER(P)]),
+            'mcr_destroy': (None, [P]),
+            'mcr_last_error': (C.c_char_p, []),
+            'mcr_version': (C.c_char_p, []),
+            'mcr_build_bvh': (C.c_int, [P, P, C.c_uint32, P, P, P, C.POINTER(C.c_uint32)]),
+            'mcr_build_bvh_device': (C.c_int, [P, P, C.c_uint32, P, P, P, C.POINTER(C.c_uint32)]),
+            'mcr_scene_create': (C.c_int, [P, C.POINTER(P)]),
+            'mcr_scene_set_geometry': (C.c_int, [P, P, C.c_uint32]),
+            'mcr_scene_set_shading': (C.c_int, [P, P, C.c_uint32]),
+            'mcr_scene_release': (None, [P]),
+            'mcr_scene_bytes': (C.c_uint64, [P]),
+            'mcr_render_tile': (C.c_int, [P, P, C.c_uint32, C.c_uint32, P, P, abi.Tile, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_uint32, P, P]),
+            'mcr_render_tiles': (C.c_int, [P, P, P, P, P, C.c_uint32, P, C.c_int, P]),
+            'mcr_encode_tile': (C.c_int, [P, P, C.c_uint32, abi.Tile, P, P, C.c_uint32, C.c_int, C.c_int]),
+            'mcr_intersect_closest': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P, P,
+                                                P, C.c_int, P]),
+            'mcr_intersect_any_count': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, P, C.c_int, P]),
+            'mcr_device_alloc': (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
+            'mcr_device_free': (C.c_int, [P, P]),
+            'mcr_copy': (C.c_int, [P, P, P, C.c_uint64, C.c_int]),
+            'mcr_memset': (C.c_int, [P, P, C.c_int, C.c_uint64]),
+            'mcr_last_timing': (C.c_int, [P, C.POINTER(abi.Timing)]),
+            'mcr_set_timing': (C.c_int, [P, C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+        return L
+
+
+def check(rc):
+    if rc != abi.MCR_OK:
+        msg = lib().mcr_last_error().decode(errors='replace')
+        raise NativeError(rc, msg)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class DeviceBuffer:
+    """Device memory owned through the C-ABI (no torch needed)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        ptr = C.c_void_p()
+        check(lib().mcr_device_alloc(ctx.handle, self.nbytes, C.byref(ptr)))
+        self.ptr = ptr.value
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        assert a.nbytes <= self.nbytes
+        check(lib().mcr_copy(self.ctx.handle, self.ptr, a.ctypes.data, a.nbytes, 0))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        check(lib().mcr_copy(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes, 1))
+        return out
+
+    def zero(self):
+        check(lib().mcr_memset(self.ctx.handle, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().mcr_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU (mcr_create)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().mcr_create(device, C.byref(h)))
+        self.handle = h.value
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().mcr_destroy(self.handle)
+            self.handle = None
+
+    def set_timing(self, on=True):
+        check(lib().mcr_set_timing(self.handle, int(on)))
+
+    def last_timing(self):
+        t = abi.Timing()
+        check(lib().mcr_last_timing(self.handle, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def build_bvh(self, triangles, count):
+        """bvh.ts:16 buildBvh on the GPU -> (bounds, nodes, order, nodeCount)."""
+        tri = np.ascontiguousarray(triangles, dtype=np.float32)
+        m = max(1, 2 * count - 1)
+        bounds = np.zeros(m * 6, np.float32)
+        nodes = np.zeros(m * 3, np.int32)
+        order = np.zeros(max(count, 1), np.uint32)
+        nc = C.c_uint32(0)
+        check(lib().mcr_build_bvh(self.handle, _p(tri), count, _p(bounds), _p(nodes), _p(order), C.byref(nc)))
+        n = nc.value
+        return bounds[:n * 6].copy(), nodes[:n * 3].copy(), order[:count].copy(), n
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def build_bvh(triangles, count):
+    return default_context().build_bvh(triangles, count)
+
+
+class Scene:
+    """A scene resident in HBM (mcr_scene): geometry and shading uploaded once per version."""
+
+    def __init__(self, ctx=None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        check(lib().mcr_scene_create(self.ctx.handle, C.byref(h)))
+        self.handle = h.value
+        self.geometry_version = None
+        self.shading_version = None
+
+    def set_geometry(self, geometry, version):
+        g = geometry.to_c()
+        check(lib().mcr_scene_set_geometry(self.handle, C.byref(g), version))
+        self.geometry_version = version
+
+    def set_shading(self, shading, version):
+        s = shading.to_c()
+        check(lib().mcr_scene_set_shading(self.handle, C.byref(s), version))
+        self.shading_version = version
+
+    def bytes(self):
+        return int(lib().mcr_scene_bytes(self.handle))
+
+    def release(self):
+        if self.handle:
+            lib().mcr_scene_release(self.handle)
+            self.handle = None
+
+    def render_tile(self, camera, settings, tile, width, height, pass_, block=1, counters=None):
+        """tracer.ts:62 renderTile on the GPU -> float32 premultiplied RGBA of the tile."""
+        out = np.zeros(tile.width * tile.height * 4, np.float32)
+        cam = camera.to_c()
+        st = settings.to_c()
+        cnt = counters if counters is not None else abi.Counters()
+        check(lib().mcr_render_tile(self.ctx.handle, self.handle, self.geometry_version, self.shading_version,
+                                    C.byref(cam), C.byref(st), tile.to_c(), width, height, pass_, block,
+                                    _p(out), C.byref(cnt)))
+        return out
+
+    def render_jobs(self, camera, settings, jobs, out=None, out_device=False, counters=None):
+        """Batch of mcr_tile_job (abi.TileJob) in one device run. Returns the host output array
+        (or None when `out` is a device pointer)."""
+        cam = camera.to_c()
+        st = settings.to_c()
+        arr = (abi.TileJob * max(1, len(jobs)))(*jobs)
+        cnt = counters if counters is not None else abi.Counters()
+        if out is None:
+            total = max((j.out_offset + j.tile.width * j.tile.height * 4 for j in jobs), default=0)
+            out = np.zeros(total, np.float32)
+        ptr = out if out_device else _p(out)
+        check(lib().mcr_render_tiles(self.ctx.handle, self.handle, C.byref(cam), C.byref(st),
+                                     C.cast(arr, C.c_void_p), len(jobs), ptr, int(out_device), C.byref(cnt)))
+        return None if out_device else out
+
+    def intersect_closest(self, rays, t_min=0.0, t_max=np.inf, cull=True, counters=None):
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+        n = len(rays)
+        tri = np.empty(n, np.int32)
+        t = np.empty(n)
+        u = np.empty(n)
+        v = np.empty(n)
+        front = np.empty(n, np.uint8)
+        cnt = counters if counters is not None else abi.Counters()
+        check(lib().mcr_intersect_closest(self.ctx.handle, self.handle, _p(rays), n, t_min, t_max, int(cull),
+                                          _p(tri), _p(t), _p(u), _p(v), _p(front), 0, C.byref(cnt)))
+        return tri, t, u, v, front
+
+    def intersect_any_count(self, rays, t_min=0.0, t_max=np.inf, counters=None):
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 6)
+        count = np.empty(len(rays), np.uint32)
+        cnt = counters if counters is not None else abi.Counters()
+        check(lib().mcr_intersect_any_count(self.ctx.handle, self.handle, _p(rays), len(rays), t_min, t_max,
+                                            _p(count), 0, C.byref(cnt)))
+        return count
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def encode_tile_device(ctx, image_ptr, width, tile, data_ptr, sum_ptr, count, srgb):
+    """progressive.ts:288 writeTile on device buffers."""
+    check(lib().mcr_encode_tile(ctx.handle, image_ptr, width, tile.to_c(), data_ptr, sum_ptr, count, int(srgb), 1))

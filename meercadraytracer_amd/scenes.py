@@ -1,0 +1,126 @@
+"""Seeded scene generators for the benchmark and parity workloads.
+
+  plain_cpu_scene(seed)   8 icospheres (4-way subdivision 3: 1,280 triangles each, centres
+                          U[-1,1]^3, radii U[0.2,0.4]) and a 2-triangle 20x20 ground plane at
+                          y = -1.2; MeshStandardMaterial with albedo U[0.2,0.9]^3, roughness
+                          U[0.1,0.9], metalness Bernoulli(0.3) in {0,1}; one directional and one
+                          point light; perspective camera, fov 50, at (0,1,4) looking at the origin.
+                          10,242 triangles. Rendered at 256x256, 16 spp, 4 bounces, GI off, ACES.
+  plain_gpu_scene(seed)   A 724x724-quad heightfield (1,048,352 triangles) over 40x40 with fBm
+                          displacement of amplitude 0.1, plus 64 icospheres (subdivision 4: 5,120
+                          triangles each) at seeded positions above it, 1,376,032 triangles in all;
+                          the same material distributions with 20% glass (MeshPhysicalMaterial,
+                          transmission 1, ior 1.5, thickness = diameter); a sun and 4 point lights;
+                          perspective camera, fov 50. Rendered at 1920x1080, 64 spp, GI on, 8 bounces.
+Every value is drawn from numpy's PCG64 with the given seed, so a seed names the scene.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+from . import three as T
+from .compile import compile_camera, compile_scene
+
+
+def _standard_material(rng):
+    albedo = rng.uniform(0.2, 0.9, 3)
+    return T.MeshStandardMaterial(color=T.Color(*albedo), roughness=float(rng.uniform(0.1, 0.9)),
+                                  metalness=float(rng.random() < 0.3))
+
+
+def _glass_material(rng, diameter):
+    albedo = rng.uniform(0.2, 0.9, 3)
+    return T.MeshPhysicalMaterial(color=T.Color(*albedo), roughness=float(rng.uniform(0.1, 0.9)),
+                                  metalness=0.0, transmission=1.0, ior=1.5, thickness=float(diameter))
+
+
+def plain_cpu_scene(seed=0, width=256, height=256):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scene = T.Scene()
+    for _ in range(8):
+        centre = rng.uniform(-1, 1, 3)
+        radius = float(rng.uniform(0.2, 0.4))
+        m = T.Mesh(T.IcosphereGeometry(radius, 3), _standard_material(rng))
+        m.position = [float(c) for c in centre]
+        scene.add(m)
+    ground = T.Mesh(T.PlaneGeometry(20, 20), _standard_material(rng))
+    ground.rotation = [-math.pi / 2, 0.0, 0.0]
+    ground.position = [0.0, -1.2, 0.0]
+    scene.add(ground)
+    sun = T.DirectionalLight(0xffffff, float(rng.uniform(1.5, 3.0)))
+    sun.position = [float(v) for v in rng.uniform([-5, 4, -5], [5, 8, 5])]
+    point = T.PointLight(0xffffff, float(rng.uniform(5, 15)), 0, 2)
+    point.position = [float(v) for v in rng.uniform([-2, 1.5, -2], [2, 3, 2])]
+    scene.add(sun, point)
+    camera = T.PerspectiveCamera(50, width / height, 0.1, 100)
+    camera.position = [0.0, 1.0, 4.0]
+    camera.lookAt(0, 0, 0)
+    camera.updateMatrixWorld()
+    options = dict(samples=16, maxBounces=4, globalIllumination=False,
+                   toneMapping=abi.ACESFilmicToneMapping, outputColorSpace='srgb', seed=0)
+    return scene, camera, options
+
+
+def fbm_heights(rng, rows, cols, amplitude=0.1, octaves=5, lacunarity=2.0, gain=0.5, base_cells=4):
+    """Value-noise fBm on a rows x cols grid: `octaves` layers of bilinearly interpolated random
+    lattices, each `lacunarity` times finer and `gain` times weaker; peak amplitude `amplitude`."""
+    h = np.zeros((rows, cols))
+    total = 0.0
+    amp = 1.0
+    cells = base_cells
+    ys = np.linspace(0, 1, rows)
+    xs = np.linspace(0, 1, cols)
+    for _ in range(octaves):
+        lattice = rng.uniform(-1, 1, (cells + 1, cells + 1))
+        gy, gx = ys * cells, xs * cells
+        iy = np.minimum(gy.astype(int), cells - 1)
+        ix = np.minimum(gx.astype(int), cells - 1)
+        fy = (gy - iy)[:, None]
+        fx = (gx - ix)[None, :]
+        a = lattice[iy][:, ix]
+        b = lattice[iy][:, ix + 1]
+        c = lattice[iy + 1][:, ix]
+        d = lattice[iy + 1][:, ix + 1]
+        h += amp * ((a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy)
+        total += amp
+        amp *= gain
+        cells = int(cells * lacunarity)
+    return h * (amplitude / total)
+
+
+def plain_gpu_scene(seed=0, width=1920, height=1080, quads=724, spheres=64, subdivisions=4):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scene = T.Scene()
+    size = 40.0
+    heights = fbm_heights(rng, quads + 1, quads + 1)
+    terrain = T.Mesh(T.HeightfieldGeometry(heights, size, size), _standard_material(rng))
+    scene.add(terrain)
+    for _ in range(spheres):
+        radius = float(rng.uniform(0.25, 0.6))
+        pos = [float(rng.uniform(-8, 8)), float(rng.uniform(0.2, 2.5)) + radius, float(rng.uniform(-8, 4))]
+        mat = _glass_material(rng, 2 * radius) if rng.random() < 0.2 else _standard_material(rng)
+        m = T.Mesh(T.IcosphereGeometry(radius, subdivisions), mat)
+        m.position = pos
+        scene.add(m)
+    sun = T.DirectionalLight(0xfff4e0, 3.0)
+    sun.position = [4.0, 10.0, 3.0]
+    scene.add(sun)
+    for _ in range(4):
+        p = T.PointLight(0xffffff, float(rng.uniform(5, 20)), 0, 2)
+        p.position = [float(rng.uniform(-8, 8)), float(rng.uniform(2, 5)), float(rng.uniform(-8, 4))]
+        scene.add(p)
+    camera = T.PerspectiveCamera(50, width / height, 0.1, 200)
+    camera.position = [0.0, 3.5, 13.0]
+    camera.lookAt(0, 0.6, -2)
+    camera.updateMatrixWorld()
+    options = dict(samples=64, maxBounces=8, globalIllumination=True,
+                   toneMapping=abi.ACESFilmicToneMapping, outputColorSpace='srgb', seed=0)
+    return scene, camera, options
+
+
+def compiled(kind, seed=0, build_bvh=None, **kw):
+    """(CompiledScene, CompiledCamera, options) of a named workload."""
+    maker = {'plain_cpu': plain_cpu_scene, 'plain_gpu': plain_gpu_scene}[kind]
+    scene, camera, options = maker(seed, **kw)
+    return compile_scene(scene, build_bvh), compile_camera(camera), options
