@@ -160,6 +160,8 @@ typedef struct mcr_counters {
   uint64_t leaf_visits;
   uint64_t triangle_tests;   /* triangle records read */
   uint64_t paths;            /* camera samples traced */
+  uint64_t closest_node_visits, closest_triangle_tests;  /* split by query kind, for the */
+  uint64_t any_node_visits, any_triangle_tests;          /* per-kernel algorithmic bytes   */
 } mcr_counters;
 
 typedef struct mcr_ctx mcr_ctx;

@@ -113,6 +113,8 @@ class Counters(C.Structure):
         ('closest_queries', C.c_uint64), ('any_queries', C.c_uint64),
         ('node_visits', C.c_uint64), ('leaf_visits', C.c_uint64),
         ('triangle_tests', C.c_uint64), ('paths', C.c_uint64),
+        ('closest_node_visits', C.c_uint64), ('closest_triangle_tests', C.c_uint64),
+        ('any_node_visits', C.c_uint64), ('any_triangle_tests', C.c_uint64),
     ]
 
     def as_dict(self):

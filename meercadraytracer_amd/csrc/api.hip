@@ -582,6 +582,10 @@ static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const
     counters->leaf_visits += hc.leaf_visits;
     counters->triangle_tests += hc.triangle_tests;
     counters->paths += total;
+    counters->closest_node_visits += hc.closest_node_visits;
+    counters->closest_triangle_tests += hc.closest_triangle_tests;
+    counters->any_node_visits += hc.any_node_visits;
+    counters->any_triangle_tests += hc.any_triangle_tests;
   }
   return MCR_OK;
 }

@@ -66,6 +66,7 @@ struct DevScene {
 
 struct DevCounters {  // accumulated per wave, flushed with one atomic per field
   unsigned long long closest_queries, any_queries, node_visits, leaf_visits, triangle_tests, paths;
+  unsigned long long closest_node_visits, closest_triangle_tests, any_node_visits, any_triangle_tests;
 };
 
 __device__ inline bool ref_is_leaf(int32_t ref) { return (uint32_t)ref & MCR_LEAF_BIT; }
@@ -293,6 +294,14 @@ __device__ inline void flush_counts(DevCounters *c, const TravCount &t, unsigned
     if (k) atomicAdd(&c->triangle_tests, k);
     if (q) atomicAdd(&c->closest_queries, q);
     if (a) atomicAdd(&c->any_queries, a);
+    // A kernel runs one kind of query, so its node and triangle counts belong to that kind.
+    if (q) {
+      atomicAdd(&c->closest_node_visits, n);
+      atomicAdd(&c->closest_triangle_tests, k);
+    } else if (a) {
+      atomicAdd(&c->any_node_visits, n);
+      atomicAdd(&c->any_triangle_tests, k);
+    }
   }
 }
 

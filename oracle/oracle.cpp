@@ -146,7 +146,7 @@ bool intersectClosest(const float *triangles, const uint8_t *sides, const Bvh &b
       if (cnt) cnt->leaf_visits++;
       int first = nodes[n3];
       for (int k = first; k < first + count; k++) {
-        if (cnt) cnt->triangle_tests++;
+        if (cnt) { cnt->triangle_tests++; cnt->closest_triangle_tests++; }
         const float *tr = triangles + (size_t)k * 9;
         double e1x = tr[3], e1y = tr[4], e1z = tr[5];
         double e2x = tr[6], e2y = tr[7], e2z = tr[8];
@@ -175,7 +175,7 @@ bool intersectClosest(const float *triangles, const uint8_t *sides, const Bvh &b
         }
       }
     } else {
-      if (cnt) cnt->node_visits++;
+      if (cnt) { cnt->node_visits++; cnt->closest_node_visits++; }
       int left = nodes[n3];
       int near = neg[nodes[n3 + 2]] ? left + 1 : left;
       int far = near == left ? left + 1 : left;
@@ -224,7 +224,7 @@ void intersectAny(const float *triangles, const Bvh &bvh, double ox, double oy, 
     int n3 = node * 3;
     int count = nodes[n3 + 1];
     if (count == 0) {
-      if (cnt) cnt->node_visits++;
+      if (cnt) { cnt->node_visits++; cnt->any_node_visits++; }
       stack[sp++] = nodes[n3];
       stack[sp++] = nodes[n3] + 1;
       continue;
@@ -232,7 +232,7 @@ void intersectAny(const float *triangles, const Bvh &bvh, double ox, double oy, 
     if (cnt) cnt->leaf_visits++;
     int first = nodes[n3];
     for (int k = first; k < first + count; k++) {
-      if (cnt) cnt->triangle_tests++;
+      if (cnt) { cnt->triangle_tests++; cnt->any_triangle_tests++; }
       const float *tr = triangles + (size_t)k * 9;
       double e1x = tr[3], e1y = tr[4], e1z = tr[5];
       double e2x = tr[6], e2y = tr[7], e2z = tr[8];
@@ -1364,6 +1364,10 @@ double orc_render_items_mt(const mcr_geometry *g, const mcr_shading *s, const mc
       counters->leaf_visits += c.leaf_visits;
       counters->triangle_tests += c.triangle_tests;
       counters->paths += c.paths;
+      counters->closest_node_visits += c.closest_node_visits;
+      counters->closest_triangle_tests += c.closest_triangle_tests;
+      counters->any_node_visits += c.any_node_visits;
+      counters->any_triangle_tests += c.any_triangle_tests;
     }
   }
   return secs;
