@@ -1,0 +1,170 @@
+"""Benchmark: trace throughput (Mrays/s) of the plain GPU workload on one MI355X.
+
+Workload: a 724x724-quad fBm heightfield plus 64 subdivision-4 icospheres (1,376,032 triangles,
+20% glass), a sun and 4 point lights, 1920x1080, global illumination on, 8 bounces, ACES. The BVH
+is built on the GPU. One step is one full-frame pass (1 sample per pixel) of the 64-sample frame,
+traced with the scene resident in HBM, followed by the on-device accumulate + encode (writeTile).
+Mrays/s counts every BVH query: closest hits (camera, bounce, pass-through) and shadow any-hits.
+
+Prints one JSON line. `roofline` is computed for the kernel that takes the most device time,
+from algorithmic bytes (inner-node records x 64 B + triangle records x 48 B, counted by the
+kernels) over its HIP-event-timed duration. `cpu_baseline` times the oracle (the C++ restatement
+of the reference's CPU path) on a bounded sample of the same frame on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'Mrays/s @1M tris 1080p 64spp vs CPU; p50/p99 time-to-first-preview in mix'
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES = 64, 48
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, cam, st, width, height, seconds=15.0):
+    """Times the oracle on a sample of 64x64 tiles of pass 1 (centre-first order, as makeTiles
+    sorts them) with one thread per host core."""
+    import oracle
+    from meercadraytracer_amd import abi
+    threads = os.cpu_count() or 1
+    tiles = oracle.make_tiles(width, height, 64)
+
+    def jobs_for(k):
+        jobs, off = [], 0
+        for (x, y, w, h) in tiles[:k]:
+            jobs.append(abi.TileJob(abi.Tile(x, y, w, h), width, height, 1, 1, off))
+            off += w * h * 4
+        return jobs
+    # calibrate: one tile per thread
+    probe = jobs_for(threads)
+    secs, cnt = oracle.render_items_mt(sc, cam, st, probe, threads)
+    per_tile = secs / max(1, len(probe)) * threads
+    k = int(min(len(tiles), max(threads, seconds / max(per_tile, 1e-6) * threads)))
+    jobs = jobs_for(k)
+    secs, cnt = oracle.render_items_mt(sc, cam, st, jobs, threads)
+    rays = cnt['closest_queries'] + cnt['any_queries']
+    return {'value': rays / secs / 1e6, 'unit': 'Mrays/s', 'cores': threads, 'kind': 'port',
+            'sample': f'{k} of {len(tiles)} centre-first 64x64 tiles of pass 1 of the same 1920x1080 frame, '
+                      f'{rays} queries in {secs:.2f} s (portable-math oracle, std::thread pool)'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=8)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--width', type=int, default=1920)
+    ap.add_argument('--height', type=int, default=1080)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU: --gpus {args.gpus} is not supported by this build '
+              '(every GPU box here has one GPU; the multi-GPU sharding is a design, not a run)', file=sys.stderr)
+        sys.exit(2)
+
+    from meercadraytracer_amd import abi, native, scenes
+    from meercadraytracer_amd.compile import compile_camera, compile_scene
+    from meercadraytracer_amd.raytrace import trace_settings
+    from meercadraytracer_amd.types import Tile
+
+    W, H = args.width, args.height
+    ctx = native.default_context()
+    t0 = time.time()
+    scene, camera, opts = scenes.plain_gpu_scene(0, W, H)
+    build_times = []
+
+    def timed_build(tris, count):
+        t = time.time()
+        r = ctx.build_bvh(tris, count)
+        build_times.append(time.time() - t)
+        return r
+    sc = compile_scene(scene, timed_build)
+    cam = compile_camera(camera)
+    st = trace_settings(opts)
+    log(f'compiled {sc.geometry.triangleCount} triangles, {sc.geometry.bvh.nodeCount} nodes in '
+        f'{time.time() - t0:.1f} s (GPU build {build_times[0] * 1e3:.1f} ms incl. copies)')
+    gpu = native.Scene(ctx)
+    gpu.set_geometry(sc.geometry, 1)
+    gpu.set_shading(sc.shading, 1)
+    npx = W * H
+    data = native.DeviceBuffer(ctx, npx * 16)
+    acc = native.DeviceBuffer(ctx, npx * 16)
+    image = native.DeviceBuffer(ctx, npx * 4)
+    acc.zero()
+    tile = Tile(0, 0, W, H)
+
+    def step(p, cnt):
+        job = abi.TileJob(tile.to_c(), W, H, p, 1, 0)
+        gpu.render_jobs(cam, st, [job], out=data.ptr, out_device=True, counters=cnt)
+        native.encode_tile_device(ctx, image.ptr, W, tile, data.ptr, acc.ptr, p + 1, True)
+
+    ctx.set_timing(True)
+    for p in range(args.warmup):
+        step(p, abi.Counters())
+        log(f'warmup pass {p} done')
+    cnt = abi.Counters()
+    timing = {'closest_ms': 0.0, 'any_ms': 0.0, 'shade_ms': 0.0, 'total_ms': 0.0, 'closest_launches': 0,
+              'any_launches': 0}
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i, cnt)
+        t = ctx.last_timing()
+        for k in timing:
+            timing[k] += t[k]
+        log(f'pass {args.warmup + i} done')
+    elapsed = time.perf_counter() - t_start
+    c = cnt.as_dict()
+    rays = c['closest_queries'] + c['any_queries']
+    value = rays / elapsed / 1e6
+    kern = {
+        'extend (closest hit)': (timing['closest_ms'], timing['closest_launches'],
+                                 c['closest_node_visits'] * NODE_BYTES + c['closest_triangle_tests'] * TRI_BYTES),
+        'shadow (any hit)': (timing['any_ms'], timing['any_launches'],
+                             c['any_node_visits'] * NODE_BYTES + c['any_triangle_tests'] * TRI_BYTES),
+    }
+    name = max(kern, key=lambda k: kern[k][0])
+    ms, launches, algo = kern[name]
+    achieved = algo / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': name,
+                'avg_launch_ms': round(ms / max(1, launches), 4),
+                'algorithmic_bytes_per_launch': int(algo / max(1, launches))}
+    result = {
+        'metric': METRIC, 'value': round(value, 2), 'unit': 'Mrays/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(elapsed / args.steps * 1e3, 2), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (seeded scene generator)',
+        'config': {'workload': 'plain_gpu: 724x724-quad fBm heightfield + 64 icospheres (1,376,032 triangles, '
+                               '20% glass), sun + 4 point lights, 1920x1080, GI on, 8 bounces, ACES; one step = '
+                               'one 1-spp pass of the 64-spp frame, scene resident in HBM',
+                   'triangles': int(sc.geometry.triangleCount), 'width': W, 'height': H, 'spp_per_step': 1,
+                   'frame_spp': 64, 'paths_per_step': npx,
+                   'rays_per_step': int(rays / args.steps),
+                   'kernel_ms': {'extend': round(timing['closest_ms'] / args.steps, 2),
+                                 'shadow': round(timing['any_ms'] / args.steps, 2),
+                                 'shade': round(timing['shade_ms'] / args.steps, 2),
+                                 'device_total': round(timing['total_ms'] / args.steps, 2)},
+                   'bvh_build_ms_incl_copies': round(build_times[0] * 1e3, 1)},
+        'roofline': roofline,
+    }
+    if not args.no_cpu:
+        log('cpu baseline ...')
+        result['cpu_baseline'] = cpu_baseline(sc, cam, st, W, H, args.cpu_seconds)
+    for b in (data, acc, image):
+        b.free()
+    gpu.release()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()
