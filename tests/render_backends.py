@@ -30,8 +30,8 @@ def raytrace(backend, scene, camera, width, height, options):
         srgb = o.get('outputColorSpace', 'srgb') != 'srgb-linear'
         data, _ = oracle.raytrace(sc, cam, trace_settings(o), width, height, samples, srgb)
         return data
-    from meercadraytracer_amd import raytrace as rt
-    return rt.raytrace(scene, camera, width, height, options)['data']
+    from meercadraytracer_amd.raytrace import raytrace as gpu_raytrace
+    return gpu_raytrace(scene, camera, width, height, options)['data']
 
 
 def render(backend, scene, camera, options=None, width=16, height=16):
