@@ -2,8 +2,9 @@
 
 Workload: a 724x724-quad fBm heightfield plus 64 subdivision-4 icospheres (1,376,032 triangles,
 20% glass), a sun and 4 point lights, 1920x1080, global illumination on, 8 bounces, ACES. The BVH
-is built on the GPU. One step is one full-frame pass (1 sample per pixel) of the 64-sample frame,
-traced with the scene resident in HBM, followed by the on-device accumulate + encode (writeTile).
+is built on the GPU. One step is one full-frame pass (1 sample per pixel); the default 64 steps
+are the 64-sample frame, traced in one device run with the scene resident in HBM, followed by the
+on-device accumulate + encode (writeTile) of every pass in order.
 Mrays/s counts every BVH query: closest hits (camera, bounce, pass-through) and shadow any-hits.
 
 Prints one JSON line. `roofline` is computed for the kernel that takes the most device time,
@@ -32,8 +33,9 @@ def log(msg):
 
 
 def cpu_baseline(sc, cam, st, width, height, seconds=15.0):
-    """Times the oracle on a sample of 64x64 tiles of pass 1 (centre-first order, as makeTiles
-    sorts them) with one thread per host core."""
+    """Times the oracle on a sample of the frame's (tile, pass) work items in the reference's
+    progressive order (64x64 tiles centre-first, pass-major from pass 1) with one thread per host
+    core, sized to about `seconds` of work."""
     import oracle
     from meercadraytracer_amd import abi
     threads = os.cpu_count() or 1
@@ -41,28 +43,30 @@ def cpu_baseline(sc, cam, st, width, height, seconds=15.0):
 
     def jobs_for(k):
         jobs, off = [], 0
-        for (x, y, w, h) in tiles[:k]:
-            jobs.append(abi.TileJob(abi.Tile(x, y, w, h), width, height, 1, 1, off))
+        for i in range(k):
+            x, y, w, h = tiles[i % len(tiles)]
+            jobs.append(abi.TileJob(abi.Tile(x, y, w, h), width, height, 1 + i // len(tiles), 1, off))
             off += w * h * 4
         return jobs
     # calibrate: one tile per thread
     probe = jobs_for(threads)
     secs, cnt = oracle.render_items_mt(sc, cam, st, probe, threads)
     per_tile = secs / max(1, len(probe)) * threads
-    k = int(min(len(tiles), max(threads, seconds / max(per_tile, 1e-6) * threads)))
+    k = int(min(64 * len(tiles), max(threads, seconds / max(per_tile, 1e-6) * threads)))
     jobs = jobs_for(k)
     secs, cnt = oracle.render_items_mt(sc, cam, st, jobs, threads)
     rays = cnt['closest_queries'] + cnt['any_queries']
     return {'value': rays / secs / 1e6, 'unit': 'Mrays/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{k} of {len(tiles)} centre-first 64x64 tiles of pass 1 of the same 1920x1080 frame, '
+            'sample': f'{k} (tile, pass) items = {k / len(tiles):.2f} passes of {len(tiles)} centre-first 64x64 tiles '
+                      f'of the same 1920x1080 frame from pass 1, '
                       f'{rays} queries in {secs:.2f} s (portable-math oracle, std::thread pool)'}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=8)
-    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=64)
+    ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--width', type=int, default=1920)
@@ -75,7 +79,7 @@ def main():
 
     from meercadraytracer_amd import abi, native, scenes
     from meercadraytracer_amd.compile import compile_camera, compile_scene
-    from meercadraytracer_amd.raytrace import trace_settings
+    from meercadraytracer_amd.raytrace import render_passes, trace_settings
     from meercadraytracer_amd.types import Tile
 
     W, H = args.width, args.height
@@ -98,32 +102,29 @@ def main():
     gpu.set_geometry(sc.geometry, 1)
     gpu.set_shading(sc.shading, 1)
     npx = W * H
-    data = native.DeviceBuffer(ctx, npx * 16)
+    data = native.DeviceBuffer(ctx, max(args.steps, args.warmup, 1) * npx * 16)
     acc = native.DeviceBuffer(ctx, npx * 16)
     image = native.DeviceBuffer(ctx, npx * 4)
     acc.zero()
     tile = Tile(0, 0, W, H)
 
-    def step(p, cnt):
-        job = abi.TileJob(tile.to_c(), W, H, p, 1, 0)
-        gpu.render_jobs(cam, st, [job], out=data.ptr, out_device=True, counters=cnt)
-        native.encode_tile_device(ctx, image.ptr, W, tile, data.ptr, acc.ptr, p + 1, True)
+    def run(passes, cnt):
+        # All the passes stream through the wavefront in one device run; each pass's frame is
+        # then summed and encoded in pass order (writeTile).
+        render_passes(gpu, cam, st, W, H, passes, data.ptr, cnt)
+        for i, p in enumerate(passes):
+            native.encode_tile_device(ctx, image.ptr, W, tile, data.ptr + i * npx * 16, acc.ptr, p + 1, True)
 
     ctx.set_timing(True)
-    for p in range(args.warmup):
-        step(p, abi.Counters())
-        log(f'warmup pass {p} done')
+    if args.warmup:
+        run(range(args.warmup), abi.Counters())
+        log(f'warmup: {args.warmup} passes done')
     cnt = abi.Counters()
-    timing = {'closest_ms': 0.0, 'any_ms': 0.0, 'shade_ms': 0.0, 'total_ms': 0.0, 'closest_launches': 0,
-              'any_launches': 0}
     t_start = time.perf_counter()
-    for i in range(args.steps):
-        step(args.warmup + i, cnt)
-        t = ctx.last_timing()
-        for k in timing:
-            timing[k] += t[k]
-        log(f'pass {args.warmup + i} done')
+    run(range(args.warmup, args.warmup + args.steps), cnt)
     elapsed = time.perf_counter() - t_start
+    timing = ctx.last_timing()
+    log(f'timed: {args.steps} passes in {elapsed:.3f} s')
     c = cnt.as_dict()
     rays = c['closest_queries'] + c['any_queries']
     value = rays / elapsed / 1e6

@@ -239,6 +239,9 @@ typedef struct mcr_timing {
 } mcr_timing;
 int mcr_last_timing(mcr_ctx *ctx, mcr_timing *out);
 int mcr_set_timing(mcr_ctx *ctx, int enabled);
+/* Path slots of the wavefront (samples in flight). Default 2^21; at least 256. Samples beyond the
+ * capacity start as earlier ones finish, so the capacity changes memory use, not results. */
+int mcr_set_wavefront_capacity(mcr_ctx *ctx, uint32_t slots);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,7 @@ struct mcr_ctx {
   Wavefront w = {};
   void *w_block = nullptr;
   uint32_t capacity = 0;
+  uint32_t capacity_limit = DEFAULT_CAPACITY;
   // traversal scratch
   int32_t *spill = nullptr;
   double *seen = nullptr;
@@ -127,6 +128,7 @@ int ensure_wavefront(mcr_ctx *c, uint32_t capacity) {
   size_t o_q1 = take(sizeof(int32_t) * P);
   size_t o_counts = take(sizeof(uint32_t) * 8);
   size_t o_sref = take(sizeof(int32_t) * L);
+  size_t o_free = take(sizeof(int32_t) * P);
   HIP_TRY(hipMalloc(&c->w_block, bytes));
   char *b = (char *)c->w_block;
   Wavefront &w = c->w;
@@ -142,6 +144,7 @@ int ensure_wavefront(mcr_ctx *c, uint32_t capacity) {
   w.queue[1] = (int32_t *)(b + o_q1);
   w.counts = (uint32_t *)(b + o_counts);
   w.sref = (int32_t *)(b + o_sref);
+  w.free_list = (int32_t *)(b + o_free);
   c->capacity = capacity;
   return MCR_OK;
 }
@@ -269,6 +272,21 @@ void mcr_destroy(mcr_ctx *c) {
     if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
+}
+
+int mcr_set_wavefront_capacity(mcr_ctx *c, uint32_t slots) {
+  if (!c) return fail(MCR_E_INVALID, "ctx is null");
+  if (slots < 256 || slots > (1u << 26)) return fail(MCR_E_INVALID, "capacity must be in [256, 2^26]");
+  std::lock_guard<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->capacity_limit = slots;
+  if (c->capacity > slots) {
+    hipFree(c->w_block);
+    c->w_block = nullptr;
+    c->capacity = 0;
+  }
+  return MCR_OK;
 }
 
 int mcr_set_timing(mcr_ctx *c, int enabled) {
@@ -511,9 +529,9 @@ static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const
   if (njobs == 0 || total == 0) return MCR_OK;
   int rc = ensure_jobs(c, njobs);
   if (rc) return rc;
-  uint32_t cap = (uint32_t)std::min<uint64_t>(total, DEFAULT_CAPACITY);
+  uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->capacity_limit);
   cap = std::max<uint32_t>(cap, 256);
-  rc = ensure_wavefront(c, std::max(cap, c->capacity));
+  rc = ensure_wavefront(c, std::min(std::max(cap, c->capacity), c->capacity_limit));
   if (rc) return rc;
   hipStream_t sm = c->stream;
   HIP_TRY(hipMemcpyAsync(c->jobs, jd.data(), njobs * sizeof(JobDev), hipMemcpyHostToDevice, sm));
@@ -524,47 +542,55 @@ static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const
   double t_closest = 0, t_any = 0, t_shade = 0;
   uint64_t n_closest = 0, n_any = 0;
   if (c->timing) HIP_TRY(hipEventRecord(c->ev[6], sm));
-  for (uint64_t s0 = 0; s0 < total; s0 += c->capacity) {
-    uint32_t count = (uint32_t)std::min<uint64_t>(c->capacity, total - s0);
-    c->counts_host[0] = count;
-    HIP_TRY(hipMemsetAsync(w.counts, 0, sizeof(uint32_t) * 4, sm));
-    HIP_TRY(hipMemcpyAsync(w.counts, c->counts_host, sizeof(uint32_t), hipMemcpyHostToDevice, sm));
-    launch_raygen(sm, c->jobs, njobs, (uint32_t)s0, count, w, *camera, st->seed);
-    int q = 0;
-    uint32_t live = count;
-    while (live > 0) {
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], sm));
-      launch_extend(sm, s->d, w, q, std::min<uint32_t>(TRAVERSE_GRID, (live + 255) / 256), c->spill,
-                    spill_stride, c->counters);
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], sm));
-      HIP_TRY(hipMemsetAsync(w.counts + 2, 0, sizeof(uint32_t), sm));
-      launch_shade(sm, s->d, w, q, live, *st, *camera, c->jobs, njobs, dout);
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], sm));
-      uint32_t max_shadow = live * std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
-      if (max_shadow)
-        launch_shadow(sm, s->d, w, cast_only, std::min<uint32_t>(TRAVERSE_GRID, (max_shadow + 255) / 256),
-                      c->spill, spill_stride, c->seen, c->counters);
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], sm));
-      HIP_TRY(hipMemsetAsync(w.counts + (q ^ 1), 0, sizeof(uint32_t), sm));
-      launch_resolve(sm, w, q, live, *st, s->d, c->jobs, njobs, dout);
-      HIP_TRY(hipMemcpyAsync(c->counts_host, w.counts, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, sm));
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], sm));
-      HIP_TRY(hipStreamSynchronize(sm));
-      if (c->timing) {
-        float a, b, d;
-        hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-        hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-        hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
-        t_closest += a;
-        t_shade += b;
-        t_any += d;
-        n_closest++;
-        if (max_shadow) n_any++;
-      }
-      live = c->counts_host[q ^ 1];
-      if (c->counts_host[3]) return fail(MCR_E_OVERFLOW, "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
-      q ^= 1;
+  // Path regeneration: slots freed by finished samples are refilled with the next samples of the
+  // call, so the wavefront stays full across passes and tiles and only the very end of the call
+  // runs with few live paths.
+  HIP_TRY(hipMemsetAsync(w.counts, 0, sizeof(uint32_t) * 8, sm));
+  uint64_t s_next = std::min<uint64_t>(c->capacity, total);
+  launch_raygen(sm, c->jobs, njobs, 0, (uint32_t)s_next, w, *camera, st->seed, 0, 0, 0);
+  int q = 0;
+  uint32_t live = (uint32_t)s_next;
+  while (live > 0) {
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], sm));
+    launch_extend(sm, s->d, w, q, std::min<uint32_t>(TRAVERSE_GRID, (live + 255) / 256), c->spill,
+                  spill_stride, c->counters);
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], sm));
+    HIP_TRY(hipMemsetAsync(w.counts + 2, 0, sizeof(uint32_t), sm));
+    launch_shade(sm, s->d, w, q, live, *st, *camera, c->jobs, njobs, dout);
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], sm));
+    uint32_t max_shadow = live * std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
+    if (max_shadow)
+      launch_shadow(sm, s->d, w, cast_only, std::min<uint32_t>(TRAVERSE_GRID, (max_shadow + 255) / 256),
+                    c->spill, spill_stride, c->seen, c->counters);
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], sm));
+    HIP_TRY(hipMemsetAsync(w.counts + (q ^ 1), 0, sizeof(uint32_t), sm));
+    HIP_TRY(hipMemsetAsync(w.counts + 4, 0, sizeof(uint32_t), sm));
+    launch_resolve(sm, w, q, live, *st, s->d, c->jobs, njobs, dout);
+    HIP_TRY(hipMemcpyAsync(c->counts_host, w.counts, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, sm));
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+    if (c->timing) {
+      float a, b, d;
+      hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+      hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+      hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
+      t_closest += a;
+      t_shade += b;
+      t_any += d;
+      n_closest++;
+      if (max_shadow) n_any++;
     }
+    if (c->counts_host[3])
+      return fail(MCR_E_OVERFLOW, "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
+    live = c->counts_host[q ^ 1];
+    uint32_t nfree = c->counts_host[4];
+    uint32_t refill = (uint32_t)std::min<uint64_t>(nfree, total - s_next);
+    if (refill) {
+      launch_raygen(sm, c->jobs, njobs, (uint32_t)s_next, refill, w, *camera, st->seed, q ^ 1, live, 1);
+      s_next += refill;
+      live += refill;
+    }
+    q ^= 1;
   }
   if (c->timing) {
     HIP_TRY(hipEventRecord(c->ev[7], sm));

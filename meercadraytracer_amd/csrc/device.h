@@ -22,7 +22,7 @@
 #define MCR_LEAF_FIRST_BITS 26
 #define MCR_LEAF_FIRST_MASK ((1u << MCR_LEAF_FIRST_BITS) - 1u)
 #define MCR_MAX_TRIANGLES (1u << MCR_LEAF_FIRST_BITS)
-#define MCR_STACK_LDS 24      // per-lane traversal stack entries kept in LDS
+#define MCR_STACK_LDS 16      // per-lane traversal stack entries kept in LDS
 #define MCR_STACK_MAX 128     // total per-lane stack (tree depth <= 100, bvh.ts:5)
 
 struct DevMaterial {
@@ -87,22 +87,48 @@ __device__ inline void leaf_range(const DevScene &s, int32_t ref, int &first, in
 // bvh.ts:336-338
 __device__ inline double safe_inverse(double d) { return d != 0 ? 1.0 / d : 1e30; }
 
-// bvh.ts:340-356 on one child box given as 6 floats.
-__device__ inline double box_entry(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
-                                   double ox, double oy, double oz, double ix, double iy, double iz,
-                                   double tmin, double tmax) {
+// bvh.ts:340-356 on one child box given as 6 floats. EXACT = false uses the hardware's min/max,
+// which differs from JS Math.min/max only for NaN operands and the order of -0/+0. Neither can
+// change the result here: with finite inverse directions every slab distance is a number, and
+// signed zeros only meet comparisons. Traversals take the EXACT form for rays whose inverse
+// direction overflows (|d| below 2^-1023), where 0 * inf = NaN can occur.
+template <bool EXACT>
+__device__ inline double box_entry_t(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
+                                     double ox, double oy, double oz, double ix, double iy, double iz,
+                                     double tmin, double tmax) {
   double t0 = ((double)bx0 - ox) * ix, t1 = ((double)bx1 - ox) * ix;
-  double lo = js_min(t0, t1), hi = js_max(t0, t1);
+  double lo, hi;
+  if (EXACT) {
+    lo = js_min(t0, t1);
+    hi = js_max(t0, t1);
+  } else {
+    lo = __builtin_fmin(t0, t1);
+    hi = __builtin_fmax(t0, t1);
+  }
   t0 = ((double)by0 - oy) * iy;
   t1 = ((double)by1 - oy) * iy;
-  lo = js_max(lo, js_min(t0, t1));
-  hi = js_min(hi, js_max(t0, t1));
+  if (EXACT) {
+    lo = js_max(lo, js_min(t0, t1));
+    hi = js_min(hi, js_max(t0, t1));
+  } else {
+    lo = __builtin_fmax(lo, __builtin_fmin(t0, t1));
+    hi = __builtin_fmin(hi, __builtin_fmax(t0, t1));
+  }
   t0 = ((double)bz0 - oz) * iz;
   t1 = ((double)bz1 - oz) * iz;
-  lo = js_max(lo, js_min(t0, t1));
-  hi = js_min(hi, js_max(t0, t1));
-  if (hi >= lo && hi >= tmin && lo <= tmax) return js_max(lo, tmin);
+  if (EXACT) {
+    lo = js_max(lo, js_min(t0, t1));
+    hi = js_min(hi, js_max(t0, t1));
+  } else {
+    lo = __builtin_fmax(lo, __builtin_fmin(t0, t1));
+    hi = __builtin_fmin(hi, __builtin_fmax(t0, t1));
+  }
+  if (hi >= lo && hi >= tmin && lo <= tmax) return EXACT ? js_max(lo, tmin) : __builtin_fmax(lo, tmin);
   return JM_INF;
+}
+
+__device__ inline bool finite_inverse(double ix, double iy, double iz) {
+  return jm_fabs(ix) < JM_INF && jm_fabs(iy) < JM_INF && jm_fabs(iz) < JM_INF;
 }
 
 // Per-lane traversal stack: the first MCR_STACK_LDS entries in LDS (lane-interleaved, so a
@@ -160,16 +186,11 @@ __device__ inline bool mt_test(const float4 a, const float4 b, const float4 c, d
 
 // bvh.ts:202-282 over the device layout. cull: drop back faces of FrontSide and front faces of
 // BackSide triangles.
-__device__ inline bool intersect_closest(const DevScene &s, double ox, double oy, double oz,
-                                         double dx, double dy, double dz, double tmin, double tmax,
-                                         bool cull, TriHit &hit, LaneStack &st, TravCount &cnt) {
-  hit.tri = -1;
-  hit.t = tmax;
-  hit.u = 0;
-  hit.v = 0;
-  hit.front = true;
-  if (s.root_ref == -1) return false;
-  double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+template <bool EXACT>
+__device__ inline bool closest_impl(const DevScene &s, double ox, double oy, double oz, double dx,
+                                    double dy, double dz, double ix, double iy, double iz,
+                                    double tmin, bool cull, TriHit &hit, LaneStack &st,
+                                    TravCount &cnt) {
   bool neg[3] = {ix < 0, iy < 0, iz < 0};
   st.sp = 0;
   int32_t ref = s.root_ref;
@@ -199,8 +220,8 @@ __device__ inline bool intersect_closest(const DevScene &s, double ox, double oy
       float4 p = r[0], q = r[1], w = r[2];
       int4 m = *reinterpret_cast<const int4 *>(r + 3);
       bool right_first = neg[m.z];
-      double tl = box_entry(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, hit.t);
-      double tr = box_entry(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, hit.t);
+      double tl = box_entry_t<EXACT>(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, hit.t);
+      double tr = box_entry_t<EXACT>(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, hit.t);
       int32_t near_ref = right_first ? m.y : m.x, far_ref = right_first ? m.x : m.y;
       double tn = right_first ? tr : tl, tf = right_first ? tl : tr;
       if (tn < JM_INF) {
@@ -228,18 +249,31 @@ __device__ inline bool intersect_closest(const DevScene &s, double ox, double oy
   return hit.tri >= 0;
 }
 
+__device__ inline bool intersect_closest(const DevScene &s, double ox, double oy, double oz,
+                                         double dx, double dy, double dz, double tmin, double tmax,
+                                         bool cull, TriHit &hit, LaneStack &st, TravCount &cnt) {
+  hit.tri = -1;
+  hit.t = tmax;
+  hit.u = 0;
+  hit.v = 0;
+  hit.front = true;
+  if (s.root_ref == -1) return false;
+  double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+  if (finite_inverse(ix, iy, iz))
+    return closest_impl<false>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, cull, hit, st, cnt);
+  return closest_impl<true>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, cull, hit, st, cnt);
+}
+
 // bvh.ts:289-332 over the device layout: every hit in (tmin, tmax), no culling, in the
 // reference's depth-first order (right child first). Both child boxes are tested when their parent
 // is visited; a box test does not depend on when it runs, so the visit order is unchanged.
 // visit(k, t, u, v, front) returns false to stop.
-template <class Visit>
-__device__ inline void intersect_any(const DevScene &s, double ox, double oy, double oz, double dx,
-                                     double dy, double dz, double tmin, double tmax, LaneStack &st,
-                                     TravCount &cnt, Visit &visit) {
-  if (s.root_ref == -1) return;
-  double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+template <bool EXACT, class Visit>
+__device__ inline void any_impl(const DevScene &s, double ox, double oy, double oz, double dx,
+                                double dy, double dz, double ix, double iy, double iz, double tmin,
+                                double tmax, LaneStack &st, TravCount &cnt, Visit &visit) {
   const float *rb = s.root_box;
-  if (!(box_entry(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF))
+  if (!(box_entry_t<EXACT>(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF))
     return;
   st.sp = 0;
   int32_t ref = s.root_ref;
@@ -261,8 +295,8 @@ __device__ inline void intersect_any(const DevScene &s, double ox, double oy, do
       const float4 *r = s.inner + (size_t)ref * 4;
       float4 p = r[0], q = r[1], w = r[2];
       int4 m = *reinterpret_cast<const int4 *>(r + 3);
-      bool hl = box_entry(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
-      bool hr = box_entry(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
+      bool hl = box_entry_t<EXACT>(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
+      bool hr = box_entry_t<EXACT>(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
       if (hr) {
         if (hl) st.push(m.x);
         ref = m.y;
@@ -276,6 +310,18 @@ __device__ inline void intersect_any(const DevScene &s, double ox, double oy, do
     if (st.sp == 0) break;
     ref = st.pop();
   }
+}
+
+template <class Visit>
+__device__ inline void intersect_any(const DevScene &s, double ox, double oy, double oz, double dx,
+                                     double dy, double dz, double tmin, double tmax, LaneStack &st,
+                                     TravCount &cnt, Visit &visit) {
+  if (s.root_ref == -1) return;
+  double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+  if (finite_inverse(ix, iy, iz))
+    any_impl<false>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, visit);
+  else
+    any_impl<true>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, visit);
 }
 
 // Sum over the 64 lanes of a wave.

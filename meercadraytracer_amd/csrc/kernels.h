@@ -10,7 +10,8 @@
 
 hipError_t mcr_upload_dfg(const float *lut512);
 void launch_raygen(hipStream_t s, const JobDev *jobs, uint32_t njobs, uint32_t s0, uint32_t count,
-                   const Wavefront &w, const mcr_camera &cam, uint32_t seed);
+                   const Wavefront &w, const mcr_camera &cam, uint32_t seed, int qn, uint32_t qbase,
+                   int use_free);
 void launch_extend(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
 void launch_shade(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t max_live,

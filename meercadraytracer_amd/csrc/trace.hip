@@ -64,12 +64,18 @@ __device__ inline void unproject(const mcr_camera &c, double x, double y, double
 }
 
 // ---- ray generation: tracer.ts:62-80 + 705-713 --------------------------------------------------
+// Starts samples s0 .. s0+count-1 in path slots (slot i, or free_list[i] when slots are being
+// reused) and appends the slots to queue qn at position qbase. Samples keep their own seeds and
+// output positions, so the order in which slots are refilled never changes a result.
 __global__ __launch_bounds__(256) void k_raygen(const JobDev *jobs, uint32_t njobs, uint32_t s0,
                                                 uint32_t count, Wavefront w, mcr_camera cam,
-                                                uint32_t seed) {
-  uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= count) return;
-  uint32_t s = s0 + slot;
+                                                uint32_t seed, int qn, uint32_t qbase, int use_free) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  if (i == 0) w.counts[qn] = qbase + count;
+  uint32_t slot = use_free ? (uint32_t)w.free_list[i] : i;
+  w.queue[qn][qbase + i] = (int32_t)slot;
+  uint32_t s = s0 + i;
   const JobDev &j = jobs[find_job(jobs, njobs, s)];
   uint32_t li = s - j.sample_start;
   uint32_t x = (li % j.cols) * j.block, y = (li / j.cols) * j.block;
@@ -117,7 +123,6 @@ __global__ __launch_bounds__(256) void k_raygen(const JobDev *jobs, uint32_t njo
   iv[I_BOUNCE * P + slot] = 0;
   iv[I_PASSES * P + slot] = 0;
   iv[I_SAMPLE * P + slot] = (int32_t)s;
-  w.queue[0][slot] = (int32_t)slot;
 }
 
 // ---- extend: closest hit for every live slot (bvh.ts:202-282) ------------------------------------
@@ -760,6 +765,9 @@ __global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_setting
   bool requeue = mode == MODE_EXTEND || mode == MODE_ADD_EXTEND;
   uint32_t pos = wave_append(&w.counts[q ^ 1], requeue ? 1u : 0u);
   if (requeue) w.queue[q ^ 1][pos] = slot;
+  bool freed = live && !requeue;
+  uint32_t fpos = wave_append(&w.counts[4], freed ? 1u : 0u);
+  if (freed) w.free_list[fpos] = slot;
   if (mode == MODE_ADD_FINISH || mode == MODE_FINISH) {
     // tracer.ts:472-477: tone map, then fog, alpha 1.
     double c[4] = {f[F_LR * P + slot], f[F_LG * P + slot], f[F_LB * P + slot], 1};
@@ -884,9 +892,10 @@ hipError_t mcr_upload_dfg(const float *lut512) {
 }
 
 void launch_raygen(hipStream_t s, const JobDev *jobs, uint32_t njobs, uint32_t s0, uint32_t count,
-                   const Wavefront &w, const mcr_camera &cam, uint32_t seed) {
+                   const Wavefront &w, const mcr_camera &cam, uint32_t seed, int qn, uint32_t qbase,
+                   int use_free) {
   hipLaunchKernelGGL(k_raygen, dim3(blocks_for(count)), dim3(256), 0, s, jobs, njobs, s0, count, w,
-                     cam, seed);
+                     cam, seed, qn, qbase, use_free);
 }
 void launch_extend(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {

@@ -58,8 +58,10 @@ struct Wavefront {
   uint8_t *status;           // 1 row: LightStatus
   // queues
   int32_t *queue[2];         // live slots
-  uint32_t *counts;          // [0],[1] queue lengths, [2] shadow queue length, [3] overflow flag
+  uint32_t *counts;          // [0],[1] queue lengths, [2] shadow queue length, [3] overflow flag,
+                             // [4] free-slot count
   int32_t *sref;             // shadow queue: slot * MCR_MAX_LIGHTS + light
+  int32_t *free_list;        // slots whose sample finished this iteration
 };
 
 struct JobDev {  // one tile job, device side

@@ -63,6 +63,7 @@ def lib():
             'mcr_memset': (C.c_int, [P, P, C.c_int, C.c_uint64]),
             'mcr_last_timing': (C.c_int, [P, C.POINTER(abi.Timing)]),
             'mcr_set_timing': (C.c_int, [P, C.c_int]),
+            'mcr_set_wavefront_capacity': (C.c_int, [P, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -133,6 +134,9 @@ class Context:
 
     def set_timing(self, on=True):
         check(lib().mcr_set_timing(self.handle, int(on)))
+
+    def set_capacity(self, slots):
+        check(lib().mcr_set_wavefront_capacity(self.handle, int(slots)))
 
     def last_timing(self):
         t = abi.Timing()

@@ -40,9 +40,12 @@ def raytrace(scene, camera, width, height, options=None, ctx=None):  # raytrace.
     return raytrace_compiled(compiled, compile_camera(camera), width, height, options, ctx=ctx)
 
 
-def raytrace_compiled(scene, camera, width, height, options=None, ctx=None, counters=None):
+def raytrace_compiled(scene, camera, width, height, options=None, ctx=None, counters=None,
+                      pass_batch_bytes=4 << 30):
     """raytrace.ts:69-82: `samples` passes of one full-image tile, each accumulated into the FP32
-    sum and encoded as writeTile does."""
+    sum and encoded as writeTile does. Passes are traced in batches (one device run streams all
+    the batch's samples through the wavefront); each pass keeps its own output so the passes are
+    still summed in order, as the reference sums them."""
     o = options or {}
     settings = trace_settings(o)
     samples = max(1, math.floor(o.get('samples', 16)))
@@ -52,21 +55,30 @@ def raytrace_compiled(scene, camera, width, height, options=None, ctx=None, coun
     gpu.set_geometry(scene.geometry, 1)
     gpu.set_shading(scene.shading, 1)
     npx = width * height
-    data = native.DeviceBuffer(ctx, npx * 16)
+    batch = max(1, min(samples, pass_batch_bytes // (npx * 16)))
+    data = native.DeviceBuffer(ctx, batch * npx * 16)
     acc = native.DeviceBuffer(ctx, npx * 16)
     image = native.DeviceBuffer(ctx, npx * 4)
     acc.zero()
     image.zero()
-    tile = abi.Tile(0, 0, width, height)
     from .types import Tile
     t = Tile(0, 0, width, height)
     cnt = counters if counters is not None else abi.Counters()
-    for p in range(samples):
-        job = abi.TileJob(tile, width, height, p, 1, 0)
-        gpu.render_jobs(camera, settings, [job], out=data.ptr, out_device=True, counters=cnt)
-        native.encode_tile_device(ctx, image.ptr, width, t, data.ptr, acc.ptr, p + 1, srgb)
+    for p0 in range(0, samples, batch):
+        passes = range(p0, min(samples, p0 + batch))
+        render_passes(gpu, camera, settings, width, height, passes, data.ptr, cnt)
+        for i, p in enumerate(passes):
+            native.encode_tile_device(ctx, image.ptr, width, t, data.ptr + i * npx * 16, acc.ptr, p + 1, srgb)
     out = image.download(np.uint8, npx * 4)
     for b in (data, acc, image):
         b.free()
     gpu.release()
     return RaytraceResult(width=width, height=height, data=out)
+
+
+def render_passes(gpu, camera, settings, width, height, passes, out_ptr, counters=None):
+    """Full-image passes in one device run; pass i of the list lands at out_ptr + i * frame."""
+    tile = abi.Tile(0, 0, width, height)
+    npx = width * height
+    jobs = [abi.TileJob(tile, width, height, p, 1, i * npx * 4) for i, p in enumerate(passes)]
+    gpu.render_jobs(camera, settings, jobs, out=out_ptr, out_device=True, counters=counters)

@@ -76,6 +76,32 @@ def test_batched_jobs_equal_single_tiles(c0):
     s.release()
 
 
+def test_slot_refill_small_capacity(c0):
+    """A 256-slot wavefront streams 18 jobs (6,912 samples) through refilled slots: every tile is
+    still bit-identical to the oracle's."""
+    from meercadraytracer_amd import native
+    sc, cam, base = c0
+    st = trace_settings(dict(base, globalIllumination=True, maxBounces=5))
+    ctx = native.default_context()
+    s = _gpu_scene(sc)
+    jobs, tiles, off = [], [], 0
+    for p in range(6):
+        for t in (Tile(0, 0, 48, 32), Tile(48, 0, 48, 32), Tile(20, 32, 40, 32)):
+            jobs.append(abi.TileJob(t.to_c(), 96, 64, p, 1 if p % 3 else 2, off))
+            tiles.append(t)
+            off += t.width * t.height * 4
+    ctx.set_capacity(256)
+    try:
+        out = s.render_jobs(cam, st, jobs)
+    finally:
+        ctx.set_capacity(1 << 21)
+    for j, t in zip(jobs, tiles):
+        ref, _ = oracle.render_tile(sc, cam, st, t, 96, 64, j.pass_, j.block)
+        got = out[j.out_offset:j.out_offset + t.width * t.height * 4]
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    s.release()
+
+
 def test_raytrace_bytes_equal_oracle(c0):
     from meercadraytracer_amd.raytrace import raytrace_compiled
     sc, cam, base = c0
