@@ -57,6 +57,7 @@ struct mcr_ctx {
   // traversal scratch
   int32_t *spill = nullptr;
   double *seen = nullptr;
+  int2 *seen_meta = nullptr;
   DevCounters *counters = nullptr;
   JobDev *jobs = nullptr;
   uint32_t jobs_cap = 0;
@@ -245,6 +246,7 @@ int mcr_create(int device, mcr_ctx **out) {
   e = mcr_upload_dfg(lut);
   if (e == hipSuccess) e = hipMalloc(&c->spill, sizeof(int32_t) * (size_t)(MCR_STACK_MAX - MCR_STACK_LDS) * TRAVERSE_THREADS);
   if (e == hipSuccess) e = hipMalloc(&c->seen, sizeof(double) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS);
+  if (e == hipSuccess) e = hipMalloc(&c->seen_meta, sizeof(int2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS);
   if (e == hipSuccess) e = hipMalloc(&c->counters, sizeof(DevCounters));
   if (e == hipSuccess) e = hipHostMalloc(&c->counts_host, sizeof(uint32_t) * 8, hipHostMallocDefault);
   for (int i = 0; i < 8 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
@@ -263,6 +265,7 @@ void mcr_destroy(mcr_ctx *c) {
   hipFree(c->w_block);
   hipFree(c->spill);
   hipFree(c->seen);
+  hipFree(c->seen_meta);
   hipFree(c->counters);
   hipFree(c->jobs);
   hipFree(c->out_tmp);
@@ -561,7 +564,7 @@ static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const
     uint32_t max_shadow = live * std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
     if (max_shadow)
       launch_shadow(sm, s->d, w, cast_only, std::min<uint32_t>(TRAVERSE_GRID, (max_shadow + 255) / 256),
-                    c->spill, spill_stride, c->seen, c->counters);
+                    c->spill, spill_stride, c->seen, c->seen_meta, c->counters);
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], sm));
     HIP_TRY(hipMemsetAsync(w.counts + (q ^ 1), 0, sizeof(uint32_t), sm));
     HIP_TRY(hipMemsetAsync(w.counts + 4, 0, sizeof(uint32_t), sm));

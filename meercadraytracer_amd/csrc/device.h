@@ -324,6 +324,77 @@ __device__ inline void intersect_any(const DevScene &s, double ox, double oy, do
     any_impl<true>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, visit);
 }
 
+// Shadow-ray query (tracer.ts:529-594 over bvh.ts:289-332). The reference visits every hit in
+// (tmin, tmax) in depth-first order, right child first, and stops at the first opaque one. The
+// set of triangles it can visit depends only on box tests with the fixed (tmin, tmax), never on
+// the order, and an opaque hit anywhere in that set makes the light invisible whatever came
+// before it. So this traversal takes the nearer child first and stops at the first opaque hit;
+// see-through hits are handed to on_hit with their leaf's first triangle, so the caller can replay
+// them in the reference's order (leaves by descending first triangle, then ascending index).
+// on_hit(k, leaf_first, t, front) returns true for an opaque hit.
+template <bool EXACT, class OnHit>
+__device__ inline bool occluded_impl(const DevScene &s, double ox, double oy, double oz, double dx,
+                                     double dy, double dz, double ix, double iy, double iz,
+                                     double tmin, double tmax, LaneStack &st, TravCount &cnt,
+                                     OnHit &on_hit) {
+  const float *rb = s.root_box;
+  if (!(box_entry_t<EXACT>(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF))
+    return false;
+  st.sp = 0;
+  int32_t ref = s.root_ref;
+  for (;;) {
+    if (ref_is_leaf(ref)) {
+      int first, count;
+      leaf_range(s, ref, first, count);
+      cnt.leaves++;
+      for (int k = first; k < first + count; k++) {
+        cnt.tris++;
+        const float4 *r = s.tri + (size_t)k * 3;
+        float4 a = r[0], b = r[1], c = r[2];
+        double t, u, v, det;
+        if (!mt_test(a, b, c, ox, oy, oz, dx, dy, dz, 2, t, u, v, det)) continue;
+        if (t > tmin && t < tmax && on_hit(k, first, t, det > 0, c)) return true;
+      }
+    } else {
+      cnt.nodes++;
+      const float4 *r = s.inner + (size_t)ref * 4;
+      float4 p = r[0], q = r[1], w = r[2];
+      int4 m = *reinterpret_cast<const int4 *>(r + 3);
+      double tl = box_entry_t<EXACT>(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, tmax);
+      double tr = box_entry_t<EXACT>(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, tmax);
+      bool hl = tl < JM_INF, hr = tr < JM_INF;
+      if (hl && hr) {
+        bool left_first = tl <= tr;
+        st.push(left_first ? m.y : m.x);
+        ref = left_first ? m.x : m.y;
+        continue;
+      }
+      if (hl) {
+        ref = m.x;
+        continue;
+      }
+      if (hr) {
+        ref = m.y;
+        continue;
+      }
+    }
+    if (st.sp == 0) break;
+    ref = st.pop();
+  }
+  return false;
+}
+
+template <class OnHit>
+__device__ inline bool occluded(const DevScene &s, double ox, double oy, double oz, double dx,
+                                double dy, double dz, double tmin, double tmax, LaneStack &st,
+                                TravCount &cnt, OnHit &on_hit) {
+  if (s.root_ref == -1) return false;
+  double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
+  if (finite_inverse(ix, iy, iz))
+    return occluded_impl<false>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, on_hit);
+  return occluded_impl<true>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, on_hit);
+}
+
 // Sum over the 64 lanes of a wave.
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

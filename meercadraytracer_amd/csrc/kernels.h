@@ -19,7 +19,7 @@ void launch_shade(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, 
                   float *out);
 void launch_shadow(hipStream_t s, const DevScene &sc, const Wavefront &w, int cast_only,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, double *seen,
-                   DevCounters *cnt);
+                   int2 *seen_meta, DevCounters *cnt);
 void launch_resolve(hipStream_t s, const Wavefront &w, int q, uint32_t max_live,
                     const mcr_settings &st, const DevScene &sc, const JobDev *jobs, uint32_t njobs,
                     float *out);

@@ -638,55 +638,43 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, Wavefront w, int q, 
 }
 
 // ---- shadow: transmittance + attenuateShadow for every queued shadow ray (tracer.ts:529-594) ----
-struct ShadowVisitor {
+// Collects the see-through hits of a shadow ray during a near-first traversal that stops at the
+// first opaque, shadow-casting hit (see occluded() in device.h).
+struct ShadowCollector {
   const DevScene *sc;
-  double col[3], depth[3];
   bool castOnly;
-  double *seen;  // this thread's list, stride `stride`
+  double *t;        // this thread's records, stride `stride`
+  int2 *meta;       // (leaf first triangle, triangle index | front << 31)
   size_t stride;
-  int nseen;
+  int n;
   bool overflow;
-  __device__ bool operator()(int tri, double t, double u, double v, bool front) {
-    (void)u;
-    (void)v;
-    float4 c = sc->tri[(size_t)tri * 3 + 2];
+  __device__ bool operator()(int k, int leaf_first, double tk, bool front, const float4 &c) {
     int flags = (int)((__float_as_uint(c.y) >> 8) & 0xffu);
-    if (castOnly && (flags & MCR_TRI_CAST_SHADOW) == 0) return true;
-    if (flags & MCR_TRI_OPAQUE) {
-      col[0] = col[1] = col[2] = 0;
-      return false;
-    }
+    if (castOnly && (flags & MCR_TRI_CAST_SHADOW) == 0) return false;  // not in the way
+    if (flags & MCR_TRI_OPAQUE) return true;
     const DevMaterial &m = sc->materials[__float_as_uint(c.z)];
-    double alpha = m.opacity;
-    double cr = m.color[0], cg = m.color[1], cb = m.color[2];
-    if (m.alpha_test > 0 && alpha < m.alpha_test) return true;
-    if (!m.transparent) alpha = 1;
-    double eps = sc->epsilon;
-    for (int j = 0; j < nseen; j++)
-      if (jm_fabs(seen[(size_t)j * stride] - t) <= eps) return true;
-    if (nseen < MCR_SHADOW_HIT_CAP) seen[(size_t)(nseen++) * stride] = t;
-    else overflow = true;
-    if (m.thickness > 0) {
-      double sgn = front ? -t : t;
-      depth[0] = depth[0] + m.attenuation[0] * sgn;
-      depth[1] = depth[1] + m.attenuation[1] * sgn;
-      depth[2] = depth[2] + m.attenuation[2] * sgn;
+    if (m.alpha_test > 0 && m.opacity < m.alpha_test) return false;    // cut out
+    if (n < MCR_SHADOW_HIT_CAP) {
+      t[(size_t)n * stride] = tk;
+      meta[(size_t)n * stride] = make_int2(leaf_first, (int)((uint32_t)k | (front ? 0x80000000u : 0u)));
+      n++;
+    } else {
+      overflow = true;
     }
-    double open = 1 - alpha, through = m.transmission;
-    col[0] = col[0] * (open + alpha * through * cr);
-    col[1] = col[1] * (open + alpha * through * cg);
-    col[2] = col[2] * (open + alpha * through * cb);
-    if (col[0] + col[1] + col[2] < 1e-4) {
-      col[0] = col[1] = col[2] = 0;
-      return false;
-    }
-    return true;
+    return false;
   }
 };
 
+// Reference visit order of two hits: leaves by descending first triangle (right subtrees first),
+// then triangles by ascending index within a leaf.
+__device__ inline bool visits_before(int2 a, int2 b) {
+  if (a.x != b.x) return a.x > b.x;
+  return (a.y & 0x7fffffff) < (b.y & 0x7fffffff);
+}
+
 __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int cast_only,
                                                 int32_t *spill, uint32_t spill_stride,
-                                                double *seen, DevCounters *cnt) {
+                                                double *seen, int2 *seen_meta, DevCounters *cnt) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t n = w.counts[2];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -701,6 +689,9 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int ca
   size_t P = w.capacity;
   const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
   bool overflow = false;
+  double *T = seen + gid;
+  int2 *M = seen_meta + gid;
+  const size_t S = nthreads;
   for (uint32_t e = gid; e < n; e += nthreads) {
     int32_t ref = w.sref[e];
     int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
@@ -708,24 +699,64 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int ca
     double ox = w.sray[0 * RS + idx], oy = w.sray[1 * RS + idx], oz = w.sray[2 * RS + idx];
     double dx = w.sray[3 * RS + idx], dy = w.sray[4 * RS + idx], dz = w.sray[5 * RS + idx];
     double maxT = w.sray[6 * RS + idx];
-    ShadowVisitor vis;
-    vis.sc = &sc;
-    vis.col[0] = vis.col[1] = vis.col[2] = 1;
-    vis.depth[0] = vis.depth[1] = vis.depth[2] = 0;
-    vis.castOnly = cast_only != 0;
-    vis.seen = seen + gid;
-    vis.stride = nthreads;
-    vis.nseen = 0;
-    vis.overflow = false;
-    intersect_any(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, vis);
+    ShadowCollector col{&sc, cast_only != 0, T, M, S, 0, false};
+    bool blocked = occluded(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, col);
     queries++;
-    overflow |= vis.overflow;
-    for (int c = 0; c < 3; c++)
-      if (vis.depth[c] > 0) vis.col[c] = vis.col[c] * jm_exp(-vis.depth[c]);
-    bool visible = vis.col[0] + vis.col[1] + vis.col[2] > 0;
-    w.shadow[0 * RS + idx] = vis.col[0];
-    w.shadow[1 * RS + idx] = vis.col[1];
-    w.shadow[2 * RS + idx] = vis.col[2];
+    overflow |= col.overflow;
+    double c[3] = {1, 1, 1}, depth[3] = {0, 0, 0};
+    if (blocked) {
+      c[0] = c[1] = c[2] = 0;
+    } else if (col.n > 0) {
+      // Replay the see-through hits in the reference's visit order (insertion sort, few hits).
+      for (int i = 1; i < col.n; i++) {
+        double ti = T[i * S];
+        int2 mi = M[i * S];
+        int j = i - 1;
+        while (j >= 0 && visits_before(mi, M[j * S])) {
+          T[(j + 1) * S] = T[j * S];
+          M[(j + 1) * S] = M[j * S];
+          j--;
+        }
+        T[(j + 1) * S] = ti;
+        M[(j + 1) * S] = mi;
+      }
+      double eps = sc.epsilon;
+      for (int i = 0; i < col.n; i++) {  // attenuateShadow for each hit, in order
+        double t = T[i * S];
+        int2 mi = M[i * S];
+        int k = mi.y & 0x7fffffff;
+        bool front = (mi.y & 0x80000000) != 0;
+        bool dup = false;
+        for (int j = 0; j < i; j++)  // tracer.ts:572: hits already counted (marked by leaf -1)
+          if (M[j * S].x >= 0 && jm_fabs(T[j * S] - t) <= eps) dup = true;
+        if (dup) {
+          M[i * S] = make_int2(-1, mi.y);
+          continue;
+        }
+        const DevMaterial &m = sc.materials[__float_as_uint(sc.tri[(size_t)k * 3 + 2].z)];
+        double alpha = m.transparent ? m.opacity : 1.0;
+        if (m.thickness > 0) {
+          double sgn = front ? -t : t;
+          depth[0] = depth[0] + m.attenuation[0] * sgn;
+          depth[1] = depth[1] + m.attenuation[1] * sgn;
+          depth[2] = depth[2] + m.attenuation[2] * sgn;
+        }
+        double open = 1 - alpha, through = m.transmission;
+        c[0] = c[0] * (open + alpha * through * m.color[0]);
+        c[1] = c[1] * (open + alpha * through * m.color[1]);
+        c[2] = c[2] * (open + alpha * through * m.color[2]);
+        if (c[0] + c[1] + c[2] < 1e-4) {
+          c[0] = c[1] = c[2] = 0;
+          break;
+        }
+      }
+      for (int k = 0; k < 3; k++)
+        if (depth[k] > 0) c[k] = c[k] * jm_exp(-depth[k]);
+    }
+    bool visible = c[0] + c[1] + c[2] > 0;
+    w.shadow[0 * RS + idx] = c[0];
+    w.shadow[1 * RS + idx] = c[1];
+    w.shadow[2 * RS + idx] = c[2];
     w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
   }
   if (overflow) atomicOr(&w.counts[3], 1u);
@@ -909,9 +940,9 @@ void launch_shade(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, 
 }
 void launch_shadow(hipStream_t s, const DevScene &sc, const Wavefront &w, int cast_only,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, double *seen,
-                   DevCounters *cnt) {
+                   int2 *seen_meta, DevCounters *cnt) {
   hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, s, sc, w, cast_only, spill, spill_stride,
-                     seen, cnt);
+                     seen, seen_meta, cnt);
 }
 void launch_resolve(hipStream_t s, const Wavefront &w, int q, uint32_t max_live,
                     const mcr_settings &st, const DevScene &sc, const JobDev *jobs, uint32_t njobs,
