@@ -71,7 +71,18 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--width', type=int, default=1920)
     ap.add_argument('--height', type=int, default=1080)
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='only run a device-to-device copy of MIB MiB (for scaling HBM counters)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        from meercadraytracer_amd import native
+        ctx = native.default_context()
+        n = args.copy_calibration << 20
+        a, b = native.DeviceBuffer(ctx, n), native.DeviceBuffer(ctx, n)
+        a.zero()
+        native.check(native.lib().mcr_copy(ctx.handle, b.ptr, a.ptr, n, 2))
+        print(json.dumps({'copy_bytes': n}), flush=True)
+        return
     if args.gpus != 1:
         print(f'bench.py measures one GPU: --gpus {args.gpus} is not supported by this build '
               '(every GPU box here has one GPU; the multi-GPU sharding is a design, not a run)', file=sys.stderr)

@@ -129,7 +129,6 @@ int ensure_wavefront(mcr_ctx *c, uint32_t capacity) {
   size_t o_q1 = take(sizeof(int32_t) * P);
   size_t o_counts = take(sizeof(uint32_t) * 8);
   size_t o_sref = take(sizeof(int32_t) * L);
-  size_t o_free = take(sizeof(int32_t) * P);
   HIP_TRY(hipMalloc(&c->w_block, bytes));
   char *b = (char *)c->w_block;
   Wavefront &w = c->w;
@@ -145,7 +144,6 @@ int ensure_wavefront(mcr_ctx *c, uint32_t capacity) {
   w.queue[1] = (int32_t *)(b + o_q1);
   w.counts = (uint32_t *)(b + o_counts);
   w.sref = (int32_t *)(b + o_sref);
-  w.free_list = (int32_t *)(b + o_free);
   c->capacity = capacity;
   return MCR_OK;
 }
@@ -545,16 +543,18 @@ static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const
   double t_closest = 0, t_any = 0, t_shade = 0;
   uint64_t n_closest = 0, n_any = 0;
   if (c->timing) HIP_TRY(hipEventRecord(c->ev[6], sm));
-  // Path regeneration: slots freed by finished samples are refilled with the next samples of the
-  // call, so the wavefront stays full across passes and tiles and only the very end of the call
-  // runs with few live paths.
+  // Path regeneration: resolve starts the call's next sample in every slot whose sample finished,
+  // so the wavefront stays full across passes and tiles and only the very end of the call runs
+  // with few live paths.
+  c->w.seed = st->seed;
   HIP_TRY(hipMemsetAsync(w.counts, 0, sizeof(uint32_t) * 8, sm));
-  uint64_t s_next = std::min<uint64_t>(c->capacity, total);
-  launch_raygen(sm, c->jobs, njobs, 0, (uint32_t)s_next, w, *camera, st->seed, 0, 0, 0);
+  uint32_t first_fill = (uint32_t)std::min<uint64_t>(c->capacity, total);
+  launch_raygen(sm, c->jobs, njobs, first_fill, w, *camera, st->seed);
   int q = 0;
-  uint32_t live = (uint32_t)s_next;
+  uint32_t live = first_fill;
   while (live > 0) {
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], sm));
+    HIP_TRY(hipMemsetAsync(w.counts + 6, 0, sizeof(uint32_t) * 2, sm));
     launch_extend(sm, s->d, w, q, std::min<uint32_t>(TRAVERSE_GRID, (live + 255) / 256), c->spill,
                   spill_stride, c->counters);
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], sm));
@@ -567,8 +567,7 @@ static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const
                     c->spill, spill_stride, c->seen, c->seen_meta, c->counters);
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], sm));
     HIP_TRY(hipMemsetAsync(w.counts + (q ^ 1), 0, sizeof(uint32_t), sm));
-    HIP_TRY(hipMemsetAsync(w.counts + 4, 0, sizeof(uint32_t), sm));
-    launch_resolve(sm, w, q, live, *st, s->d, c->jobs, njobs, dout);
+    launch_resolve(sm, w, q, live, *st, s->d, c->jobs, njobs, dout, *camera, (uint32_t)total);
     HIP_TRY(hipMemcpyAsync(c->counts_host, w.counts, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, sm));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], sm));
     HIP_TRY(hipStreamSynchronize(sm));
@@ -586,13 +585,6 @@ static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const
     if (c->counts_host[3])
       return fail(MCR_E_OVERFLOW, "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
     live = c->counts_host[q ^ 1];
-    uint32_t nfree = c->counts_host[4];
-    uint32_t refill = (uint32_t)std::min<uint64_t>(nfree, total - s_next);
-    if (refill) {
-      launch_raygen(sm, c->jobs, njobs, (uint32_t)s_next, refill, w, *camera, st->seed, q ^ 1, live, 1);
-      s_next += refill;
-      live += refill;
-    }
     q ^= 1;
   }
   if (c->timing) {

@@ -64,18 +64,10 @@ __device__ inline void unproject(const mcr_camera &c, double x, double y, double
 }
 
 // ---- ray generation: tracer.ts:62-80 + 705-713 --------------------------------------------------
-// Starts samples s0 .. s0+count-1 in path slots (slot i, or free_list[i] when slots are being
-// reused) and appends the slots to queue qn at position qbase. Samples keep their own seeds and
-// output positions, so the order in which slots are refilled never changes a result.
-__global__ __launch_bounds__(256) void k_raygen(const JobDev *jobs, uint32_t njobs, uint32_t s0,
-                                                uint32_t count, Wavefront w, mcr_camera cam,
-                                                uint32_t seed, int qn, uint32_t qbase, int use_free) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  if (i == 0) w.counts[qn] = qbase + count;
-  uint32_t slot = use_free ? (uint32_t)w.free_list[i] : i;
-  w.queue[qn][qbase + i] = (int32_t)slot;
-  uint32_t s = s0 + i;
+// Starts sample s (its pixel, pass and block from the job list) in path slot `slot`. A sample
+// carries its own seed and output position, so which slot runs it, and when, never changes a result.
+__device__ inline void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, const JobDev *jobs,
+                                    uint32_t njobs, const mcr_camera &cam, uint32_t seed) {
   const JobDev &j = jobs[find_job(jobs, njobs, s)];
   uint32_t li = s - j.sample_start;
   uint32_t x = (li % j.cols) * j.block, y = (li / j.cols) * j.block;
@@ -125,6 +117,27 @@ __global__ __launch_bounds__(256) void k_raygen(const JobDev *jobs, uint32_t njo
   iv[I_SAMPLE * P + slot] = (int32_t)s;
 }
 
+// The first `count` samples of a call, in slots 0..count-1, queued in slot order.
+__global__ __launch_bounds__(256) void k_raygen(const JobDev *jobs, uint32_t njobs, uint32_t count,
+                                                Wavefront w, mcr_camera cam, uint32_t seed) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  if (i == 0) {
+    w.counts[0] = count;
+    w.counts[5] = count;  // next sample to start
+  }
+  w.queue[0][i] = (int32_t)i;
+  start_sample(w, i, i, jobs, njobs, cam, seed);
+}
+
+// Next work index for every lane of the wave: one atomic per wave, 64 consecutive indices.
+__device__ inline uint32_t wave_fetch(uint32_t *counter) {
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64u);
+  base = __shfl(base, 0, 64);
+  return base + (threadIdx.x & 63);
+}
+
 // ---- extend: closest hit for every live slot (bvh.ts:202-282) ------------------------------------
 __global__ __launch_bounds__(256) void k_extend(DevScene sc, Wavefront w, int q, int32_t *spill,
                                                 uint32_t spill_stride, DevCounters *cnt) {
@@ -141,7 +154,11 @@ __global__ __launch_bounds__(256) void k_extend(DevScene sc, Wavefront w, int q,
   unsigned long long queries = 0;
   size_t P = w.capacity;
   const double *f = w.f;
-  for (uint32_t i = gid; i < n; i += nthreads) {
+  (void)nthreads;
+  // Waves take 64 rays at a time from a shared counter, so no wave idles while others still
+  // hold long traversals (a static split leaves the slowest waves running alone).
+  for (uint32_t i = wave_fetch(&w.counts[6]); __shfl(i, 0, 64) < n; i = wave_fetch(&w.counts[6])) {
+    if (i >= n) continue;
     int slot = w.queue[q][i];
     double ox = f[F_OX * P + slot], oy = f[F_OY * P + slot], oz = f[F_OZ * P + slot];
     double dx = f[F_DX * P + slot], dy = f[F_DY * P + slot], dz = f[F_DZ * P + slot];
@@ -692,7 +709,8 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int ca
   double *T = seen + gid;
   int2 *M = seen_meta + gid;
   const size_t S = nthreads;
-  for (uint32_t e = gid; e < n; e += nthreads) {
+  for (uint32_t e = wave_fetch(&w.counts[7]); __shfl(e, 0, 64) < n; e = wave_fetch(&w.counts[7])) {
+    if (e >= n) continue;
     int32_t ref = w.sref[e];
     int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
     size_t idx = (size_t)li * P + slot;
@@ -765,7 +783,8 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int ca
 
 // ---- resolve: finish the vertex's radiance in the reference's summation order ---------------------
 __global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_settings st, DevScene sc,
-                                                 const JobDev *jobs, uint32_t njobs, float *out) {
+                                                 const JobDev *jobs, uint32_t njobs, float *out,
+                                                 mcr_camera cam, uint32_t total) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = w.counts[q];
   bool live = gid < n;
@@ -794,11 +813,6 @@ __global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_setting
     f[F_LB * P + slot] += f[F_QB * P + slot] * (oB + f[F_EB * P + slot]);
   }
   bool requeue = mode == MODE_EXTEND || mode == MODE_ADD_EXTEND;
-  uint32_t pos = wave_append(&w.counts[q ^ 1], requeue ? 1u : 0u);
-  if (requeue) w.queue[q ^ 1][pos] = slot;
-  bool freed = live && !requeue;
-  uint32_t fpos = wave_append(&w.counts[4], freed ? 1u : 0u);
-  if (freed) w.free_list[fpos] = slot;
   if (mode == MODE_ADD_FINISH || mode == MODE_FINISH) {
     // tracer.ts:472-477: tone map, then fog, alpha 1.
     double c[4] = {f[F_LR * P + slot], f[F_LG * P + slot], f[F_LB * P + slot], 1};
@@ -810,6 +824,16 @@ __global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_setting
       for (int k = 0; k < 3; k++) c[k] = c[k] + (sc.fog_color[k] - c[k]) * fogf;
     write_sample(jobs, njobs, out, (uint32_t)w.i[I_SAMPLE * P + slot], c);
   }
+  // A finished slot starts the call's next sample at once, so the slot keeps its place in the
+  // queue and consecutive lanes keep reading consecutive slots.
+  bool finished = live && !requeue;
+  uint32_t next = wave_append(&w.counts[5], finished ? 1u : 0u);
+  if (finished && next < total) {
+    start_sample(w, (uint32_t)slot, next, jobs, njobs, cam, w.seed);
+    requeue = true;
+  }
+  uint32_t pos = wave_append(&w.counts[q ^ 1], requeue ? 1u : 0u);
+  if (requeue) w.queue[q ^ 1][pos] = slot;
 }
 
 // ---- ray queries over caller rays (bvh.ts:202-332) ---------------------------------------------
@@ -922,11 +946,9 @@ hipError_t mcr_upload_dfg(const float *lut512) {
   return hipMemcpyToSymbol(HIP_SYMBOL(g_dfg), lut512, 512 * sizeof(float));
 }
 
-void launch_raygen(hipStream_t s, const JobDev *jobs, uint32_t njobs, uint32_t s0, uint32_t count,
-                   const Wavefront &w, const mcr_camera &cam, uint32_t seed, int qn, uint32_t qbase,
-                   int use_free) {
-  hipLaunchKernelGGL(k_raygen, dim3(blocks_for(count)), dim3(256), 0, s, jobs, njobs, s0, count, w,
-                     cam, seed, qn, qbase, use_free);
+void launch_raygen(hipStream_t s, const JobDev *jobs, uint32_t njobs, uint32_t count,
+                   const Wavefront &w, const mcr_camera &cam, uint32_t seed) {
+  hipLaunchKernelGGL(k_raygen, dim3(blocks_for(count)), dim3(256), 0, s, jobs, njobs, count, w, cam, seed);
 }
 void launch_extend(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
@@ -946,9 +968,9 @@ void launch_shadow(hipStream_t s, const DevScene &sc, const Wavefront &w, int ca
 }
 void launch_resolve(hipStream_t s, const Wavefront &w, int q, uint32_t max_live,
                     const mcr_settings &st, const DevScene &sc, const JobDev *jobs, uint32_t njobs,
-                    float *out) {
+                    float *out, const mcr_camera &cam, uint32_t total) {
   hipLaunchKernelGGL(k_resolve, dim3(blocks_for(max_live)), dim3(256), 0, s, w, q, st, sc, jobs,
-                     njobs, out);
+                     njobs, out, cam, total);
 }
 void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                           double tmin, double tmax, int cull, int32_t *tri, double *t, double *u,

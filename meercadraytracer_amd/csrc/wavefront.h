@@ -59,9 +59,9 @@ struct Wavefront {
   // queues
   int32_t *queue[2];         // live slots
   uint32_t *counts;          // [0],[1] queue lengths, [2] shadow queue length, [3] overflow flag,
-                             // [4] free-slot count
+                             // [5] next sample to start, [6],[7] extend / shadow work counters
   int32_t *sref;             // shadow queue: slot * MCR_MAX_LIGHTS + light
-  int32_t *free_list;        // slots whose sample finished this iteration
+  uint32_t seed;             // settings.seed of the call (random.ts:6 mixes it into every seed)
 };
 
 struct JobDev {  // one tile job, device side
