@@ -62,6 +62,58 @@ def cpu_baseline(sc, cam, st, width, height, seconds=15.0):
                       f'{rays} queries in {secs:.2f} s (portable-math oracle, std::thread pool)'}
 
 
+KERNEL_SYMBOLS = {'extend (closest hit)': 'k_extend', 'shadow (any hit)': 'k_shadow'}
+
+
+def _pmc_run(counter, bench_args, outdir):
+    """One rocprofv3 counter pass over a child bench run; returns the per-dispatch values of
+    every kernel, in dispatch order, as {kernel symbol: [KiB, ...]}."""
+    import csv
+    import subprocess
+    d = os.path.join(outdir, counter)
+    cmd = ['rocprofv3', '--pmc', counter, '--output-format', 'csv', '-d', d, '-o', 'pmc', '--',
+           sys.executable, os.path.abspath(__file__), *bench_args]
+    env = dict(os.environ, TMPDIR='/tmp')
+    subprocess.run(cmd, cwd='/tmp', env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                   timeout=240, check=True)
+    rows = []
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith('counter_collection.csv'):
+                rows += list(csv.DictReader(open(os.path.join(root, f))))
+    rows.sort(key=lambda r: int(r['Dispatch_Id']))
+    out = {}
+    for r in rows:
+        out.setdefault(r['Kernel_Name'].split('(')[0], []).append(float(r['Counter_Value']))
+    return out
+
+
+def measure_traffic(args, kernel, launches):
+    """HBM bytes per launch of `kernel` over the timed launches of the same bench command, from
+    FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a 4 GiB device copy."""
+    import shutil
+    import tempfile
+    if not shutil.which('rocprofv3') or launches == 0:
+        return None
+    outdir = tempfile.mkdtemp(prefix='mcr_pmc_', dir='/tmp')
+    bench_args = ['--steps', str(args.steps), '--warmup', str(args.warmup), '--no-cpu', '--no-traffic',
+                  '--width', str(args.width), '--height', str(args.height)]
+    copy_args = ['--copy-calibration', '4096']
+    res = {}
+    for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
+        copy = _pmc_run(counter, copy_args, os.path.join(outdir, 'copy'))
+        copied_kib = sum(copy.get('__amd_rocclr_copyBuffer', []))
+        factor = (4096 * 1024) / copied_kib if copied_kib else None
+        vals = _pmc_run(counter, bench_args, outdir).get(KERNEL_SYMBOLS[kernel], [])[-launches:]
+        if not factor or len(vals) < launches:
+            return None
+        res[counter] = (sum(vals) * factor * 1024 / launches, factor)
+    shutil.rmtree(outdir, ignore_errors=True)
+    return {'bytes_per_launch': int(res['FETCH_SIZE'][0] + res['WRITE_SIZE'][0]),
+            'read_bytes_per_launch': int(res['FETCH_SIZE'][0]), 'write_bytes_per_launch': int(res['WRITE_SIZE'][0]),
+            'fetch_scale': round(res['FETCH_SIZE'][1], 4), 'write_scale': round(res['WRITE_SIZE'][1], 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -69,6 +121,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-traffic', action='store_true', help='skip the rocprofv3 HBM-traffic passes')
     ap.add_argument('--width', type=int, default=1920)
     ap.add_argument('--height', type=int, default=1080)
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -169,6 +222,16 @@ def main():
                    'bvh_build_ms_incl_copies': round(build_times[0] * 1e3, 1)},
         'roofline': roofline,
     }
+    if not args.no_traffic:
+        log('HBM traffic passes (rocprofv3 FETCH_SIZE / WRITE_SIZE) ...')
+        try:
+            t = measure_traffic(args, name, launches)
+        except Exception as e:  # the counters are a report, never a reason to fail the bench
+            log(f'traffic measurement failed: {e}')
+            t = None
+        if t:
+            roofline['traffic'] = t['bytes_per_launch']
+            roofline['traffic_detail'] = t
     if not args.no_cpu:
         log('cpu baseline ...')
         result['cpu_baseline'] = cpu_baseline(sc, cam, st, W, H, args.cpu_seconds)
