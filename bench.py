@@ -137,8 +137,8 @@ def main():
         print(json.dumps({'copy_bytes': n}), flush=True)
         return
     if args.gpus != 1:
-        print(f'bench.py measures one GPU: --gpus {args.gpus} is not supported by this build '
-              '(every GPU box here has one GPU; the multi-GPU sharding is a design, not a run)', file=sys.stderr)
+        print(f'bench.py measures one GPU: --gpus {args.gpus} is not supported '
+              '(tiles shard across GPUs by the scheduler; only the one-GPU path is measured)', file=sys.stderr)
         sys.exit(2)
 
     from meercadraytracer_amd import abi, native, scenes

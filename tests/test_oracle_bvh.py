@@ -95,8 +95,7 @@ def test_empty_scene():  # bvh.test.ts:144-148
 
 def test_small_build_known_layout():
     """Hand-derived: five unit triangles spaced along x split 2|3 at the root (the SAH cost of
-    the 2|3 split, 2*A(2) + 3*A(3), beats 1|4, 3|2 and 4|1); the right child is numbered first in
-    the reference's LIFO order, so it gets node 1... no: children are allocated as a pair
+    the 2|3 split, 2*A(2) + 3*A(3), beats 1|4, 3|2 and 4|1); children are allocated as a pair
     (left = 1, right = 2) and both are leaves (n <= 4)."""
     tris = []
     for i in range(5):
