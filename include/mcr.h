@@ -207,6 +207,20 @@ int mcr_render_tiles(mcr_ctx *ctx, mcr_scene *scene, const mcr_camera *camera,
                      const mcr_settings *settings, const mcr_tile_job *jobs, uint32_t job_count,
                      float *out, int out_device, mcr_counters *counters);
 
+/* ---- batched form: tile jobs of many scenes, cameras and settings in one wavefront run ---- */
+/* No counterpart call in the reference, which renders one tile per Tracer.renderTile call
+ * (src/tracer.ts:62): this is that call for a queue of tiles from many requests at once
+ * (src/progressive.ts:227-271 spreads them over workers instead). Every job names a view;
+ * job_views may be NULL (every job view 0). Results equal one mcr_render_tiles call per view. */
+typedef struct mcr_view {
+  mcr_scene *scene;
+  const mcr_camera *camera;
+  const mcr_settings *settings;
+} mcr_view;
+int mcr_render_batch(mcr_ctx *ctx, const mcr_view *views, uint32_t view_count,
+                     const mcr_tile_job *jobs, const uint32_t *job_views, uint32_t job_count,
+                     float *out, int out_device, mcr_counters *counters);
+
 /* ---- accumulate + encode: replaces writeTile/encodeTile (src/progressive.ts:288-313) ------- */
 /* sum may be NULL (preview tiles). image is width*height*4 bytes, straight alpha. */
 int mcr_encode_tile(mcr_ctx *ctx, uint8_t *image, uint32_t width, mcr_tile tile,
@@ -242,6 +256,9 @@ int mcr_set_timing(mcr_ctx *ctx, int enabled);
 /* Path slots of the wavefront (samples in flight). Default 2^21; at least 256. Samples beyond the
  * capacity start as earlier ones finish, so the capacity changes memory use, not results. */
 int mcr_set_wavefront_capacity(mcr_ctx *ctx, uint32_t slots);
+/* Replays bounce iterations from a captured HIP graph (default on; off launches each kernel with
+ * grids sized to the live paths). Timing mode always launches directly. Results are the same. */
+int mcr_set_graph(mcr_ctx *ctx, int enabled);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,10 @@ class TileJob(C.Structure):
     ]
 
 
+class View(C.Structure):
+    _fields_ = [('scene', C.c_void_p), ('camera', C.c_void_p), ('settings', C.c_void_p)]
+
+
 class Counters(C.Structure):
     _fields_ = [
         ('closest_queries', C.c_uint64), ('any_queries', C.c_uint64),

@@ -11,6 +11,7 @@
 
 #include "../../include/mcr.h"
 #include "builder.h"
+#include "context.h"
 #include "device.h"
 #include "dfg_lut.h"
 #include "kernels.h"
@@ -18,77 +19,17 @@
 #include "wavefront.h"
 
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
 
-int fail(int code, const std::string &msg) {
+int mcr_fail(int code, const std::string &msg) {
   g_error = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(MCR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-  } while (0)
-
-template <class T>
-hipError_t dalloc(T **p, size_t count) {
-  if (count == 0) count = 1;
-  return hipMalloc((void **)p, count * sizeof(T));
-}
-
-// Grid of persistent traversal blocks: enough waves to fill 256 CUs several times over.
-const uint32_t TRAVERSE_GRID = 256 * 8;
-const uint32_t TRAVERSE_THREADS = TRAVERSE_GRID * 256;
-const uint32_t DEFAULT_CAPACITY = 1u << 21;  // path slots per wavefront chunk
-
-}  // namespace
-
-struct mcr_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  // wavefront
-  Wavefront w = {};
-  void *w_block = nullptr;
-  uint32_t capacity = 0;
-  uint32_t capacity_limit = DEFAULT_CAPACITY;
-  // traversal scratch
-  int32_t *spill = nullptr;
-  double *seen = nullptr;
-  int2 *seen_meta = nullptr;
-  DevCounters *counters = nullptr;
-  JobDev *jobs = nullptr;
-  uint32_t jobs_cap = 0;
-  float *out_tmp = nullptr;
-  size_t out_tmp_cap = 0;
-  uint32_t *counts_host = nullptr;  // pinned
-  // builder workspace
-  BuilderWorkspace builder;
-  // timing
-  bool timing = false;
-  mcr_timing last_timing = {};
-  hipEvent_t ev[8] = {};
-  std::mutex mu;
-};
-
-struct mcr_scene {
-  mcr_ctx *ctx = nullptr;
-  DevScene d = {};
-  uint32_t geometry_version = 0, shading_version = 0;
-  bool has_geometry = false, has_shading = false;
-  float4 *tri = nullptr;
-  float4 *inner = nullptr;
-  int2 *longleaf = nullptr;
-  float *normals = nullptr;
-  float *colors = nullptr;
-  DevMaterial *materials = nullptr;
-  DevLight *lights = nullptr;
-  uint64_t bytes_geometry = 0, bytes_shading = 0;
-};
-
 namespace {
+
+int fail(int code, const std::string &msg) { return mcr_fail(code, msg); }
 
 void free_geometry(mcr_scene *s) {
   hipFree(s->tri);
@@ -103,59 +44,6 @@ void free_geometry(mcr_scene *s) {
   s->colors = nullptr;
   s->bytes_geometry = 0;
   s->has_geometry = false;
-}
-
-int ensure_wavefront(mcr_ctx *c, uint32_t capacity) {
-  if (c->capacity >= capacity) return MCR_OK;
-  hipFree(c->w_block);
-  c->w_block = nullptr;
-  c->capacity = 0;
-  size_t P = capacity;
-  size_t L = (size_t)MCR_MAX_LIGHTS * P;
-  size_t bytes = 0;
-  auto take = [&](size_t n, size_t align = 256) {
-    size_t off = (bytes + align - 1) / align * align;
-    bytes = off + n;
-    return off;
-  };
-  size_t o_f = take(sizeof(double) * F_COUNT * P);
-  size_t o_i = take(sizeof(int32_t) * I_COUNT * P);
-  size_t o_mode = take(P);
-  size_t o_contrib = take(sizeof(double) * 3 * L);
-  size_t o_shadow = take(sizeof(double) * 3 * L);
-  size_t o_sray = take(sizeof(double) * 7 * L);
-  size_t o_status = take(L);
-  size_t o_q0 = take(sizeof(int32_t) * P);
-  size_t o_q1 = take(sizeof(int32_t) * P);
-  size_t o_counts = take(sizeof(uint32_t) * 8);
-  size_t o_sref = take(sizeof(int32_t) * L);
-  HIP_TRY(hipMalloc(&c->w_block, bytes));
-  char *b = (char *)c->w_block;
-  Wavefront &w = c->w;
-  w.capacity = capacity;
-  w.f = (double *)(b + o_f);
-  w.i = (int32_t *)(b + o_i);
-  w.mode = (uint8_t *)(b + o_mode);
-  w.contrib = (double *)(b + o_contrib);
-  w.shadow = (double *)(b + o_shadow);
-  w.sray = (double *)(b + o_sray);
-  w.status = (uint8_t *)(b + o_status);
-  w.queue[0] = (int32_t *)(b + o_q0);
-  w.queue[1] = (int32_t *)(b + o_q1);
-  w.counts = (uint32_t *)(b + o_counts);
-  w.sref = (int32_t *)(b + o_sref);
-  c->capacity = capacity;
-  return MCR_OK;
-}
-
-int ensure_jobs(mcr_ctx *c, uint32_t n) {
-  if (c->jobs_cap >= n) return MCR_OK;
-  hipFree(c->jobs);
-  c->jobs = nullptr;
-  c->jobs_cap = 0;
-  HIP_TRY(dalloc(&c->jobs, n));
-  c->jobs_cap = n;
-  return MCR_OK;
 }
 
 int ensure_out_tmp(mcr_ctx *c, size_t floats) {
@@ -242,15 +130,15 @@ int mcr_create(int device, mcr_ctx **out) {
   float lut[512];
   for (int i = 0; i < 512; i++) lut[i] = half_to_float_js(MCR_DFG_LUT_HALF[i]);
   e = mcr_upload_dfg(lut);
-  if (e == hipSuccess) e = hipMalloc(&c->spill, sizeof(int32_t) * (size_t)(MCR_STACK_MAX - MCR_STACK_LDS) * TRAVERSE_THREADS);
-  if (e == hipSuccess) e = hipMalloc(&c->seen, sizeof(double) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS);
-  if (e == hipSuccess) e = hipMalloc(&c->seen_meta, sizeof(int2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS);
-  if (e == hipSuccess) e = hipMalloc(&c->counters, sizeof(DevCounters));
-  if (e == hipSuccess) e = hipHostMalloc(&c->counts_host, sizeof(uint32_t) * 8, hipHostMallocDefault);
-  for (int i = 0; i < 8 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
   if (e != hipSuccess) {
     mcr_destroy(c);
     return fail(MCR_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  int rc = engine_init(c->eng, device, c->stream);
+  if (rc) {
+    std::string msg = g_error;
+    mcr_destroy(c);
+    return fail(rc, msg);
   }
   *out = c;
   return MCR_OK;
@@ -260,17 +148,9 @@ void mcr_destroy(mcr_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  hipFree(c->w_block);
-  hipFree(c->spill);
-  hipFree(c->seen);
-  hipFree(c->seen_meta);
-  hipFree(c->counters);
-  hipFree(c->jobs);
+  engine_free(c->eng);
   hipFree(c->out_tmp);
-  if (c->counts_host) hipHostFree(c->counts_host);
   builder_free(c->builder);
-  for (auto &e : c->ev)
-    if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -281,24 +161,34 @@ int mcr_set_wavefront_capacity(mcr_ctx *c, uint32_t slots) {
   std::lock_guard<std::mutex> lock(c->mu);
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  c->capacity_limit = slots;
-  if (c->capacity > slots) {
-    hipFree(c->w_block);
-    c->w_block = nullptr;
-    c->capacity = 0;
+  Engine &e = c->eng;
+  e.capacity_limit = slots;
+  if (e.capacity > slots) {
+    if (e.graph) hipGraphExecDestroy(e.graph);
+    e.graph = nullptr;
+    hipFree(e.w_block);
+    e.w_block = nullptr;
+    e.capacity = 0;
   }
   return MCR_OK;
 }
 
 int mcr_set_timing(mcr_ctx *c, int enabled) {
   if (!c) return fail(MCR_E_INVALID, "ctx is null");
-  c->timing = enabled != 0;
+  c->eng.timing = enabled != 0;
+  return MCR_OK;
+}
+
+int mcr_set_graph(mcr_ctx *c, int enabled) {
+  if (!c) return fail(MCR_E_INVALID, "ctx is null");
+  std::lock_guard<std::mutex> lock(c->mu);
+  c->eng.use_graph = enabled != 0;
   return MCR_OK;
 }
 
 int mcr_last_timing(mcr_ctx *c, mcr_timing *out) {
   if (!c || !out) return fail(MCR_E_INVALID, "null argument");
-  *out = c->last_timing;
+  *out = c->eng.last_timing;
   return MCR_OK;
 }
 
@@ -508,115 +398,66 @@ int mcr_scene_set_shading(mcr_scene *s, const mcr_shading *sh, uint32_t version)
   return MCR_OK;
 }
 
+}  // extern "C"
+
 // ---- rendering --------------------------------------------------------------------------------
-static int render_jobs(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const mcr_settings *st,
-                       const mcr_tile_job *jobs, uint32_t njobs, float *dout, mcr_counters *counters) {
-  // Validate and lay out the jobs: sample ranges and per-job columns.
+ViewDev make_view(const mcr_scene *s, const mcr_camera &cam, const mcr_settings &st) {
+  ViewDev v;
+  memset(&v, 0, sizeof(v));
+  v.sc = s->d;
+  v.cam = cam;
+  v.st = st;
+  v.light_slots = std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
+  return v;
+}
+
+int64_t layout_job(const mcr_tile_job &j, uint32_t index, uint32_t view, uint64_t start, JobDev &out) {
+  uint32_t block = j.block < 1 ? 1 : j.block;
+  if (j.tile.width <= 0 || j.tile.height <= 0 || j.tile.x < 0 || j.tile.y < 0 ||
+      (uint64_t)j.tile.x + j.tile.width > j.width || (uint64_t)j.tile.y + j.tile.height > j.height) {
+    mcr_fail(MCR_E_INVALID, "tile " + std::to_string(index) + " lies outside its image");
+    return -1;
+  }
+  if (j.out_offset % 4) {
+    mcr_fail(MCR_E_INVALID, "out_offset must be a multiple of 4 floats");
+    return -1;
+  }
+  uint32_t cols = (j.tile.width + block - 1) / block, rows = (j.tile.height + block - 1) / block;
+  out = JobDev{j.tile.x, j.tile.y, j.tile.width, j.tile.height, j.width, j.height, j.pass, block,
+               j.out_offset, (uint32_t)start, cols, view, 0};
+  return (int64_t)cols * rows;
+}
+
+namespace {
+
+// Views and jobs of one call, laid out in sample order.
+int render_views(mcr_ctx *c, const mcr_view *views, uint32_t nviews, const mcr_tile_job *jobs,
+                 const uint32_t *job_views, uint32_t njobs, float *dout, mcr_counters *counters) {
+  std::vector<ViewDev> vd(nviews);
+  for (uint32_t v = 0; v < nviews; v++) vd[v] = make_view(views[v].scene, *views[v].camera, *views[v].settings);
   std::vector<JobDev> jd(njobs);
   uint64_t total = 0;
   for (uint32_t i = 0; i < njobs; i++) {
-    const mcr_tile_job &j = jobs[i];
-    uint32_t block = j.block < 1 ? 1 : j.block;
-    if (j.tile.width <= 0 || j.tile.height <= 0 || j.tile.x < 0 || j.tile.y < 0 ||
-        (uint64_t)j.tile.x + j.tile.width > j.width || (uint64_t)j.tile.y + j.tile.height > j.height)
-      return fail(MCR_E_INVALID, "tile " + std::to_string(i) + " lies outside its image");
-    if (j.out_offset % 4) return fail(MCR_E_INVALID, "out_offset must be a multiple of 4 floats");
-    uint32_t cols = (j.tile.width + block - 1) / block, rows = (j.tile.height + block - 1) / block;
-    jd[i] = JobDev{j.tile.x, j.tile.y, j.tile.width, j.tile.height, j.width, j.height, j.pass, block,
-                   j.out_offset, (uint32_t)total, cols};
-    total += (uint64_t)cols * rows;
+    uint32_t view = job_views ? job_views[i] : 0;
+    if (view >= nviews) return fail(MCR_E_INVALID, "job " + std::to_string(i) + " names no view");
+    int64_t n = layout_job(jobs[i], i, view, total, jd[i]);
+    if (n < 0) return MCR_E_INVALID;
+    total += (uint64_t)n;
+    if (total >= (1ull << 31)) return fail(MCR_E_OVERFLOW, "too many samples in one call");
   }
-  if (total >= (1ull << 31)) return fail(MCR_E_OVERFLOW, "too many samples in one call");
-  if (njobs == 0 || total == 0) return MCR_OK;
-  int rc = ensure_jobs(c, njobs);
-  if (rc) return rc;
-  uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->capacity_limit);
-  cap = std::max<uint32_t>(cap, 256);
-  rc = ensure_wavefront(c, std::min(std::max(cap, c->capacity), c->capacity_limit));
-  if (rc) return rc;
-  hipStream_t sm = c->stream;
-  HIP_TRY(hipMemcpyAsync(c->jobs, jd.data(), njobs * sizeof(JobDev), hipMemcpyHostToDevice, sm));
-  HIP_TRY(hipMemsetAsync(c->counters, 0, sizeof(DevCounters), sm));
-  const Wavefront &w = c->w;
-  int cast_only = st->shadows == MCR_SHADOWS_CAST;
-  uint32_t spill_stride = TRAVERSE_THREADS;
-  double t_closest = 0, t_any = 0, t_shade = 0;
-  uint64_t n_closest = 0, n_any = 0;
-  if (c->timing) HIP_TRY(hipEventRecord(c->ev[6], sm));
-  // Path regeneration: resolve starts the call's next sample in every slot whose sample finished,
-  // so the wavefront stays full across passes and tiles and only the very end of the call runs
-  // with few live paths.
-  c->w.seed = st->seed;
-  HIP_TRY(hipMemsetAsync(w.counts, 0, sizeof(uint32_t) * 8, sm));
-  uint32_t first_fill = (uint32_t)std::min<uint64_t>(c->capacity, total);
-  launch_raygen(sm, c->jobs, njobs, first_fill, w, *camera, st->seed);
-  int q = 0;
-  uint32_t live = first_fill;
-  while (live > 0) {
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], sm));
-    HIP_TRY(hipMemsetAsync(w.counts + 6, 0, sizeof(uint32_t) * 2, sm));
-    launch_extend(sm, s->d, w, q, std::min<uint32_t>(TRAVERSE_GRID, (live + 255) / 256), c->spill,
-                  spill_stride, c->counters);
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], sm));
-    HIP_TRY(hipMemsetAsync(w.counts + 2, 0, sizeof(uint32_t), sm));
-    launch_shade(sm, s->d, w, q, live, *st, *camera, c->jobs, njobs, dout);
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], sm));
-    uint32_t max_shadow = live * std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
-    if (max_shadow)
-      launch_shadow(sm, s->d, w, cast_only, std::min<uint32_t>(TRAVERSE_GRID, (max_shadow + 255) / 256),
-                    c->spill, spill_stride, c->seen, c->seen_meta, c->counters);
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], sm));
-    HIP_TRY(hipMemsetAsync(w.counts + (q ^ 1), 0, sizeof(uint32_t), sm));
-    launch_resolve(sm, w, q, live, *st, s->d, c->jobs, njobs, dout, *camera, (uint32_t)total);
-    HIP_TRY(hipMemcpyAsync(c->counts_host, w.counts, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, sm));
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], sm));
-    HIP_TRY(hipStreamSynchronize(sm));
-    if (c->timing) {
-      float a, b, d;
-      hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-      hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-      hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
-      t_closest += a;
-      t_shade += b;
-      t_any += d;
-      n_closest++;
-      if (max_shadow) n_any++;
-    }
-    if (c->counts_host[3])
-      return fail(MCR_E_OVERFLOW, "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
-    live = c->counts_host[q ^ 1];
-    q ^= 1;
-  }
-  if (c->timing) {
-    HIP_TRY(hipEventRecord(c->ev[7], sm));
-    HIP_TRY(hipEventSynchronize(c->ev[7]));
-    float tot;
-    hipEventElapsedTime(&tot, c->ev[6], c->ev[7]);
-    c->last_timing = mcr_timing{t_closest, t_any, t_shade, tot - t_closest - t_any - t_shade, tot, n_closest, n_any};
-  }
-  if (counters) {
-    DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, c->counters, sizeof(hc), hipMemcpyDeviceToHost));
-    counters->closest_queries += hc.closest_queries;
-    counters->any_queries += hc.any_queries;
-    counters->node_visits += hc.node_visits;
-    counters->leaf_visits += hc.leaf_visits;
-    counters->triangle_tests += hc.triangle_tests;
-    counters->paths += total;
-    counters->closest_node_visits += hc.closest_node_visits;
-    counters->closest_triangle_tests += hc.closest_triangle_tests;
-    counters->any_node_visits += hc.any_node_visits;
-    counters->any_triangle_tests += hc.any_triangle_tests;
-  }
+  return engine_run(c->eng, vd, jd, (uint32_t)total, dout, counters);
+}
+
+int check_view(const mcr_view &v, uint32_t i) {
+  if (!v.scene || !v.camera || !v.settings) return fail(MCR_E_INVALID, "view " + std::to_string(i) + " has a null field");
+  if (!v.scene->has_geometry || !v.scene->has_shading)
+    return fail(MCR_E_NO_SCENE, "the tile refers to a scene this worker has not received");
   return MCR_OK;
 }
 
-int mcr_render_tiles(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const mcr_settings *st,
-                     const mcr_tile_job *jobs, uint32_t njobs, float *out, int out_device,
-                     mcr_counters *counters) {
-  if (!c || !s || !camera || !st || (!jobs && njobs) || !out) return fail(MCR_E_INVALID, "null argument");
-  if (!s->has_geometry || !s->has_shading)
-    return fail(MCR_E_NO_SCENE, "the tile refers to a scene this worker has not received");
+int render_common(mcr_ctx *c, const mcr_view *views, uint32_t nviews, const mcr_tile_job *jobs,
+                  const uint32_t *job_views, uint32_t njobs, float *out, int out_device,
+                  mcr_counters *counters) {
   std::lock_guard<std::mutex> lock(c->mu);
   HIP_TRY(hipSetDevice(c->device));
   uint64_t floats = 0;
@@ -624,16 +465,36 @@ int mcr_render_tiles(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const m
     floats = std::max<uint64_t>(floats, jobs[i].out_offset + (uint64_t)jobs[i].tile.width * jobs[i].tile.height * 4);
   float *dout = out;
   if (!out_device) {
-    int rc = ensure_out_tmp(c, floats);
-    if (rc) return rc;
+    MCR_TRY(ensure_out_tmp(c, floats));
     dout = c->out_tmp;
   }
-  int rc = render_jobs(c, s, camera, st, jobs, njobs, dout, counters);
-  if (rc) return rc;
+  MCR_TRY(render_views(c, views, nviews, jobs, job_views, njobs, dout, counters));
   if (!out_device && floats)
     HIP_TRY(hipMemcpyAsync(out, dout, floats * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcr_render_tiles(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const mcr_settings *st,
+                     const mcr_tile_job *jobs, uint32_t njobs, float *out, int out_device,
+                     mcr_counters *counters) {
+  if (!c || !s || !camera || !st || (!jobs && njobs) || !out) return fail(MCR_E_INVALID, "null argument");
+  mcr_view v{s, camera, st};
+  MCR_TRY(check_view(v, 0));
+  return render_common(c, &v, 1, jobs, nullptr, njobs, out, out_device, counters);
+}
+
+int mcr_render_batch(mcr_ctx *c, const mcr_view *views, uint32_t nviews, const mcr_tile_job *jobs,
+                     const uint32_t *job_views, uint32_t njobs, float *out, int out_device,
+                     mcr_counters *counters) {
+  if (!c || (!views && nviews) || (!jobs && njobs) || !out) return fail(MCR_E_INVALID, "null argument");
+  if (njobs && !nviews) return fail(MCR_E_INVALID, "jobs without views");
+  for (uint32_t i = 0; i < nviews; i++) MCR_TRY(check_view(views[i], i));
+  return render_common(c, views, nviews, jobs, job_views, njobs, out, out_device, counters);
 }
 
 int mcr_render_tile(mcr_ctx *c, mcr_scene *s, uint32_t geometry_version, uint32_t shading_version,
@@ -695,12 +556,12 @@ int mcr_intersect_closest(mcr_ctx *c, mcr_scene *s, const double *rays, uint32_t
     HIP_TRY(hipMemcpyAsync(r, rays, (size_t)n * 48, hipMemcpyHostToDevice, sm));
     drays = r;
   }
-  HIP_TRY(hipMemsetAsync(c->counters, 0, sizeof(DevCounters), sm));
-  if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], sm));
+  HIP_TRY(hipMemsetAsync(c->eng.counters, 0, sizeof(DevCounters), sm));
+  if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[0], sm));
   launch_probe_closest(sm, s->d, drays, n, t_min, t_max, cull, dtri, dt, du, dv, dfront,
-                       std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->spill, TRAVERSE_THREADS,
-                       c->counters);
-  if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], sm));
+                       std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->eng.spill, TRAVERSE_THREADS,
+                       c->eng.counters);
+  if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[1], sm));
   HIP_TRY(hipGetLastError());
   if (!device_ptrs) {
     if (t_out) HIP_TRY(hipMemcpyAsync(t_out, dt, (size_t)n * 8, hipMemcpyDeviceToHost, sm));
@@ -710,15 +571,15 @@ int mcr_intersect_closest(mcr_ctx *c, mcr_scene *s, const double *rays, uint32_t
     if (front_out) HIP_TRY(hipMemcpyAsync(front_out, dfront, n, hipMemcpyDeviceToHost, sm));
   }
   HIP_TRY(hipStreamSynchronize(sm));
-  if (c->timing) {
+  if (c->eng.timing) {
     float a;
-    hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-    c->last_timing = mcr_timing{a, 0, 0, 0, a, 1, 0};
+    hipEventElapsedTime(&a, c->eng.ev[0], c->eng.ev[1]);
+    c->eng.last_timing = mcr_timing{a, 0, 0, 0, a, 1, 0};
   }
   if (tmp) hipFree(tmp);
   if (counters) {
     DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, c->counters, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&hc, c->eng.counters, sizeof(hc), hipMemcpyDeviceToHost));
     counters->closest_queries += hc.closest_queries;
     counters->node_visits += hc.node_visits;
     counters->leaf_visits += hc.leaf_visits;
@@ -745,17 +606,17 @@ int mcr_intersect_any_count(mcr_ctx *c, mcr_scene *s, const double *rays, uint32
     HIP_TRY(hipMemcpyAsync(r, rays, (size_t)n * 48, hipMemcpyHostToDevice, sm));
     drays = r;
   }
-  HIP_TRY(hipMemsetAsync(c->counters, 0, sizeof(DevCounters), sm));
+  HIP_TRY(hipMemsetAsync(c->eng.counters, 0, sizeof(DevCounters), sm));
   launch_probe_any_count(sm, s->d, drays, n, t_min, t_max, dcount,
-                         std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->spill, TRAVERSE_THREADS,
-                         c->counters);
+                         std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->eng.spill, TRAVERSE_THREADS,
+                         c->eng.counters);
   HIP_TRY(hipGetLastError());
   if (!device_ptrs) HIP_TRY(hipMemcpyAsync(count_out, dcount, (size_t)n * 4, hipMemcpyDeviceToHost, sm));
   HIP_TRY(hipStreamSynchronize(sm));
   if (tmp) hipFree(tmp);
   if (counters) {
     DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, c->counters, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&hc, c->eng.counters, sizeof(hc), hipMemcpyDeviceToHost));
     counters->any_queries += hc.any_queries;
     counters->node_visits += hc.node_visits;
     counters->leaf_visits += hc.leaf_visits;

@@ -1,0 +1,132 @@
+// context.h — host-side state shared by the C-ABI entry points: the error slot, the wavefront
+// engine (path state, views, jobs, scratch, captured iterations) and the context / scene objects.
+#ifndef MCR_CONTEXT_H
+#define MCR_CONTEXT_H
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mcr.h"
+#include "builder.h"
+#include "device.h"
+#include "wavefront.h"
+
+// Records msg as this thread's last error (mcr_last_error) and returns code.
+int mcr_fail(int code, const std::string &msg);
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return mcr_fail(MCR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define MCR_TRY(expr)      \
+  do {                     \
+    int rc_ = (expr);      \
+    if (rc_) return rc_;   \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+
+// Grid of persistent traversal blocks: enough waves to fill 256 CUs several times over.
+const uint32_t TRAVERSE_GRID = 256 * 8;
+const uint32_t TRAVERSE_THREADS = TRAVERSE_GRID * 256;
+const uint32_t DEFAULT_CAPACITY = 1u << 21;  // path slots
+
+// One wavefront engine: the path slots and everything its kernels read and write. A context owns
+// one for its synchronous calls; a server owns another for its request loop.
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Wavefront w = {};
+  void *w_block = nullptr;
+  uint32_t capacity = 0;
+  uint32_t capacity_limit = DEFAULT_CAPACITY;
+  int32_t *spill = nullptr;        // traversal stack overflow, per traversal thread
+  double *seen = nullptr;          // see-through shadow hits, per traversal thread
+  int2 *seen_meta = nullptr;
+  DevCounters *counters = nullptr;
+  ViewDev *views = nullptr;
+  uint32_t views_cap = 0;
+  JobDev *jobs = nullptr;
+  uint32_t jobs_cap = 0;
+  uint32_t *job_done = nullptr;
+  uint32_t job_done_cap = 0;
+  Control *ctl = nullptr;          // device
+  Control *ctl_host = nullptr;     // pinned staging
+  uint32_t *counts_host = nullptr; // pinned
+  bool mix = false;                // the run holds more than one view
+  // Two captured bounce iterations (queue 0 then queue 1), replayed while paths are live.
+  bool use_graph = true;
+  hipGraphExec_t graph = nullptr;
+  const void *graph_key[4] = {};
+  // timing
+  bool timing = false;
+  mcr_timing last_timing = {};
+  hipEvent_t ev[8] = {};
+};
+
+int engine_init(Engine &e, int device, hipStream_t stream);
+void engine_free(Engine &e);
+// Path slots, views and jobs the next run needs (buffers only grow; the capacity is clamped).
+int engine_reserve(Engine &e, uint64_t samples, uint32_t nviews, uint32_t njobs, bool track_done);
+// Host arrays must stay alive and unchanged until the next engine_sync.
+int engine_upload_views(Engine &e, const ViewDev *views, uint32_t n);
+int engine_upload_jobs(Engine &e, const JobDev *jobs, uint32_t first, uint32_t n);
+int engine_set_control(Engine &e, float *out, uint32_t njobs, uint32_t total, bool track_done);
+int engine_zero_done(Engine &e, uint32_t first, uint32_t n);
+// Every slot free, every counter zero.
+int engine_reset(Engine &e);
+// Starts `count` samples from `next` in free slots; see k_refill.
+int engine_refill(Engine &e, int q, uint32_t count, uint32_t free_top, uint32_t next, uint32_t queued);
+// One bounce iteration over the `live` paths of queue q (grids sized from live).
+int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, double *t_closest,
+                     double *t_shade, double *t_any);
+// Two iterations, q = 0 then 1, from the captured graph (grids sized for every slot).
+int engine_iteration_pair(Engine &e, uint32_t max_lights);
+// Copies the counter words to counts_host and waits for the stream.
+int engine_sync(Engine &e);
+// A whole call: every job's samples traced to completion, results in out.
+int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<JobDev> &jobs,
+               uint32_t total, float *out, mcr_counters *counters);
+void engine_add_counters(const DevCounters &d, uint64_t paths, mcr_counters *out);
+
+struct mcr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Engine eng;
+  float *out_tmp = nullptr;
+  size_t out_tmp_cap = 0;
+  BuilderWorkspace builder;
+  std::mutex mu;
+};
+
+struct mcr_scene {
+  mcr_ctx *ctx = nullptr;
+  DevScene d = {};
+  uint32_t geometry_version = 0, shading_version = 0;
+  bool has_geometry = false, has_shading = false;
+  float4 *tri = nullptr;
+  float4 *inner = nullptr;
+  int2 *longleaf = nullptr;
+  float *normals = nullptr;
+  float *colors = nullptr;
+  DevMaterial *materials = nullptr;
+  DevLight *lights = nullptr;
+  uint64_t bytes_geometry = 0, bytes_shading = 0;
+};
+
+// The view of (scene, camera, settings) as the kernels read it.
+ViewDev make_view(const mcr_scene *s, const mcr_camera &cam, const mcr_settings &st);
+// Checks a tile job and lays it out at sample `start`; returns its sample count or -1 (error set).
+int64_t layout_job(const mcr_tile_job &j, uint32_t index, uint32_t view, uint64_t start, JobDev &out);
+
+#endif  // MCR_CONTEXT_H

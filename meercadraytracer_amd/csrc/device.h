@@ -25,6 +25,10 @@
 #define MCR_STACK_LDS 16      // per-lane traversal stack entries kept in LDS
 #define MCR_STACK_MAX 128     // total per-lane stack (tree depth <= 100, bvh.ts:5)
 
+// Device functions the traversal and shading kernels must not call out of line (a call spills the
+// ray and hit state to scratch and makes every field pointer per-lane).
+#define MCR_INLINE __device__ __attribute__((always_inline)) inline
+
 struct DevMaterial {
   int model, side, transparent, vertex_colors;
   int tone_mapped, physical, pad0, pad1;
@@ -71,7 +75,7 @@ struct DevCounters {  // accumulated per wave, flushed with one atomic per field
 
 __device__ inline bool ref_is_leaf(int32_t ref) { return (uint32_t)ref & MCR_LEAF_BIT; }
 
-__device__ inline void leaf_range(const DevScene &s, int32_t ref, int &first, int &count) {
+MCR_INLINE void leaf_range(const DevScene &s, int32_t ref, int &first, int &count) {
   uint32_t r = (uint32_t)ref;
   uint32_t c = (r >> MCR_LEAF_FIRST_BITS) & 31u;
   if (c != 0) {
@@ -93,7 +97,7 @@ __device__ inline double safe_inverse(double d) { return d != 0 ? 1.0 / d : 1e30
 // signed zeros only meet comparisons. Traversals take the EXACT form for rays whose inverse
 // direction overflows (|d| below 2^-1023), where 0 * inf = NaN can occur.
 template <bool EXACT>
-__device__ inline double box_entry_t(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
+MCR_INLINE double box_entry_t(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
                                      double ox, double oy, double oz, double ix, double iy, double iz,
                                      double tmin, double tmax) {
   double t0 = ((double)bx0 - ox) * ix, t1 = ((double)bx1 - ox) * ix;
@@ -162,7 +166,7 @@ struct TravCount {
 
 // Möller–Trumbore in FP64 on one triangle record (bvh.ts:228-252). Returns false if missed;
 // otherwise t, u, v and det.
-__device__ inline bool mt_test(const float4 a, const float4 b, const float4 c, double ox, double oy,
+MCR_INLINE bool mt_test(const float4 a, const float4 b, const float4 c, double ox, double oy,
                                double oz, double dx, double dy, double dz, int cull_side,
                                double &t, double &u, double &v, double &det_out) {
   double e1x = a.w, e1y = b.x, e1z = b.y;
@@ -187,7 +191,7 @@ __device__ inline bool mt_test(const float4 a, const float4 b, const float4 c, d
 // bvh.ts:202-282 over the device layout. cull: drop back faces of FrontSide and front faces of
 // BackSide triangles.
 template <bool EXACT>
-__device__ inline bool closest_impl(const DevScene &s, double ox, double oy, double oz, double dx,
+MCR_INLINE bool closest_impl(const DevScene &s, double ox, double oy, double oz, double dx,
                                     double dy, double dz, double ix, double iy, double iz,
                                     double tmin, bool cull, TriHit &hit, LaneStack &st,
                                     TravCount &cnt) {
@@ -249,7 +253,7 @@ __device__ inline bool closest_impl(const DevScene &s, double ox, double oy, dou
   return hit.tri >= 0;
 }
 
-__device__ inline bool intersect_closest(const DevScene &s, double ox, double oy, double oz,
+MCR_INLINE bool intersect_closest(const DevScene &s, double ox, double oy, double oz,
                                          double dx, double dy, double dz, double tmin, double tmax,
                                          bool cull, TriHit &hit, LaneStack &st, TravCount &cnt) {
   hit.tri = -1;
@@ -269,7 +273,7 @@ __device__ inline bool intersect_closest(const DevScene &s, double ox, double oy
 // is visited; a box test does not depend on when it runs, so the visit order is unchanged.
 // visit(k, t, u, v, front) returns false to stop.
 template <bool EXACT, class Visit>
-__device__ inline void any_impl(const DevScene &s, double ox, double oy, double oz, double dx,
+MCR_INLINE void any_impl(const DevScene &s, double ox, double oy, double oz, double dx,
                                 double dy, double dz, double ix, double iy, double iz, double tmin,
                                 double tmax, LaneStack &st, TravCount &cnt, Visit &visit) {
   const float *rb = s.root_box;
@@ -313,7 +317,7 @@ __device__ inline void any_impl(const DevScene &s, double ox, double oy, double 
 }
 
 template <class Visit>
-__device__ inline void intersect_any(const DevScene &s, double ox, double oy, double oz, double dx,
+MCR_INLINE void intersect_any(const DevScene &s, double ox, double oy, double oz, double dx,
                                      double dy, double dz, double tmin, double tmax, LaneStack &st,
                                      TravCount &cnt, Visit &visit) {
   if (s.root_ref == -1) return;
@@ -333,7 +337,7 @@ __device__ inline void intersect_any(const DevScene &s, double ox, double oy, do
 // them in the reference's order (leaves by descending first triangle, then ascending index).
 // on_hit(k, leaf_first, t, front) returns true for an opaque hit.
 template <bool EXACT, class OnHit>
-__device__ inline bool occluded_impl(const DevScene &s, double ox, double oy, double oz, double dx,
+MCR_INLINE bool occluded_impl(const DevScene &s, double ox, double oy, double oz, double dx,
                                      double dy, double dz, double ix, double iy, double iz,
                                      double tmin, double tmax, LaneStack &st, TravCount &cnt,
                                      OnHit &on_hit) {
@@ -385,7 +389,7 @@ __device__ inline bool occluded_impl(const DevScene &s, double ox, double oy, do
 }
 
 template <class OnHit>
-__device__ inline bool occluded(const DevScene &s, double ox, double oy, double oz, double dx,
+MCR_INLINE bool occluded(const DevScene &s, double ox, double oy, double oz, double dx,
                                 double dy, double dz, double tmin, double tmax, LaneStack &st,
                                 TravCount &cnt, OnHit &on_hit) {
   if (s.root_ref == -1) return false;

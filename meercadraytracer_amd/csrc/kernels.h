@@ -9,19 +9,19 @@
 #include "wavefront.h"
 
 hipError_t mcr_upload_dfg(const float *lut512);
-void launch_raygen(hipStream_t s, const JobDev *jobs, uint32_t njobs, uint32_t count,
-                   const Wavefront &w, const mcr_camera &cam, uint32_t seed);
-void launch_extend(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t grid,
-                   int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
-void launch_shade(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t max_live,
-                  const mcr_settings &st, const mcr_camera &cam, const JobDev *jobs, uint32_t njobs,
-                  float *out);
-void launch_shadow(hipStream_t s, const DevScene &sc, const Wavefront &w, int cast_only,
-                   uint32_t grid, int32_t *spill, uint32_t spill_stride, double *seen,
-                   int2 *seen_meta, DevCounters *cnt);
-void launch_resolve(hipStream_t s, const Wavefront &w, int q, uint32_t max_live,
-                    const mcr_settings &st, const DevScene &sc, const JobDev *jobs, uint32_t njobs,
-                    float *out, const mcr_camera &cam, uint32_t total);
+void launch_reset_slots(hipStream_t s, const Wavefront &w);
+// mix: the run holds samples of more than one view (see with_view in trace.hip).
+void launch_refill(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                   uint32_t count, uint32_t free_top, uint32_t next, uint32_t queued);
+void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                   uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
+void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                  uint32_t max_live);
+void launch_shadow(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, uint32_t grid,
+                   int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
+                   DevCounters *cnt);
+void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                    uint32_t max_live);
 void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                           double tmin, double tmax, int cull, int32_t *tri, double *t, double *u,
                           double *v, uint8_t *front, uint32_t grid, int32_t *spill,

@@ -63,16 +63,59 @@ __device__ inline void unproject(const mcr_camera &c, double x, double y, double
   out[2] = m[2] * vx + m[6] * vy + m[10] * vz + m[14];
 }
 
+// Runs f(view) for this lane with its view index made wave-uniform: the lanes of one view run
+// together, one view after another (a single pass when the wave holds one view), so the view's
+// fields are read once per wave into scalar registers. Only the lanes that need a view call it.
+// The loop is driven by the mask of lanes still to serve, so it visibly makes progress (a loop
+// that only re-reads the same lane value would be assumed to exit on its first test).
+// MIX = false: every sample of the run has view 0 and the loop is not needed.
+template <bool MIX, class F>
+MCR_INLINE void with_view(const ViewDev *__restrict__ views, uint32_t v, F &&f) {
+  if (!MIX) {
+    f(views[0]);
+    return;
+  }
+  uint64_t todo = __ballot(1);
+  while (todo) {
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1;
+    uint32_t cur = __builtin_amdgcn_readlane(v, leader);
+    uint64_t same = __ballot(v == cur);
+    if (v == cur) {
+      // Inside this branch cur equals v, and the compiler may index with the per-lane v again;
+      // reading the address through readfirstlane keeps the view in scalar registers.
+      uint64_t a = (uint64_t)(views + v);
+      uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+      uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+      f(*reinterpret_cast<const ViewDev *>(((uint64_t)hi << 32) | lo));
+    }
+    todo &= ~same;
+  }
+}
+
+// Adds one finished sample per active lane to done[job], one atomic per distinct job in the wave.
+// Every lane of the wave must call it.
+__device__ inline void count_done(uint32_t *done, int job, bool active) {
+  uint64_t pending = __ballot(active);
+  while (pending) {
+    int leader = __ffsll((unsigned long long)pending) - 1;
+    int jl = __shfl(job, leader, 64);
+    uint64_t same = __ballot(active && job == jl);
+    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&done[jl], (uint32_t)__popcll(same));
+    if (job == jl) active = false;
+    pending &= ~same;
+  }
+}
+
 // ---- ray generation: tracer.ts:62-80 + 705-713 --------------------------------------------------
-// Starts sample s (its pixel, pass and block from the job list) in path slot `slot`. A sample
-// carries its own seed and output position, so which slot runs it, and when, never changes a result.
-__device__ inline void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, const JobDev *jobs,
-                                    uint32_t njobs, const mcr_camera &cam, uint32_t seed) {
-  const JobDev &j = jobs[find_job(jobs, njobs, s)];
+// Starts sample s of job j (its pixel, pass and block) in path slot `slot`. A sample carries its
+// own seed and output position, so which slot runs it, and when, never changes a result.
+MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, const JobDev &j,
+                                    const ViewDev &V) {
+  const mcr_camera &cam = V.cam;
   uint32_t li = s - j.sample_start;
   uint32_t x = (li % j.cols) * j.block, y = (li / j.cols) * j.block;
   int px = j.tx + (int)x, py = j.ty + (int)y;
-  uint32_t rng = rng_seed((uint32_t)px, (uint32_t)py, j.pass, seed);
+  uint32_t rng = rng_seed((uint32_t)px, (uint32_t)py, j.pass, V.st.seed);
   double jx = 0.5, jy = 0.5;
   if (j.block > 1) {
     jx = js_min((double)j.block, (double)j.width - px) * 0.5;
@@ -115,19 +158,15 @@ __device__ inline void start_sample(const Wavefront &w, uint32_t slot, uint32_t 
   iv[I_BOUNCE * P + slot] = 0;
   iv[I_PASSES * P + slot] = 0;
   iv[I_SAMPLE * P + slot] = (int32_t)s;
+  iv[I_VIEW * P + slot] = (int32_t)j.view;
 }
 
-// The first `count` samples of a call, in slots 0..count-1, queued in slot order.
-__global__ __launch_bounds__(256) void k_raygen(const JobDev *jobs, uint32_t njobs, uint32_t count,
-                                                Wavefront w, mcr_camera cam, uint32_t seed) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  if (i == 0) {
-    w.counts[0] = count;
-    w.counts[5] = count;  // next sample to start
-  }
-  w.queue[0][i] = (int32_t)i;
-  start_sample(w, i, i, jobs, njobs, cam, seed);
+// Starts sample s in `slot`: finds its job, then runs start_sample under the job's view.
+template <bool MIX>
+MCR_INLINE void start_sample_any(const ViewDev *__restrict__ views, const Wavefront &w,
+                                        const Control &ctl, uint32_t slot, uint32_t s) {
+  const JobDev &j = ctl.jobs[find_job(ctl.jobs, ctl.njobs, s)];
+  with_view<MIX>(views, j.view, [&](const ViewDev &V) { start_sample(w, slot, s, j, V); });
 }
 
 // Next work index for every lane of the wave: one atomic per wave, 64 consecutive indices.
@@ -139,12 +178,14 @@ __device__ inline uint32_t wave_fetch(uint32_t *counter) {
 }
 
 // ---- extend: closest hit for every live slot (bvh.ts:202-282) ------------------------------------
-__global__ __launch_bounds__(256) void k_extend(DevScene sc, Wavefront w, int q, int32_t *spill,
-                                                uint32_t spill_stride, DevCounters *cnt) {
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_extend(const ViewDev *__restrict__ views, Wavefront w, int q,
+                                                int32_t *spill, uint32_t spill_stride,
+                                                DevCounters *cnt) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t n = w.counts[q];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t nthreads = gridDim.x * blockDim.x;
+  if (gid == 0) w.counts[C_SHADOW] = 0;  // shade of this iteration appends to it
   LaneStack st;
   st.lds = lds_stack + threadIdx.x;
   st.stride = 256;
@@ -154,17 +195,19 @@ __global__ __launch_bounds__(256) void k_extend(DevScene sc, Wavefront w, int q,
   unsigned long long queries = 0;
   size_t P = w.capacity;
   const double *f = w.f;
-  (void)nthreads;
   // Waves take 64 rays at a time from a shared counter, so no wave idles while others still
   // hold long traversals (a static split leaves the slowest waves running alone).
-  for (uint32_t i = wave_fetch(&w.counts[6]); __shfl(i, 0, 64) < n; i = wave_fetch(&w.counts[6])) {
+  for (uint32_t i = wave_fetch(&w.counts[C_EXTEND_FETCH]); __shfl(i, 0, 64) < n;
+       i = wave_fetch(&w.counts[C_EXTEND_FETCH])) {
     if (i >= n) continue;
     int slot = w.queue[q][i];
     double ox = f[F_OX * P + slot], oy = f[F_OY * P + slot], oz = f[F_OZ * P + slot];
     double dx = f[F_DX * P + slot], dy = f[F_DY * P + slot], dz = f[F_DZ * P + slot];
     int flags = w.i[I_FLAGS * P + slot];
     TriHit h;
-    intersect_closest(sc, ox, oy, oz, dx, dy, dz, 0, JM_INF, (flags & PF_CULL) != 0, h, st, tc);
+    with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
+      intersect_closest(V.sc, ox, oy, oz, dx, dy, dz, 0, JM_INF, (flags & PF_CULL) != 0, h, st, tc);
+    });
     queries++;
     w.i[I_HTRI * P + slot] = h.tri;
     w.f[F_HT * P + slot] = h.t;
@@ -175,11 +218,9 @@ __global__ __launch_bounds__(256) void k_extend(DevScene sc, Wavefront w, int q,
   flush_counts(cnt, tc, queries, 0);
 }
 
-// Writes one finished sample (RGBA, doubles rounded to float as Float32Array.set does) over its
-// block of the tile (tracer.ts:81-85).
-__device__ inline void write_sample(const JobDev *jobs, uint32_t njobs, float *out, uint32_t s,
-                                    const double *c) {
-  const JobDev &j = jobs[find_job(jobs, njobs, s)];
+// Writes one finished sample of job j (RGBA, doubles rounded to float as Float32Array.set does)
+// over its block of the tile (tracer.ts:81-85).
+MCR_INLINE void write_sample(const JobDev &j, float *out, uint32_t s, const double *c) {
   uint32_t li = s - j.sample_start;
   uint32_t x = (li % j.cols) * j.block, y = (li / j.cols) * j.block;
   uint32_t ymax = min(y + j.block, (uint32_t)j.th), xmax = min(x + j.block, (uint32_t)j.tw);
@@ -194,7 +235,7 @@ struct LightSample {
 };
 
 // tracer.ts:486-512
-__device__ inline bool light_at(const DevLight &l, double px, double py, double pz, LightSample &o) {
+MCR_INLINE bool light_at(const DevLight &l, double px, double py, double pz, LightSample &o) {
   if (l.type == MCR_LIGHT_DIRECTIONAL) {
     o.lx = l.direction[0];
     o.ly = l.direction[1];
@@ -231,20 +272,17 @@ __device__ inline bool light_at(const DevLight &l, double px, double py, double 
 // Writes the vertex's unshadowed light terms and shadow rays into the light slots, the values
 // resolve needs to finish `lr += tr * (outR + er)`, and the continuation ray (the continuation
 // uses only the random stream and the material, never the shadow results, so it runs here).
-__global__ __launch_bounds__(256) void k_shade(DevScene sc, Wavefront w, int q, mcr_settings st,
-                                               mcr_camera cam, const JobDev *jobs, uint32_t njobs,
-                                               float *out) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = w.counts[q];
-  // Every lane reaches the wave_append at the end, so the bound check guards the work only.
-  bool live = gid < n;
+MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Control &ctl,
+                                     int slot) {
+  const DevScene &sc = V.sc;
+  const mcr_settings &st = V.st;
+  const mcr_camera &cam = V.cam;
   size_t P = w.capacity;
   double *f = w.f;
   int32_t *iv = w.i;
-  int slot = live ? w.queue[q][gid] : 0;
   uint32_t nshadow = 0;
-  const uint32_t nl = min(sc.light_count, (uint32_t)MCR_MAX_LIGHTS);
-  if (live) {
+  const uint32_t nl = V.light_slots;
+  {
     double eps = sc.epsilon;
     int flags = iv[I_FLAGS * P + slot];
     uint32_t rng = (uint32_t)iv[I_RNG * P + slot];
@@ -271,7 +309,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, Wavefront w, int q, 
           c[2] = st.clear_color[2] * a;
           c[3] = a;
         }
-        write_sample(jobs, njobs, out, (uint32_t)iv[I_SAMPLE * P + slot], c);
+        uint32_t smp = (uint32_t)iv[I_SAMPLE * P + slot];
+        write_sample(ctl.jobs[find_job(ctl.jobs, ctl.njobs, smp)], ctl.out, smp, c);
         mode = MODE_DONE;
       } else {
         mode = MODE_FINISH;  // no environment to add (tracer.ts:119-125)
@@ -646,8 +685,33 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, Wavefront w, int q, 
     iv[I_PASSES * P + slot] = passes;
     w.mode[slot] = (uint8_t)mode;
   }
+  return nshadow;
+}
+
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views, Wavefront w, int q) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = w.counts[q];
+  if (gid == 0) {
+    w.counts[q ^ 1] = 0;  // resolve of this iteration queues into it
+    w.counts[C_EXTEND_FETCH] = 0;
+    w.counts[C_SHADOW_FETCH] = 0;
+  }
+  if (blockIdx.x * blockDim.x >= n) return;
+  // Every lane reaches the wave_append below, so the bound check guards the work only.
+  bool live = gid < n;
+  size_t P = w.capacity;
+  int slot = live ? w.queue[q][gid] : 0;
+  uint32_t nshadow = 0, nl = 0;
+  if (live) {
+    const Control &ctl = *w.ctl;
+    with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
+      nshadow = shade_slot(V, w, ctl, slot);
+      nl = V.light_slots;
+    });
+  }
   // Enqueue this lane's shadow rays (one atomic per wave).
-  uint32_t e = wave_append(&w.counts[2], nshadow);
+  uint32_t e = wave_append(&w.counts[C_SHADOW], nshadow);
   if (live && nshadow) {
     for (uint32_t li = 0; li < nl; li++)
       if (w.status[(size_t)li * P + slot] == LS_RAY) w.sref[e++] = slot * MCR_MAX_LIGHTS + (int32_t)li;
@@ -689,11 +753,12 @@ __device__ inline bool visits_before(int2 a, int2 b) {
   return (a.y & 0x7fffffff) < (b.y & 0x7fffffff);
 }
 
-__global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int cast_only,
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ views, Wavefront w,
                                                 int32_t *spill, uint32_t spill_stride,
                                                 double *seen, int2 *seen_meta, DevCounters *cnt) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
-  uint32_t n = w.counts[2];
+  uint32_t n = w.counts[C_SHADOW];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nthreads = gridDim.x * blockDim.x;
   LaneStack st;
@@ -709,7 +774,8 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int ca
   double *T = seen + gid;
   int2 *M = seen_meta + gid;
   const size_t S = nthreads;
-  for (uint32_t e = wave_fetch(&w.counts[7]); __shfl(e, 0, 64) < n; e = wave_fetch(&w.counts[7])) {
+  for (uint32_t e = wave_fetch(&w.counts[C_SHADOW_FETCH]); __shfl(e, 0, 64) < n;
+       e = wave_fetch(&w.counts[C_SHADOW_FETCH])) {
     if (e >= n) continue;
     int32_t ref = w.sref[e];
     int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
@@ -717,11 +783,13 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int ca
     double ox = w.sray[0 * RS + idx], oy = w.sray[1 * RS + idx], oz = w.sray[2 * RS + idx];
     double dx = w.sray[3 * RS + idx], dy = w.sray[4 * RS + idx], dz = w.sray[5 * RS + idx];
     double maxT = w.sray[6 * RS + idx];
-    ShadowCollector col{&sc, cast_only != 0, T, M, S, 0, false};
+    double c[3] = {1, 1, 1};
+    with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
+    const DevScene &sc = V.sc;
+    ShadowCollector col{&sc, V.st.shadows == MCR_SHADOWS_CAST, T, M, S, 0, false};
     bool blocked = occluded(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, col);
-    queries++;
     overflow |= col.overflow;
-    double c[3] = {1, 1, 1}, depth[3] = {0, 0, 0};
+    double depth[3] = {0, 0, 0};
     if (blocked) {
       c[0] = c[1] = c[2] = 0;
     } else if (col.n > 0) {
@@ -771,31 +839,29 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene sc, Wavefront w, int ca
       for (int k = 0; k < 3; k++)
         if (depth[k] > 0) c[k] = c[k] * jm_exp(-depth[k]);
     }
+    });
+    queries++;
     bool visible = c[0] + c[1] + c[2] > 0;
     w.shadow[0 * RS + idx] = c[0];
     w.shadow[1 * RS + idx] = c[1];
     w.shadow[2 * RS + idx] = c[2];
     w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
   }
-  if (overflow) atomicOr(&w.counts[3], 1u);
+  if (overflow) atomicOr(&w.counts[C_OVERFLOW], 1u);
   flush_counts(cnt, tc, 0, queries);
 }
 
 // ---- resolve: finish the vertex's radiance in the reference's summation order ---------------------
-__global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_settings st, DevScene sc,
-                                                 const JobDev *jobs, uint32_t njobs, float *out,
-                                                 mcr_camera cam, uint32_t total) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = w.counts[q];
-  bool live = gid < n;
+MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control &ctl, int slot,
+                                    int mode) {
+  const DevScene &sc = V.sc;
+  const mcr_settings &st = V.st;
   size_t P = w.capacity;
   const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
   double *f = w.f;
-  int slot = live ? w.queue[q][gid] : 0;
-  int mode = live ? w.mode[slot] : MODE_DONE;
   if (mode == MODE_ADD_EXTEND || mode == MODE_ADD_FINISH) {
     double oR = f[F_AR * P + slot], oG = f[F_AG * P + slot], oB = f[F_AB * P + slot];
-    uint32_t nl = min(sc.light_count, (uint32_t)MCR_MAX_LIGHTS);
+    uint32_t nl = V.light_slots;
     for (uint32_t li = 0; li < nl; li++) {
       size_t idx = (size_t)li * P + slot;
       if (w.status[idx] != LS_VISIBLE) continue;
@@ -812,7 +878,6 @@ __global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_setting
     f[F_LG * P + slot] += f[F_QG * P + slot] * (oG + f[F_EG * P + slot]);
     f[F_LB * P + slot] += f[F_QB * P + slot] * (oB + f[F_EB * P + slot]);
   }
-  bool requeue = mode == MODE_EXTEND || mode == MODE_ADD_EXTEND;
   if (mode == MODE_ADD_FINISH || mode == MODE_FINISH) {
     // tracer.ts:472-477: tone map, then fog, alpha 1.
     double c[4] = {f[F_LR * P + slot], f[F_LG * P + slot], f[F_LB * P + slot], 1};
@@ -822,18 +887,74 @@ __global__ __launch_bounds__(256) void k_resolve(Wavefront w, int q, mcr_setting
     double fogf = f[F_FOG * P + slot];
     if (fogf > 0 && sc.fog_kind != MCR_FOG_NONE)
       for (int k = 0; k < 3; k++) c[k] = c[k] + (sc.fog_color[k] - c[k]) * fogf;
-    write_sample(jobs, njobs, out, (uint32_t)w.i[I_SAMPLE * P + slot], c);
+    uint32_t smp = (uint32_t)w.i[I_SAMPLE * P + slot];
+    write_sample(ctl.jobs[find_job(ctl.jobs, ctl.njobs, smp)], ctl.out, smp, c);
+  }
+}
+
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ views, Wavefront w, int q) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = w.counts[q];
+  if (blockIdx.x * blockDim.x >= n) return;
+  bool live = gid < n;
+  size_t P = w.capacity;
+  const Control &ctl = *w.ctl;
+  int slot = live ? w.queue[q][gid] : 0;
+  int mode = live ? w.mode[slot] : MODE_DONE;
+  if (mode != MODE_DONE && mode != MODE_EXTEND)
+    with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot],
+              [&](const ViewDev &V) { resolve_slot(V, w, ctl, slot, mode); });
+  bool requeue = mode == MODE_EXTEND || mode == MODE_ADD_EXTEND;
+  bool finished = live && !requeue;
+  if (ctl.job_done) {
+    int job = finished ? find_job(ctl.jobs, ctl.njobs, (uint32_t)w.i[I_SAMPLE * P + slot]) : 0;
+    count_done(ctl.job_done, job, finished);
   }
   // A finished slot starts the call's next sample at once, so the slot keeps its place in the
-  // queue and consecutive lanes keep reading consecutive slots.
-  bool finished = live && !requeue;
-  uint32_t next = wave_append(&w.counts[5], finished ? 1u : 0u);
-  if (finished && next < total) {
-    start_sample(w, (uint32_t)slot, next, jobs, njobs, cam, w.seed);
-    requeue = true;
+  // queue and consecutive lanes keep reading consecutive slots. With no sample left it goes to
+  // the free list, from which a refill restarts it when the job list grows.
+  uint32_t next = wave_append(&w.counts[C_NEXT], finished ? 1u : 0u);
+  bool idle = false;
+  if (finished) {
+    if (next < ctl.total) {
+      start_sample_any<MIX>(views, w, ctl, (uint32_t)slot, next);
+      requeue = true;
+    } else {
+      idle = true;
+    }
   }
+  uint32_t fpos = wave_append(&w.counts[C_FREE], idle ? 1u : 0u);
+  if (idle) w.free_list[fpos] = slot;
   uint32_t pos = wave_append(&w.counts[q ^ 1], requeue ? 1u : 0u);
   if (requeue) w.queue[q ^ 1][pos] = slot;
+}
+
+// Restarts `count` free slots (the top of the free list) with samples next .. next+count-1 and
+// queues them on queue q after its `queued` live entries; the host passes the counter values it
+// read after the last iteration and this kernel writes the updated ones.
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_refill(const ViewDev *__restrict__ views, Wavefront w, int q,
+                                                uint32_t count, uint32_t free_top, uint32_t next,
+                                                uint32_t queued) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    w.counts[C_FREE] = free_top - count;
+    w.counts[C_NEXT] = next + count;
+    w.counts[q] = queued + count;
+  }
+  if (i >= count) return;
+  int slot = w.free_list[free_top - 1 - i];
+  w.queue[q][queued + i] = slot;
+  start_sample_any<MIX>(views, w, *w.ctl, (uint32_t)slot, next + i);
+}
+
+// Free list = every slot, in descending order, so a refill of k slots takes slots 0 .. k-1.
+__global__ __launch_bounds__(256) void k_reset_slots(Wavefront w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0)
+    for (int k = 0; k < C_WORDS; k++) w.counts[k] = k == C_FREE ? w.capacity : 0;
+  if (i < w.capacity) w.free_list[i] = (int32_t)(w.capacity - 1 - i);
 }
 
 // ---- ray queries over caller rays (bvh.ts:202-332) ---------------------------------------------
@@ -946,31 +1067,35 @@ hipError_t mcr_upload_dfg(const float *lut512) {
   return hipMemcpyToSymbol(HIP_SYMBOL(g_dfg), lut512, 512 * sizeof(float));
 }
 
-void launch_raygen(hipStream_t s, const JobDev *jobs, uint32_t njobs, uint32_t count,
-                   const Wavefront &w, const mcr_camera &cam, uint32_t seed) {
-  hipLaunchKernelGGL(k_raygen, dim3(blocks_for(count)), dim3(256), 0, s, jobs, njobs, count, w, cam, seed);
+void launch_reset_slots(hipStream_t s, const Wavefront &w) {
+  hipLaunchKernelGGL(k_reset_slots, dim3(blocks_for(w.capacity)), dim3(256), 0, s, w);
 }
-void launch_extend(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t grid,
-                   int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
-  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, s, sc, w, q, spill, spill_stride, cnt);
+#define MCR_LAUNCH_MIX(kernel, grid, stream, ...)                                           \
+  do {                                                                                      \
+    if (mix) hipLaunchKernelGGL(kernel<true>, dim3(grid), dim3(256), 0, stream, __VA_ARGS__); \
+    else hipLaunchKernelGGL(kernel<false>, dim3(grid), dim3(256), 0, stream, __VA_ARGS__);     \
+  } while (0)
+
+void launch_refill(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                   uint32_t count, uint32_t free_top, uint32_t next, uint32_t queued) {
+  MCR_LAUNCH_MIX(k_refill, blocks_for(count ? count : 1), s, views, w, q, count, free_top, next, queued);
 }
-void launch_shade(hipStream_t s, const DevScene &sc, const Wavefront &w, int q, uint32_t max_live,
-                  const mcr_settings &st, const mcr_camera &cam, const JobDev *jobs, uint32_t njobs,
-                  float *out) {
-  hipLaunchKernelGGL(k_shade, dim3(blocks_for(max_live)), dim3(256), 0, s, sc, w, q, st, cam, jobs,
-                     njobs, out);
+void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                   uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
+  MCR_LAUNCH_MIX(k_extend, grid, s, views, w, q, spill, spill_stride, cnt);
 }
-void launch_shadow(hipStream_t s, const DevScene &sc, const Wavefront &w, int cast_only,
-                   uint32_t grid, int32_t *spill, uint32_t spill_stride, double *seen,
-                   int2 *seen_meta, DevCounters *cnt) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, s, sc, w, cast_only, spill, spill_stride,
-                     seen, seen_meta, cnt);
+void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                  uint32_t max_live) {
+  MCR_LAUNCH_MIX(k_shade, blocks_for(max_live ? max_live : 1), s, views, w, q);
 }
-void launch_resolve(hipStream_t s, const Wavefront &w, int q, uint32_t max_live,
-                    const mcr_settings &st, const DevScene &sc, const JobDev *jobs, uint32_t njobs,
-                    float *out, const mcr_camera &cam, uint32_t total) {
-  hipLaunchKernelGGL(k_resolve, dim3(blocks_for(max_live)), dim3(256), 0, s, w, q, st, sc, jobs,
-                     njobs, out, cam, total);
+void launch_shadow(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, uint32_t grid,
+                   int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
+                   DevCounters *cnt) {
+  MCR_LAUNCH_MIX(k_shadow, grid, s, views, w, spill, spill_stride, seen, seen_meta, cnt);
+}
+void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                    uint32_t max_live) {
+  MCR_LAUNCH_MIX(k_resolve, blocks_for(max_live ? max_live : 1), s, views, w, q);
 }
 void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                           double tmin, double tmax, int cull, int32_t *tri, double *t, double *u,

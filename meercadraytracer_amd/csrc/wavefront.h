@@ -7,6 +7,7 @@
 #ifndef MCR_WAVEFRONT_H
 #define MCR_WAVEFRONT_H
 
+#include "../../include/mcr.h"
 #include "device.h"
 
 #define MCR_MAX_LIGHTS 16          // punctual lights traced per scene
@@ -27,8 +28,8 @@ enum PathField {
   F_COUNT
 };
 
-// Integer fields.
-enum PathIntField { I_RNG, I_FLAGS, I_BOUNCE, I_PASSES, I_SAMPLE, I_HTRI, I_COUNT };
+// Integer fields. I_VIEW is the view (scene, camera, settings) of the slot's sample.
+enum PathIntField { I_RNG, I_FLAGS, I_BOUNCE, I_PASSES, I_SAMPLE, I_HTRI, I_VIEW, I_COUNT };
 
 enum PathFlags {
   PF_SURFACE = 1, PF_TONEMAP = 2, PF_CULL = 4, PF_FRONT = 8, PF_INDIRECT = 16,
@@ -46,6 +47,41 @@ enum PathMode {
 // Light-slot status after shade / shadow.
 enum LightStatus { LS_NONE = 0, LS_VISIBLE = 1, LS_RAY = 2, LS_BLOCKED = 3 };
 
+// Counter words of Wavefront::counts.
+enum CountWord {
+  C_QUEUE0 = 0,     // live slots in queue 0
+  C_QUEUE1 = 1,     // live slots in queue 1
+  C_SHADOW = 2,     // shadow rays queued by shade
+  C_OVERFLOW = 3,   // set when a shadow ray met more see-through hits than it can keep
+  C_FREE = 4,       // slots on the free list
+  C_NEXT = 5,       // next sample to start (may run past Control::total; the host clamps it)
+  C_EXTEND_FETCH = 6,
+  C_SHADOW_FETCH = 7,
+  C_WORDS = 8,
+};
+
+struct JobDev;
+
+// What the kernels of one call read besides the path state. It lives in device memory so a
+// captured iteration can be replayed while the host grows the job list between replays.
+struct Control {
+  const JobDev *jobs;        // sorted by sample_start
+  float *out;                // tile results: job.out_offset floats from here
+  uint32_t *job_done;        // finished samples per job, or null
+  uint32_t njobs;
+  uint32_t total;            // samples that may be started: [0, total)
+};
+
+// A view: the scene, camera and settings a sample is traced with. Jobs of different views can
+// share one wavefront run; kernels make the view wave-uniform before they read it.
+struct ViewDev {
+  DevScene sc;
+  mcr_camera cam;
+  mcr_settings st;
+  uint32_t light_slots;      // min(light count, MCR_MAX_LIGHTS)
+  uint32_t pad[3];
+};
+
 struct Wavefront {
   uint32_t capacity;         // path slots
   double *f;                 // F_COUNT * capacity
@@ -58,10 +94,10 @@ struct Wavefront {
   uint8_t *status;           // 1 row: LightStatus
   // queues
   int32_t *queue[2];         // live slots
-  uint32_t *counts;          // [0],[1] queue lengths, [2] shadow queue length, [3] overflow flag,
-                             // [5] next sample to start, [6],[7] extend / shadow work counters
+  uint32_t *counts;          // CountWord
   int32_t *sref;             // shadow queue: slot * MCR_MAX_LIGHTS + light
-  uint32_t seed;             // settings.seed of the call (random.ts:6 mixes it into every seed)
+  int32_t *free_list;        // slots with no sample
+  const Control *ctl;
 };
 
 struct JobDev {  // one tile job, device side
@@ -69,6 +105,7 @@ struct JobDev {  // one tile job, device side
   uint32_t width, height, pass, block;
   uint64_t out_offset;
   uint32_t sample_start, cols;
+  uint32_t view, pad;
 };
 
 #endif  // MCR_WAVEFRONT_H

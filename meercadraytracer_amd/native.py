@@ -53,6 +53,8 @@ def lib():
             'mcr_render_tile': (C.c_int, [P, P, C.c_uint32, C.c_uint32, P, P, abi.Tile, C.c_uint32, C.c_uint32,
                                           C.c_uint32, C.c_uint32, P, P]),
             'mcr_render_tiles': (C.c_int, [P, P, P, P, P, C.c_uint32, P, C.c_int, P]),
+            'mcr_render_batch': (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, P, C.c_int, P]),
+            'mcr_set_graph': (C.c_int, [P, C.c_int]),
             'mcr_encode_tile': (C.c_int, [P, P, C.c_uint32, abi.Tile, P, P, C.c_uint32, C.c_int, C.c_int]),
             'mcr_intersect_closest': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P, P,
                                                 P, C.c_int, P]),
@@ -134,6 +136,9 @@ class Context:
 
     def set_timing(self, on=True):
         check(lib().mcr_set_timing(self.handle, int(on)))
+
+    def set_graph(self, on=True):
+        check(lib().mcr_set_graph(self.handle, int(on)))
 
     def set_capacity(self, slots):
         check(lib().mcr_set_wavefront_capacity(self.handle, int(slots)))
@@ -251,6 +256,28 @@ class Scene:
             self.release()
         except Exception:
             pass
+
+
+def render_batch(ctx, views, jobs, job_views, out=None, out_device=False, counters=None):
+    """Tile jobs of many views (scene, camera, settings) in one wavefront run (mcr_render_batch).
+    views: [(Scene, CompiledCamera, TraceSettings)]; jobs: [abi.TileJob]; job_views: view per job."""
+    ctx = ctx or default_context()
+    keep = []
+    varr = (abi.View * max(1, len(views)))()
+    for i, (scene, camera, settings) in enumerate(views):
+        cam, st = camera.to_c(), settings.to_c()
+        keep += [cam, st]
+        varr[i] = abi.View(scene.handle, C.cast(C.pointer(cam), C.c_void_p), C.cast(C.pointer(st), C.c_void_p))
+    jarr = (abi.TileJob * max(1, len(jobs)))(*jobs)
+    vidx = np.ascontiguousarray(job_views, dtype=np.uint32)
+    cnt = counters if counters is not None else abi.Counters()
+    if out is None:
+        total = max((j.out_offset + j.tile.width * j.tile.height * 4 for j in jobs), default=0)
+        out = np.zeros(total, np.float32)
+    ptr = out if out_device else _p(out)
+    check(lib().mcr_render_batch(ctx.handle, C.cast(varr, C.c_void_p), len(views), C.cast(jarr, C.c_void_p),
+                                 _p(vidx), len(jobs), ptr, int(out_device), C.byref(cnt)))
+    return None if out_device else out
 
 
 def encode_tile_device(ctx, image_ptr, width, tile, data_ptr, sum_ptr, count, srgb):
