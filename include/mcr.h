@@ -26,6 +26,7 @@ extern "C" {
 #define MCR_E_NO_SCENE (-4)    /* protocol.ts:77-79: the tile refers to a scene not received */
 #define MCR_E_OVERFLOW (-5)    /* a fixed device capacity was exceeded */
 #define MCR_E_CANCELLED (-6)   /* the job's generation was cancelled before it ran */
+#define MCR_E_TIMEOUT (-7)     /* a wait ended before the request finished */
 
 /* ---- enums restated from src/constants.ts and src/types.ts ------------------------------ */
 enum { MCR_FRONT_SIDE = 0, MCR_BACK_SIDE = 1, MCR_DOUBLE_SIDE = 2 };           /* constants.ts:5-7 */
@@ -220,6 +221,64 @@ typedef struct mcr_view {
 int mcr_render_batch(mcr_ctx *ctx, const mcr_view *views, uint32_t view_count,
                      const mcr_tile_job *jobs, const uint32_t *job_views, uint32_t job_count,
                      float *out, int out_device, mcr_counters *counters);
+
+/* ---- served form: a request loop over many resident scenes ---------------------------------- */
+/* Replaces ProgressiveRenderer.render / stop (src/progressive.ts:186-275) for many renders at
+ * once: each request is one render() call (camera, settings, size, samples, tile size, preview
+ * block, output colour space). A server thread keeps admitted requests' tiles in one wavefront:
+ * tiles of every live request share the device (continuous batching), newly admitted requests'
+ * previews go first, tiles are accumulated and encoded in pass order into the request's 8-bit
+ * image in HBM (writeTile), and finished results are the bytes raytrace() gives for the same
+ * input. Admission is by an HBM budget that covers the scenes added to the server and every
+ * admitted request's image and running sum; requests wait, first in first out, until both
+ * memory and a request slot are free. mcr_server_cancel is stop(): queued tiles of the request
+ * are dropped and its buffers are freed once its samples in flight have drained. */
+typedef struct mcr_server mcr_server;
+typedef struct mcr_server_config {
+  uint64_t hbm_budget;      /* bytes; 0 = 200 GB */
+  uint32_t slots;           /* wavefront path slots; 0 = 2^21 */
+  uint32_t window;          /* unstarted samples kept queued ahead of the slots; 0 = slots */
+  uint32_t max_requests;    /* admitted requests at once; 0 = 64 */
+  int32_t use_graph;        /* replay captured bounce iterations (1) or launch directly (0) */
+} mcr_server_config;
+typedef struct mcr_request {
+  mcr_scene *scene;         /* added with mcr_server_add_scene */
+  mcr_camera camera;
+  mcr_settings settings;
+  uint32_t width, height;
+  uint32_t samples;         /* passes (ProgressiveOptions.samples) */
+  uint32_t tile_size;       /* 0 = 64 (RaytracingRenderer.ts:105) */
+  uint32_t preview_block;   /* <= 1: no preview; 0 = 4 */
+  int32_t srgb;             /* outputColorSpace 'srgb' (1) or 'srgb-linear' (0) */
+} mcr_request;
+enum { MCR_REQ_QUEUED = 0, MCR_REQ_RUNNING = 1, MCR_REQ_DONE = 2, MCR_REQ_CANCELLED = 3, MCR_REQ_FAILED = 4 };
+typedef struct mcr_request_status {
+  int32_t state;
+  uint32_t samples, max_samples;   /* frame samples = min over tiles (progressive.ts:214-218) */
+  int32_t preview_done;            /* every tile has its preview or a pass written */
+  double submit_ms, admit_ms, first_preview_ms, done_ms;  /* server clock; -1 = not yet */
+  uint64_t paths;                  /* samples traced for the request */
+} mcr_request_status;
+typedef struct mcr_server_stats {
+  uint64_t closest_queries, any_queries, paths, iterations;
+  uint64_t bytes_in_use, requests_live, requests_queued;
+  double device_ms;                /* time spent waiting for the device */
+  double uptime_ms;
+} mcr_server_stats;
+int mcr_server_create(mcr_ctx *ctx, const mcr_server_config *config, mcr_server **out);
+void mcr_server_destroy(mcr_server *server);
+int mcr_server_add_scene(mcr_server *server, mcr_scene *scene);
+int mcr_server_submit(mcr_server *server, const mcr_request *request, uint64_t *id_out);
+int mcr_server_cancel(mcr_server *server, uint64_t id);
+int mcr_server_status(mcr_server *server, uint64_t id, mcr_request_status *out);
+/* Waits until the request is done, cancelled or failed; MCR_E_TIMEOUT after timeout_ms (< 0:
+ * no limit). */
+int mcr_server_wait(mcr_server *server, uint64_t id, double timeout_ms);
+/* The request's current frame, width*height*4 bytes, straight alpha (the frame image). */
+int mcr_server_read(mcr_server *server, uint64_t id, uint8_t *rgba_out);
+/* Forgets a finished or cancelled request (its id is no longer valid). */
+int mcr_server_release(mcr_server *server, uint64_t id);
+int mcr_server_stats_get(mcr_server *server, mcr_server_stats *out);
 
 /* ---- accumulate + encode: replaces writeTile/encodeTile (src/progressive.ts:288-313) ------- */
 /* sum may be NULL (preview tiles). image is width*height*4 bytes, straight alpha. */

@@ -133,6 +133,39 @@ class Timing(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [('hbm_budget', C.c_uint64), ('slots', C.c_uint32), ('window', C.c_uint32),
+                ('max_requests', C.c_uint32), ('use_graph', C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [('scene', C.c_void_p), ('camera', Camera), ('settings', Settings), ('width', C.c_uint32),
+                ('height', C.c_uint32), ('samples', C.c_uint32), ('tile_size', C.c_uint32),
+                ('preview_block', C.c_uint32), ('srgb', C.c_int32)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
+MCR_E_CANCELLED, MCR_E_TIMEOUT = -6, -7
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [('state', C.c_int32), ('samples', C.c_uint32), ('max_samples', C.c_uint32),
+                ('preview_done', C.c_int32), ('submit_ms', C.c_double), ('admit_ms', C.c_double),
+                ('first_preview_ms', C.c_double), ('done_ms', C.c_double), ('paths', C.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [('closest_queries', C.c_uint64), ('any_queries', C.c_uint64), ('paths', C.c_uint64),
+                ('iterations', C.c_uint64), ('bytes_in_use', C.c_uint64), ('requests_live', C.c_uint64),
+                ('requests_queued', C.c_uint64), ('device_ms', C.c_double), ('uptime_ms', C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def to_int32(x):
     """ECMAScript ToInt32, as random.ts:7 applies it to settings.seed through `^`."""
     import math

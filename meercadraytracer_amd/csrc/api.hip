@@ -164,8 +164,7 @@ int mcr_set_wavefront_capacity(mcr_ctx *c, uint32_t slots) {
   Engine &e = c->eng;
   e.capacity_limit = slots;
   if (e.capacity > slots) {
-    if (e.graph) hipGraphExecDestroy(e.graph);
-    e.graph = nullptr;
+    engine_drop_graphs(e);
     hipFree(e.w_block);
     e.w_block = nullptr;
     e.capacity = 0;

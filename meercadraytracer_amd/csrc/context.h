@@ -64,10 +64,11 @@ struct Engine {
   Control *ctl_host = nullptr;     // pinned staging
   uint32_t *counts_host = nullptr; // pinned
   bool mix = false;                // the run holds more than one view
-  // Two captured bounce iterations (queue 0 then queue 1), replayed while paths are live.
+  // Two captured bounce iterations (queue 0 then queue 1), replayed while paths are live; one
+  // graph per kernel variant (single view, mixed views).
   bool use_graph = true;
-  hipGraphExec_t graph = nullptr;
-  const void *graph_key[4] = {};
+  hipGraphExec_t graph[2] = {};
+  const void *graph_key[2][4] = {};
   // timing
   bool timing = false;
   mcr_timing last_timing = {};
@@ -92,6 +93,7 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
                      double *t_shade, double *t_any);
 // Two iterations, q = 0 then 1, from the captured graph (grids sized for every slot).
 int engine_iteration_pair(Engine &e, uint32_t max_lights);
+void engine_drop_graphs(Engine &e);
 // Copies the counter words to counts_host and waits for the stream.
 int engine_sync(Engine &e);
 // A whole call: every job's samples traced to completion, results in out.

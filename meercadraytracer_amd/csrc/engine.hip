@@ -16,10 +16,7 @@ int alloc_wavefront(Engine &e, uint32_t capacity) {
   hipFree(e.w_block);
   e.w_block = nullptr;
   e.capacity = 0;
-  if (e.graph) {
-    hipGraphExecDestroy(e.graph);
-    e.graph = nullptr;
-  }
+  engine_drop_graphs(e);
   size_t P = capacity;
   size_t L = (size_t)MCR_MAX_LIGHTS * P;
   size_t bytes = 0;
@@ -75,6 +72,13 @@ int grow(T *&p, uint32_t &cap, uint32_t n) {
 
 }  // namespace
 
+void engine_drop_graphs(Engine &e) {
+  for (auto &g : e.graph) {
+    if (g) hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
 int engine_init(Engine &e, int device, hipStream_t stream) {
   e.device = device;
   e.stream = stream;
@@ -93,7 +97,7 @@ int engine_init(Engine &e, int device, hipStream_t stream) {
 
 void engine_free(Engine &e) {
   if (e.stream) hipStreamSynchronize(e.stream);
-  if (e.graph) hipGraphExecDestroy(e.graph);
+  engine_drop_graphs(e);
   hipFree(e.w_block);
   hipFree(e.spill);
   hipFree(e.seen);
@@ -117,10 +121,7 @@ int engine_reserve(Engine &e, uint64_t samples, uint32_t nviews, uint32_t njobs,
   if (e.capacity < cap) MCR_TRY(alloc_wavefront(e, cap));
   const void *old_views = e.views;
   MCR_TRY(grow(e.views, e.views_cap, std::max<uint32_t>(nviews, 1)));
-  if (old_views != e.views && e.graph) {
-    hipGraphExecDestroy(e.graph);
-    e.graph = nullptr;
-  }
+  if (old_views != e.views) engine_drop_graphs(e);
   MCR_TRY(grow(e.jobs, e.jobs_cap, std::max<uint32_t>(njobs, 1)));
   if (track_done) MCR_TRY(grow(e.job_done, e.job_done_cap, std::max<uint32_t>(njobs, 1)));
   return MCR_OK;
@@ -195,13 +196,15 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
 }
 
 int engine_iteration_pair(Engine &e, uint32_t max_lights) {
+  int v = e.mix ? 1 : 0;
   const void *key[4] = {e.w_block, e.views, e.counters,
-                        (const void *)(uintptr_t)((uint64_t)e.capacity * 64u + max_lights * 2u + (e.mix ? 1u : 0u))};
-  if (e.graph && memcmp(key, e.graph_key, sizeof(key)) != 0) {
-    hipGraphExecDestroy(e.graph);
-    e.graph = nullptr;
+                        (const void *)(uintptr_t)((uint64_t)e.capacity * 64u + max_lights)};
+  hipGraphExec_t &graph = e.graph[v];
+  if (graph && memcmp(key, e.graph_key[v], sizeof(key)) != 0) {
+    hipGraphExecDestroy(graph);
+    graph = nullptr;
   }
-  if (!e.graph) {
+  if (!graph) {
     // Grids for every slot: a replay does not know how many paths are live. Blocks past the live
     // count return at once, and traversal waves leave as soon as the shared counter runs out.
     hipStream_t cs;
@@ -221,16 +224,16 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
       }
       err = hipStreamEndCapture(cs, &g);
     }
-    if (err == hipSuccess) err = hipGraphInstantiate(&e.graph, g, nullptr, nullptr, 0);
+    if (err == hipSuccess) err = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
     if (g) hipGraphDestroy(g);
     hipStreamDestroy(cs);
     if (err != hipSuccess) {
-      e.graph = nullptr;
+      graph = nullptr;
       return mcr_fail(MCR_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(err));
     }
-    memcpy(e.graph_key, key, sizeof(key));
+    memcpy(e.graph_key[v], key, sizeof(key));
   }
-  HIP_TRY(hipGraphLaunch(e.graph, e.stream));
+  HIP_TRY(hipGraphLaunch(graph, e.stream));
   return MCR_OK;
 }
 

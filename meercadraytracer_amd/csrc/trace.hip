@@ -1031,20 +1031,18 @@ __device__ inline uint8_t to_uint8_clamp(double v) {
   return (uint8_t)((fi & 1) ? fi + 1 : fi);
 }
 
-__global__ __launch_bounds__(256) void k_encode(uint8_t *image, uint32_t width, mcr_tile tile,
-                                                const float *data, float *sum, uint32_t count,
-                                                int srgb) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t npx = (uint32_t)tile.width * (uint32_t)tile.height;
-  if (i >= npx) return;
-  uint32_t x = i % tile.width, y = i / tile.width;
+// One pixel of writeTile: sum += sample (the array keeps the float, the frame the double), mean,
+// un-premultiply, sRGB encode, ToUint8Clamp(v * 255 + 0.5).
+MCR_INLINE void encode_pixel(uint8_t *image, uint32_t width, int32_t tx, int32_t ty, int32_t tw,
+                             const float *data, float *sum, uint32_t count, int srgb, uint32_t i) {
+  uint32_t x = i % (uint32_t)tw, y = i / (uint32_t)tw;
   size_t s = (size_t)i * 4;
-  size_t d = (((size_t)tile.y + y) * width + tile.x + x) * 4;
+  size_t d = (((size_t)ty + y) * width + tx + x) * 4;
   double c[4] = {data[s], data[s + 1], data[s + 2], data[s + 3]};
   if (sum) {
     double scale = 1.0 / count;
     for (int k = 0; k < 4; k++) {
-      double v = (double)sum[d + k] + c[k];  // the array keeps the float, the frame the double
+      double v = (double)sum[d + k] + c[k];
       sum[d + k] = (float)v;
       c[k] = v;
     }
@@ -1058,6 +1056,29 @@ __global__ __launch_bounds__(256) void k_encode(uint8_t *image, uint32_t width, 
   if (srgb)
     for (int k = 0; k < 3; k++) c[k] = linear_to_srgb(js_max(0, c[k]));
   for (int k = 0; k < 4; k++) image[d + k] = to_uint8_clamp(c[k] * 255 + 0.5);
+}
+
+__global__ __launch_bounds__(256) void k_encode(uint8_t *image, uint32_t width, mcr_tile tile,
+                                                const float *data, float *sum, uint32_t count,
+                                                int srgb) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint32_t)tile.width * (uint32_t)tile.height) return;
+  encode_pixel(image, width, tile.x, tile.y, tile.width, data, sum, count, srgb, i);
+}
+
+// Many tiles at once: block b works on the item whose block range holds b. No two items of one
+// launch touch the same pixel (the host puts at most one pass of a tile in a launch).
+__global__ __launch_bounds__(256) void k_encode_batch(const EncodeItem *items, uint32_t n) {
+  int lo = 0, hi = (int)n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block_start <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const EncodeItem &it = items[lo];
+  uint32_t i = (blockIdx.x - it.block_start) * blockDim.x + threadIdx.x;
+  if (i >= (uint32_t)it.tw * (uint32_t)it.th) return;
+  encode_pixel(it.image, it.width, it.tx, it.ty, it.tw, it.data, it.sum, it.count, it.srgb, i);
 }
 
 // ---- launch wrappers ----------------------------------------------------------------------------
@@ -1109,6 +1130,9 @@ void launch_probe_any_count(hipStream_t s, const DevScene &sc, const double *ray
                             int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
   hipLaunchKernelGGL(k_probe_any_count, dim3(grid), dim3(256), 0, s, sc, rays, n, tmin, tmax,
                      count, spill, spill_stride, cnt);
+}
+void launch_encode_batch(hipStream_t s, const EncodeItem *items, uint32_t n, uint32_t blocks) {
+  if (n && blocks) hipLaunchKernelGGL(k_encode_batch, dim3(blocks), dim3(256), 0, s, items, n);
 }
 void launch_encode(hipStream_t s, uint8_t *image, uint32_t width, const mcr_tile &tile,
                    const float *data, float *sum, uint32_t count, int srgb) {

@@ -108,4 +108,16 @@ struct JobDev {  // one tile job, device side
   uint32_t view, pad;
 };
 
+// One tile result to accumulate and encode (writeTile): sum may be null (a preview tile).
+struct EncodeItem {
+  uint8_t *image;
+  float *sum;
+  const float *data;
+  uint32_t width;
+  int32_t tx, ty, tw, th;
+  uint32_t count;
+  int32_t srgb;
+  uint32_t block_start;      // first block of the launch that works on this item
+};
+
 #endif  // MCR_WAVEFRONT_H

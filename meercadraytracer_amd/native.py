@@ -55,6 +55,16 @@ def lib():
             'mcr_render_tiles': (C.c_int, [P, P, P, P, P, C.c_uint32, P, C.c_int, P]),
             'mcr_render_batch': (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, P, C.c_int, P]),
             'mcr_set_graph': (C.c_int, [P, C.c_int]),
+            'mcr_server_create': (C.c_int, [P, P, C.POINTER(P)]),
+            'mcr_server_destroy': (None, [P]),
+            'mcr_server_add_scene': (C.c_int, [P, P]),
+            'mcr_server_submit': (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
+            'mcr_server_cancel': (C.c_int, [P, C.c_uint64]),
+            'mcr_server_status': (C.c_int, [P, C.c_uint64, P]),
+            'mcr_server_wait': (C.c_int, [P, C.c_uint64, C.c_double]),
+            'mcr_server_read': (C.c_int, [P, C.c_uint64, P]),
+            'mcr_server_release': (C.c_int, [P, C.c_uint64]),
+            'mcr_server_stats_get': (C.c_int, [P, P]),
             'mcr_encode_tile': (C.c_int, [P, P, C.c_uint32, abi.Tile, P, P, C.c_uint32, C.c_int, C.c_int]),
             'mcr_intersect_closest': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P, P,
                                                 P, C.c_int, P]),
@@ -283,3 +293,69 @@ def render_batch(ctx, views, jobs, job_views, out=None, out_device=False, counte
 def encode_tile_device(ctx, image_ptr, width, tile, data_ptr, sum_ptr, count, srgb):
     """progressive.ts:288 writeTile on device buffers."""
     check(lib().mcr_encode_tile(ctx.handle, image_ptr, width, tile.to_c(), data_ptr, sum_ptr, count, int(srgb), 1))
+
+
+class Server:
+    """The served form (mcr_server_*): a request loop over scenes resident in HBM. Each request is
+    one progressive render (camera, settings, size, samples, tile size, preview block)."""
+
+    def __init__(self, ctx=None, hbm_budget=0, slots=0, window=0, max_requests=0, use_graph=True):
+        self.ctx = ctx or default_context()
+        cfg = abi.ServerConfig(int(hbm_budget), int(slots), int(window), int(max_requests), int(bool(use_graph)))
+        h = C.c_void_p()
+        check(lib().mcr_server_create(self.ctx.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h.value
+        self.sizes = {}
+
+    def add_scene(self, scene):
+        check(lib().mcr_server_add_scene(self.handle, scene.handle))
+
+    def submit(self, scene, camera, settings, width, height, samples, tile_size=64, preview_block=4, srgb=True):
+        r = abi.Request(scene.handle, camera.to_c(), settings.to_c(), width, height, samples, tile_size,
+                        preview_block, int(bool(srgb)))
+        rid = C.c_uint64()
+        check(lib().mcr_server_submit(self.handle, C.byref(r), C.byref(rid)))
+        self.sizes[rid.value] = (width, height)
+        return rid.value
+
+    def cancel(self, rid):
+        check(lib().mcr_server_cancel(self.handle, rid))
+
+    def status(self, rid):
+        st = abi.RequestStatus()
+        check(lib().mcr_server_status(self.handle, rid, C.byref(st)))
+        return st.as_dict()
+
+    def wait(self, rid, timeout_ms=-1.0):
+        """Returns the final state (done / cancelled); raises on failure or timeout."""
+        rc = lib().mcr_server_wait(self.handle, rid, float(timeout_ms))
+        if rc == abi.MCR_E_CANCELLED:
+            return abi.REQ_CANCELLED
+        check(rc)
+        return abi.REQ_DONE
+
+    def read(self, rid):
+        w, h = self.sizes[rid]
+        out = np.zeros(w * h * 4, np.uint8)
+        check(lib().mcr_server_read(self.handle, rid, _p(out)))
+        return out
+
+    def release(self, rid):
+        check(lib().mcr_server_release(self.handle, rid))
+        self.sizes.pop(rid, None)
+
+    def stats(self):
+        st = abi.ServerStats()
+        check(lib().mcr_server_stats_get(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().mcr_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -42,7 +42,8 @@ STRUCTS = {
     'mcr_geometry': abi.Geometry, 'mcr_material': abi.Material, 'mcr_light': abi.Light,
     'mcr_shading': abi.Shading, 'mcr_camera': abi.Camera, 'mcr_settings': abi.Settings,
     'mcr_tile': abi.Tile, 'mcr_tile_job': abi.TileJob, 'mcr_view': abi.View, 'mcr_counters': abi.Counters,
-    'mcr_timing': abi.Timing,
+    'mcr_timing': abi.Timing, 'mcr_server_config': abi.ServerConfig, 'mcr_request': abi.Request,
+    'mcr_request_status': abi.RequestStatus, 'mcr_server_stats': abi.ServerStats,
 }
 
 

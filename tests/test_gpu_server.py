@@ -1,0 +1,167 @@
+"""Served form (mcr_server_*): requests of several scenes, sizes, tile sizes and sample counts
+share one wavefront. Every finished request must hold exactly the bytes the oracle's raytrace()
+gives for it alone (progressive.test.ts:66-85 asks the same of the progressive renderer), in any
+mix, after cancellations, and when admission waits for memory."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from meercadraytracer_amd import abi, native, scenes
+from meercadraytracer_amd.raytrace import trace_settings
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def world():
+    a = scenes.compiled('plain_cpu', 0, build_bvh=oracle.build_bvh, width=64, height=48)
+    b = scenes.compiled('plain_gpu', 2, build_bvh=oracle.build_bvh, width=40, height=40, quads=30, spheres=12,
+                        subdivisions=2)
+    c = scenes.compiled('plain_cpu', 4, build_bvh=oracle.build_bvh, width=64, height=48)
+    res = []
+    for sc, cam, o in (a, b, c):
+        s = native.Scene()
+        s.set_geometry(sc.geometry, 1)
+        s.set_shading(sc.shading, 1)
+        res.append((s, sc, cam, o))
+    yield res
+    for s, *_ in res:
+        s.release()
+
+
+# (scene, width, height, samples, tile, preview, extra options)
+REQUESTS = [
+    (0, 64, 48, 3, 16, 4, {}),
+    (1, 40, 40, 4, 32, 4, {'seed': 5}),
+    (2, 50, 30, 2, 64, 1, {'shadows': 'castShadow'}),
+    (0, 33, 21, 5, 8, 3, {'globalIllumination': True, 'seed': 2}),
+    (1, 24, 40, 1, 16, 4, {}),
+]
+
+
+def _expected(world, spec):
+    k, w, h, spp, _, _, extra = spec
+    _, sc, cam, o = world[k]
+    st = trace_settings(dict(o, **extra))
+    ref, _ = oracle.raytrace(sc, cam, st, w, h, samples=spp, srgb=True)
+    return ref
+
+
+def _submit(srv, world, spec):
+    k, w, h, spp, tile, preview, extra = spec
+    s, sc, cam, o = world[k]
+    return srv.submit(s, cam, trace_settings(dict(o, **extra)), w, h, spp, tile, preview)
+
+
+def _server(world, **kw):
+    srv = native.Server(**kw)
+    for s, *_ in world:
+        srv.add_scene(s)
+    return srv
+
+
+@pytest.mark.parametrize('graph', [True, False])
+def test_alone_equals_raytrace(world, graph):
+    srv = _server(world, use_graph=graph)
+    try:
+        for spec in REQUESTS[:2]:
+            print(f'request {spec[:6]} graph {graph}', flush=True)
+            rid = _submit(srv, world, spec)
+            assert srv.wait(rid, 60000) == abi.REQ_DONE
+            st = srv.status(rid)
+            assert st['samples'] == spec[3] and st['preview_done']
+            assert 0 <= st['submit_ms'] <= st['admit_ms'] <= st['first_preview_ms'] <= st['done_ms']
+            assert np.array_equal(srv.read(rid), _expected(world, spec))
+            srv.release(rid)
+    finally:
+        srv.close()
+
+
+def test_mix_equals_alone(world):
+    """Five requests at once, then three more while they run; slots small enough that every
+    wave mixes requests."""
+    srv = _server(world, slots=512, window=1024)
+    try:
+        ids = [(_submit(srv, world, spec), spec) for spec in REQUESTS]
+        time.sleep(0.01)
+        ids += [(_submit(srv, world, spec), spec) for spec in REQUESTS[1:4]]
+        for rid, spec in ids:
+            print(f'request {rid} {spec[:6]}', flush=True)
+            assert srv.wait(rid, 120000) == abi.REQ_DONE
+            assert np.array_equal(srv.read(rid), _expected(world, spec)), f'request {rid} {spec[:6]}'
+        stats = srv.stats()
+        assert stats['paths'] > 0 and stats['closest_queries'] > 0
+    finally:
+        srv.close()
+
+
+def test_cancel_restart(world):
+    """A camera move: the in-flight render is cancelled and a new one submitted. The cancelled
+    request ends cancelled, its memory returns, and the restart is exact."""
+    srv = _server(world, slots=1024)
+    try:
+        base = srv.stats()['bytes_in_use']
+        long_spec = (0, 64, 48, 200, 16, 4, {})
+        rid = _submit(srv, world, long_spec)
+        t0 = time.time()
+        while not srv.status(rid)['preview_done'] and time.time() - t0 < 30:
+            time.sleep(0.001)
+        srv.cancel(rid)
+        assert srv.wait(rid, 30000) == abi.REQ_CANCELLED
+        assert srv.status(rid)['state'] == abi.REQ_CANCELLED
+        spec = REQUESTS[3]
+        rid2 = _submit(srv, world, spec)
+        assert srv.wait(rid2, 60000) == abi.REQ_DONE
+        assert np.array_equal(srv.read(rid2), _expected(world, spec))
+        srv.release(rid)
+        srv.release(rid2)
+        t0 = time.time()
+        while srv.stats()['bytes_in_use'] != base and time.time() - t0 < 10:
+            time.sleep(0.005)
+        assert srv.stats()['bytes_in_use'] == base
+    finally:
+        srv.close()
+
+
+def test_admission_waits_for_budget(world):
+    """A budget with room for one request at a time: the second is admitted only after the first
+    finishes and gives its running sum back."""
+    probe = _server(world)
+    fixed = probe.stats()['bytes_in_use']
+    probe.close()
+    spec = REQUESTS[0]
+    need = spec[1] * spec[2] * 20
+    srv = _server(world, hbm_budget=fixed + need + need // 2)
+    try:
+        a = _submit(srv, world, spec)
+        b = _submit(srv, world, spec)
+        assert srv.wait(a, 60000) == abi.REQ_DONE
+        assert srv.wait(b, 60000) == abi.REQ_DONE
+        sa, sb = srv.status(a), srv.status(b)
+        assert sb['admit_ms'] >= sa['done_ms']
+        ref = _expected(world, spec)
+        assert np.array_equal(srv.read(a), ref) and np.array_equal(srv.read(b), ref)
+    finally:
+        srv.close()
+
+
+def test_errors(world):
+    srv = native.Server()
+    try:
+        s, sc, cam, o = world[0]
+        with pytest.raises(native.NativeError) as e:
+            srv.submit(s, cam, trace_settings(o), 8, 8, 1)
+        assert 'not added' in str(e.value)
+        srv.add_scene(s)
+        with pytest.raises(native.NativeError):
+            srv.submit(s, cam, trace_settings(o), 0, 8, 1)
+        rid = srv.submit(s, cam, trace_settings(o), 16, 16, 1000, 8)
+        with pytest.raises(native.NativeError) as e:
+            srv.release(rid)
+        assert 'still live' in str(e.value)
+        srv.cancel(rid)
+        assert srv.wait(rid, 30000) == abi.REQ_CANCELLED
+    finally:
+        srv.close()
