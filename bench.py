@@ -84,8 +84,16 @@ def _pmc_run(counter, bench_args, outdir):
     rows.sort(key=lambda r: int(r['Dispatch_Id']))
     out = {}
     for r in rows:
-        out.setdefault(r['Kernel_Name'].split('(')[0], []).append(float(r['Counter_Value']))
+        out.setdefault(kernel_symbol(r['Kernel_Name']), []).append(float(r['Counter_Value']))
     return out
+
+
+def kernel_symbol(name):
+    """'void k_shadow<false>(ViewDev const*, ...)' -> 'k_shadow'."""
+    name = name.split('(')[0].strip()
+    if name.startswith('void '):
+        name = name[5:]
+    return name.split('<')[0]
 
 
 def measure_traffic(args, kernel, launches):
@@ -97,7 +105,7 @@ def measure_traffic(args, kernel, launches):
         return None
     outdir = tempfile.mkdtemp(prefix='mcr_pmc_', dir='/tmp')
     bench_args = ['--steps', str(args.steps), '--warmup', str(args.warmup), '--no-cpu', '--no-traffic',
-                  '--width', str(args.width), '--height', str(args.height)]
+                  '--no-batched', '--no-served', '--width', str(args.width), '--height', str(args.height)]
     copy_args = ['--copy-calibration', '4096']
     res = {}
     for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
@@ -114,6 +122,39 @@ def measure_traffic(args, kernel, launches):
             'fetch_scale': round(res['FETCH_SIZE'][1], 4), 'write_scale': round(res['WRITE_SIZE'][1], 4)}
 
 
+def served(args, ctx, workloads):
+    """The served mix at each arrival rate over the same 32 resident scenes. The top-level
+    figures are those of the highest rate (the one meant to keep the device saturated)."""
+    scenes = workloads.resident_scenes(ctx, 2, 32, 50_000, 4_000_000, log)
+    try:
+        tuned, trials = None, []
+        slots = args.served_slots
+        if not slots:
+            tuned, trials = workloads.autotune_served(ctx, scenes, log=log)
+            slots = tuned['slots']
+        sweep = []
+        for rate in [float(r) for r in args.served_rates.split(',') if r]:
+            log(f'served mix at {rate}/s for {args.served_seconds} s, {slots} slots ...')
+            r = workloads.served_mix(ctx, rate, args.served_seconds, scenes=scenes, slots=slots,
+                                     drain_seconds=10.0, log=log)
+            specs = r.pop('completed_specs')
+            log(json.dumps(r))
+            sweep.append(r)
+        # The requests the last rate completed, as one batch on the same slots.
+        same = workloads.batched_equivalent(ctx, scenes, specs, slots=slots, log=log)
+    finally:
+        for gpu, _, _ in scenes:
+            gpu.release()
+    top = sweep[-1]
+    return {'workload': '32 resident plain_gpu scenes (50k-4M triangles, log-uniform), Poisson arrivals, '
+                        'resolutions {320x240, 640x480, 1280x720, 1920x1080}, samples log-uniform [4, 256], '
+                        '30% camera-move restarts, 64-px tiles, 4x4 preview, HBM budget 200 GB',
+            'triangles': int(sum(n for _, n, _ in scenes)),
+            'ttfp_ms_p50': top['ttfp_ms_p50'], 'ttfp_ms_p99': top['ttfp_ms_p99'],
+            'saturated_mrays_s': top['mrays_s'], 'slots': slots, 'autotune': trials, 'batched_same_scenes': same,
+            'vs_batched_same_scenes': round(top['mrays_s'] / same['mrays_s'], 3), 'sweep': sweep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -124,6 +165,13 @@ def main():
     ap.add_argument('--no-traffic', action='store_true', help='skip the rocprofv3 HBM-traffic passes')
     ap.add_argument('--width', type=int, default=1920)
     ap.add_argument('--height', type=int, default=1080)
+    ap.add_argument('--no-batched', action='store_true', help='skip the batched-form measurement')
+    ap.add_argument('--no-served', action='store_true', help='skip the served-mix measurement')
+    ap.add_argument('--served-rates', default='2,8', help='request arrival rates (per second) to sweep')
+    ap.add_argument('--served-seconds', type=float, default=12.0, help='arrival window per rate')
+    ap.add_argument('--served-slots', type=int, default=0, help='server path slots (0 = default)')
+    ap.add_argument('--only', choices=['plain', 'batched', 'served'], default=None,
+                    help='run one workload and print its JSON object only')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB (for scaling HBM counters)')
     args = ap.parse_args()
@@ -141,13 +189,19 @@ def main():
               '(tiles shard across GPUs by the scheduler; only the one-GPU path is measured)', file=sys.stderr)
         sys.exit(2)
 
-    from meercadraytracer_amd import abi, native, scenes
+    from meercadraytracer_amd import abi, native, scenes, workloads
     from meercadraytracer_amd.compile import compile_camera, compile_scene
     from meercadraytracer_amd.raytrace import render_passes, trace_settings
     from meercadraytracer_amd.types import Tile
 
     W, H = args.width, args.height
     ctx = native.default_context()
+    if args.only == 'batched':
+        print(json.dumps(workloads.batched_form(ctx, log=log)), flush=True)
+        return
+    if args.only == 'served':
+        print(json.dumps(served(args, ctx, workloads)), flush=True)
+        return
     t0 = time.time()
     scene, camera, opts = scenes.plain_gpu_scene(0, W, H)
     build_times = []
@@ -232,12 +286,18 @@ def main():
         if t:
             roofline['traffic'] = t['bytes_per_launch']
             roofline['traffic_detail'] = t
-    if not args.no_cpu:
-        log('cpu baseline ...')
-        result['cpu_baseline'] = cpu_baseline(sc, cam, st, W, H, args.cpu_seconds)
     for b in (data, acc, image):
         b.free()
     gpu.release()
+    if not args.no_batched:
+        log('batched form ...')
+        result['batched'] = workloads.batched_form(ctx, log=log)
+    if not args.no_served:
+        log('served mix ...')
+        result['served'] = served(args, ctx, workloads)
+    if not args.no_cpu:
+        log('cpu baseline ...')
+        result['cpu_baseline'] = cpu_baseline(sc, cam, st, W, H, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 

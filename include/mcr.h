@@ -258,6 +258,8 @@ typedef struct mcr_request_status {
   int32_t preview_done;            /* every tile has its preview or a pass written */
   double submit_ms, admit_ms, first_preview_ms, done_ms;  /* server clock; -1 = not yet */
   uint64_t paths;                  /* samples traced for the request */
+  uint32_t preview_iterations;     /* bounce iterations from admission to the complete preview */
+  uint32_t pad0;
 } mcr_request_status;
 typedef struct mcr_server_stats {
   uint64_t closest_queries, any_queries, paths, iterations;

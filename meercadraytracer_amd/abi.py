@@ -151,7 +151,8 @@ MCR_E_CANCELLED, MCR_E_TIMEOUT = -6, -7
 class RequestStatus(C.Structure):
     _fields_ = [('state', C.c_int32), ('samples', C.c_uint32), ('max_samples', C.c_uint32),
                 ('preview_done', C.c_int32), ('submit_ms', C.c_double), ('admit_ms', C.c_double),
-                ('first_preview_ms', C.c_double), ('done_ms', C.c_double), ('paths', C.c_uint64)]
+                ('first_preview_ms', C.c_double), ('done_ms', C.c_double), ('paths', C.c_uint64),
+                ('preview_iterations', C.c_uint32), ('pad0', C.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -2,6 +2,7 @@
 // queries, tile encoding and timing. Host orchestration of the wavefront loop lives here; the
 // kernels are in trace.hip and bvh_build.hip.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -429,20 +430,32 @@ int64_t layout_job(const mcr_tile_job &j, uint32_t index, uint32_t view, uint64_
 
 namespace {
 
-// Views and jobs of one call, laid out in sample order.
+// Views and jobs of one call, laid out in sample order. Sample indices are 31-bit, so a call
+// with more samples runs as consecutive wavefront runs over consecutive groups of jobs.
 int render_views(mcr_ctx *c, const mcr_view *views, uint32_t nviews, const mcr_tile_job *jobs,
                  const uint32_t *job_views, uint32_t njobs, float *dout, mcr_counters *counters) {
   std::vector<ViewDev> vd(nviews);
   for (uint32_t v = 0; v < nviews; v++) vd[v] = make_view(views[v].scene, *views[v].camera, *views[v].settings);
-  std::vector<JobDev> jd(njobs);
+  // MCR_RUN_SAMPLES lowers the run size (tests use it to exercise the split on small calls).
+  const char *env = getenv("MCR_RUN_SAMPLES");
+  const uint64_t RUN_LIMIT = env ? std::max<uint64_t>(1, strtoull(env, nullptr, 10)) : 1ull << 30;
+  std::vector<JobDev> jd;
   uint64_t total = 0;
   for (uint32_t i = 0; i < njobs; i++) {
     uint32_t view = job_views ? job_views[i] : 0;
     if (view >= nviews) return fail(MCR_E_INVALID, "job " + std::to_string(i) + " names no view");
-    int64_t n = layout_job(jobs[i], i, view, total, jd[i]);
+    JobDev j;
+    int64_t n = layout_job(jobs[i], i, view, total, j);
     if (n < 0) return MCR_E_INVALID;
+    if (total + (uint64_t)n > RUN_LIMIT && !jd.empty()) {
+      MCR_TRY(engine_run(c->eng, vd, jd, (uint32_t)total, dout, counters));
+      jd.clear();
+      total = 0;
+      layout_job(jobs[i], i, view, 0, j);
+    }
+    if ((uint64_t)n >= RUN_LIMIT) return fail(MCR_E_OVERFLOW, "job " + std::to_string(i) + " has too many samples");
+    jd.push_back(j);
     total += (uint64_t)n;
-    if (total >= (1ull << 31)) return fail(MCR_E_OVERFLOW, "too many samples in one call");
   }
   return engine_run(c->eng, vd, jd, (uint32_t)total, dout, counters);
 }

@@ -82,6 +82,7 @@ int engine_reserve(Engine &e, uint64_t samples, uint32_t nviews, uint32_t njobs,
 // Host arrays must stay alive and unchanged until the next engine_sync.
 int engine_upload_views(Engine &e, const ViewDev *views, uint32_t n);
 int engine_upload_jobs(Engine &e, const JobDev *jobs, uint32_t first, uint32_t n);
+// The caller must have waited for the stream since the previous call (the staging is reused).
 int engine_set_control(Engine &e, float *out, uint32_t njobs, uint32_t total, bool track_done);
 int engine_zero_done(Engine &e, uint32_t first, uint32_t n);
 // Every slot free, every counter zero.
@@ -96,6 +97,9 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights);
 void engine_drop_graphs(Engine &e);
 // Copies the counter words to counts_host and waits for the stream.
 int engine_sync(Engine &e);
+// The copy alone (the caller waits), and the overflow check on the copied words.
+int engine_fetch_counts(Engine &e);
+int engine_check(Engine &e);
 // A whole call: every job's samples traced to completion, results in out.
 int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<JobDev> &jobs,
                uint32_t total, float *out, mcr_counters *counters);

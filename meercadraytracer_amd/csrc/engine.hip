@@ -73,6 +73,8 @@ int grow(T *&p, uint32_t &cap, uint32_t n) {
 }  // namespace
 
 void engine_drop_graphs(Engine &e) {
+  // A launch of the graph may still be running.
+  if (e.stream && (e.graph[0] || e.graph[1])) hipStreamSynchronize(e.stream);
   for (auto &g : e.graph) {
     if (g) hipGraphExecDestroy(g);
     g = nullptr;
@@ -139,8 +141,7 @@ int engine_upload_jobs(Engine &e, const JobDev *jobs, uint32_t first, uint32_t n
 }
 
 int engine_set_control(Engine &e, float *out, uint32_t njobs, uint32_t total, bool track_done) {
-  // The pinned staging struct may still feed an earlier copy; wait for the stream first.
-  HIP_TRY(hipStreamSynchronize(e.stream));
+  // Callers have waited for the stream since the last call, so the staging struct is free.
   *e.ctl_host = Control{e.jobs, out, track_done ? e.job_done : nullptr, njobs, total};
   HIP_TRY(hipMemcpyAsync(e.ctl, e.ctl_host, sizeof(Control), hipMemcpyHostToDevice, e.stream));
   return MCR_OK;
@@ -201,6 +202,7 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
                         (const void *)(uintptr_t)((uint64_t)e.capacity * 64u + max_lights)};
   hipGraphExec_t &graph = e.graph[v];
   if (graph && memcmp(key, e.graph_key[v], sizeof(key)) != 0) {
+    HIP_TRY(hipStreamSynchronize(e.stream));  // a launch of the old graph may still be running
     hipGraphExecDestroy(graph);
     graph = nullptr;
   }
@@ -237,9 +239,21 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
   return MCR_OK;
 }
 
-int engine_sync(Engine &e) {
+int engine_fetch_counts(Engine &e) {
   HIP_TRY(hipMemcpyAsync(e.counts_host, e.w.counts, sizeof(uint32_t) * C_WORDS, hipMemcpyDeviceToHost,
                          e.stream));
+  return MCR_OK;
+}
+
+int engine_check(Engine &e) {
+  if (e.counts_host[C_OVERFLOW])
+    return mcr_fail(MCR_E_OVERFLOW,
+                    "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
+  return MCR_OK;
+}
+
+int engine_sync(Engine &e) {
+  MCR_TRY(engine_fetch_counts(e));
   HIP_TRY(hipStreamSynchronize(e.stream));
   if (e.counts_host[C_OVERFLOW])
     return mcr_fail(MCR_E_OVERFLOW,

@@ -7,9 +7,10 @@
 //   1. frees what released or cancelled requests held, admits queued requests while the HBM
 //      budget and a request slot allow (first in, first out);
 //   2. rebuilds the job window: jobs whose samples have started stay where they are; every
-//      unstarted job is chosen afresh — the previews of newly admitted requests first, then one
-//      item per running request in turn — so a new request's preview overtakes older passes and
-//      a cancelled request's unstarted tiles simply disappear;
+//      unstarted job is chosen afresh — the previews of newly admitted requests first, then the
+//      passes of the oldest running request — so a new request's preview overtakes older passes,
+//      the wavefront works on few scenes at a time, and a cancelled request's unstarted tiles
+//      simply disappear;
 //   3. refills idle path slots and runs bounce iterations (a captured pair, or one direct);
 //   4. reads each job's finished-sample count, accumulates and encodes finished tiles in pass
 //      order (so a finished request holds the bytes raytrace() gives), and stamps the time of
@@ -64,6 +65,7 @@ struct SrvRequest {
   bool cancel = false, release = false;
   double t_submit = -1, t_admit = -1, t_preview = -1, t_done = -1;
   uint64_t paths = 0;
+  uint64_t it_admit = 0, it_preview = 0;  // bounce iterations run by the server so far
 };
 
 struct SrvJob {
@@ -159,6 +161,7 @@ struct mcr_server {
   JobDev *jobs_host = nullptr;     // pinned staging of the job list
   uint32_t *done_host = nullptr;   // pinned
   uint32_t host_cap = 0;
+  DevCounters *counters_host = nullptr;  // pinned
   // stats
   uint64_t iterations = 0, paths = 0;
   double device_ms = 0;
@@ -227,6 +230,7 @@ int admit(mcr_server *s, SrvRequest &r) {
     for (uint32_t i = 0; i < tiles.size(); i++) r.pending.push_back(Item{i, p, 1});
   r.state = MCR_REQ_RUNNING;
   r.t_admit = s->now();
+  r.it_admit = s->iterations;
   s->running.push_back(r.id);
   return MCR_OK;
 }
@@ -234,7 +238,13 @@ int admit(mcr_server *s, SrvRequest &r) {
 // Admissions, cancellations and releases (under the lock).
 int housekeeping(mcr_server *s) {
   // Encodes of the last step may still read or write what is freed below.
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  bool frees = false;
+  for (auto &kv : s->reqs) {
+    const SrvRequest &r = *kv.second;
+    frees |= r.cancel || r.release ||
+             (r.jobs_live == 0 && r.view >= 0 && r.state != MCR_REQ_RUNNING && r.state != MCR_REQ_QUEUED);
+  }
+  if (frees) HIP_TRY(hipStreamSynchronize(s->stream));
   for (auto it = s->reqs.begin(); it != s->reqs.end();) {
     SrvRequest &r = *it->second;
     if (r.cancel && (r.state == MCR_REQ_QUEUED || r.state == MCR_REQ_RUNNING)) {
@@ -271,8 +281,9 @@ int housekeeping(mcr_server *s) {
   return MCR_OK;
 }
 
-// Step 2: the job window. `next` is the first sample not yet started.
-void plan_window(mcr_server *s, uint32_t next) {
+// Step 2: the job window. `next` is the first sample not yet started; `idle` slots are free now
+// and take their samples from the window first, so the window holds that many samples more.
+void plan_window(mcr_server *s, uint32_t next, uint32_t idle) {
   std::vector<SrvJob> keep;
   std::unordered_map<uint64_t, std::vector<Item>> returned;
   uint32_t base = next;
@@ -294,7 +305,8 @@ void plan_window(mcr_server *s, uint32_t next) {
     for (auto it = kv.second.rbegin(); it != kv.second.rend(); ++it) r.pending.push_front(*it);
   }
   s->table.swap(keep);
-  uint64_t target = s->cfg.window, queued = 0;
+  // A replayed pair runs two iterations before the next step, so it needs two windows.
+  uint64_t target = (uint64_t)s->cfg.window * (s->eng.use_graph ? 2 : 1) + idle, queued = 0;
   uint32_t start = base;
   auto place = [&](SrvRequest &r) -> bool {
     Item it = r.pending.front();
@@ -327,17 +339,12 @@ void plan_window(mcr_server *s, uint32_t next) {
     while (room && r.state == MCR_REQ_RUNNING && !r.pending.empty() && r.pending.front().block > 1)
       room = place(r);
   }
-  // Then one item per running request in turn.
-  while (room && queued < target) {
-    bool any = false;
-    for (uint64_t id : s->running) {
-      SrvRequest &r = *s->reqs[id];
-      if (r.state != MCR_REQ_RUNNING || r.pending.empty()) continue;
-      if (!(room = place(r))) break;
-      any = true;
-      if (queued >= target) break;
-    }
-    if (!any) break;
+  // Then the passes of the oldest request first: the wavefront keeps working on few scenes at a
+  // time (their BVHs stay in cache), and requests finish in arrival order.
+  for (uint64_t id : s->running) {
+    SrvRequest &r = *s->reqs[id];
+    while (room && queued < target && r.state == MCR_REQ_RUNNING && !r.pending.empty()) room = place(r);
+    if (!room || queued >= target) break;
   }
   s->total = start;
 }
@@ -433,6 +440,7 @@ int deliver(mcr_server *s, bool &progress) {
     if (r.state != MCR_REQ_RUNNING) continue;
     if (r.t_preview < 0 && r.covered == r.tiles.size()) {
       r.t_preview = now;
+      r.it_preview = s->iterations;
       s->changed.notify_all();
     }
     if (r.finished_tiles == r.tiles.size() && r.pending.empty()) {
@@ -466,7 +474,7 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
   // With no path live every started sample has finished and no job in the table has started,
   // so sample numbering starts again from 0.
   uint32_t next = e.counts_host[s->q] == 0 ? 0 : std::min(e.counts_host[C_NEXT], s->total);
-  plan_window(s, next);
+  plan_window(s, next, e.counts_host[C_FREE]);
   uint32_t n = (uint32_t)s->table.size();
   if (n == 0 && e.counts_host[s->q] == 0) return deliver(s, progress);
   progress = true;
@@ -477,8 +485,7 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
   // Samples of other views may still be in flight with no job left in the table.
   mixed |= s->running.size() > 1;
   e.mix = mixed;
-  // Wait for the previous step's copies out of the pinned staging before overwriting it.
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  // The previous step waited for its copies, so the pinned staging is free.
   for (uint32_t i = 0; i < n; i++) {
     s->jobs_host[i] = s->table[i].dev;
     s->done_host[i] = s->table[i].done;
@@ -516,13 +523,17 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
       s->iterations++;
     }
   }
-  if (!rc) rc = engine_sync(e);
-  if (!rc && n) {
-    hipError_t err = hipMemcpyAsync(s->done_host, e.job_done, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
-    if (err == hipSuccess) err = hipMemcpyAsync(&s->counters, e.counters, sizeof(DevCounters), hipMemcpyDeviceToHost, s->stream);
+  // One wait per step: counter words, finished samples per job and ray counters together.
+  if (!rc) rc = engine_fetch_counts(e);
+  if (!rc) {
+    hipError_t err = hipSuccess;
+    if (n) err = hipMemcpyAsync(s->done_host, e.job_done, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+    if (err == hipSuccess) err = hipMemcpyAsync(s->counters_host, e.counters, sizeof(DevCounters), hipMemcpyDeviceToHost, s->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(s->stream);
     if (err != hipSuccess) rc = mcr_fail(MCR_E_DEVICE, std::string("job counts: ") + hipGetErrorString(err));
   }
+  if (!rc) rc = engine_check(e);
+  if (!rc) s->counters = *s->counters_host;
   lk.lock();
   s->device_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_dev).count();
   if (rc) return rc;
@@ -596,11 +607,14 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   s->arena_floats = std::max<uint64_t>((uint64_t)s->cfg.window * 4 * 4, 1ull << 24);
   HIP_TRY(hipMalloc(&s->arena_dev, s->arena_floats * sizeof(float)));
   s->arena.reset(s->arena_floats);
+  HIP_TRY(hipHostMalloc(&s->counters_host, sizeof(DevCounters), hipHostMallocDefault));
+  memset(s->counters_host, 0, sizeof(DevCounters));
   s->enc_cap = 1u << 16;
   HIP_TRY(hipHostMalloc(&s->enc_host, s->enc_cap * sizeof(EncodeItem), hipHostMallocDefault));
   HIP_TRY(hipMalloc(&s->enc_dev, s->enc_cap * sizeof(EncodeItem)));
   MCR_TRY(engine_reserve(s->eng, s->cfg.slots, s->cfg.max_requests, 1024, true));
   MCR_TRY(engine_reset(s->eng));
+  HIP_TRY(hipMemsetAsync(s->eng.counters, 0, sizeof(DevCounters), s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   memset(s->eng.counts_host, 0, sizeof(uint32_t) * C_WORDS);
   s->eng.counts_host[C_FREE] = s->eng.capacity;
@@ -634,6 +648,7 @@ void mcr_server_destroy(mcr_server *s) {
   if (s->enc_host) hipHostFree(s->enc_host);
   if (s->jobs_host) hipHostFree(s->jobs_host);
   if (s->done_host) hipHostFree(s->done_host);
+  if (s->counters_host) hipHostFree(s->counters_host);
   if (s->encoded) hipEventDestroy(s->encoded);
   if (s->copy_stream) hipStreamDestroy(s->copy_stream);
   if (s->stream) hipStreamDestroy(s->stream);
@@ -701,6 +716,7 @@ static void fill_status(const SrvRequest &r, mcr_request_status *o) {
   o->first_preview_ms = r.t_preview;
   o->done_ms = r.t_done;
   o->paths = r.paths;
+  o->preview_iterations = r.t_preview >= 0 ? (uint32_t)(r.it_preview - r.it_admit) : 0;
 }
 
 int mcr_server_status(mcr_server *s, uint64_t id, mcr_request_status *out) {

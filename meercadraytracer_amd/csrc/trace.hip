@@ -63,33 +63,12 @@ __device__ inline void unproject(const mcr_camera &c, double x, double y, double
   out[2] = m[2] * vx + m[6] * vy + m[10] * vz + m[14];
 }
 
-// Runs f(view) for this lane with its view index made wave-uniform: the lanes of one view run
-// together, one view after another (a single pass when the wave holds one view), so the view's
-// fields are read once per wave into scalar registers. Only the lanes that need a view call it.
-// The loop is driven by the mask of lanes still to serve, so it visibly makes progress (a loop
-// that only re-reads the same lane value would be assumed to exit on its first test).
-// MIX = false: every sample of the run has view 0 and the loop is not needed.
+// Runs f(view) for this lane. MIX = false: every sample of the run has view 0, which the kernel
+// reads once into scalar registers. MIX = true: each lane reads its own view; the lanes of a wave
+// mostly share one, so the reads hit the same cache lines.
 template <bool MIX, class F>
 MCR_INLINE void with_view(const ViewDev *__restrict__ views, uint32_t v, F &&f) {
-  if (!MIX) {
-    f(views[0]);
-    return;
-  }
-  uint64_t todo = __ballot(1);
-  while (todo) {
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1;
-    uint32_t cur = __builtin_amdgcn_readlane(v, leader);
-    uint64_t same = __ballot(v == cur);
-    if (v == cur) {
-      // Inside this branch cur equals v, and the compiler may index with the per-lane v again;
-      // reading the address through readfirstlane keeps the view in scalar registers.
-      uint64_t a = (uint64_t)(views + v);
-      uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
-      uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-      f(*reinterpret_cast<const ViewDev *>(((uint64_t)hi << 32) | lo));
-    }
-    todo &= ~same;
-  }
+  f(views[MIX ? v : 0]);
 }
 
 // Adds one finished sample per active lane to done[job], one atomic per distinct job in the wave.

@@ -119,6 +119,27 @@ def plain_gpu_scene(seed=0, width=1920, height=1080, quads=724, spheres=64, subd
     return scene, camera, options
 
 
+def sized_gpu_scene(seed, triangles):
+    """The plain_gpu generator scaled to about `triangles` triangles, in the same proportions as
+    the 1.38M-triangle scene: about 76% heightfield and 24% icospheres (subdivision 4, or 3 below
+    400k triangles)."""
+    sub = 4 if triangles >= 400_000 else 3
+    per = 20 * 4 ** sub
+    spheres = max(1, int(round(0.24 * triangles / per)))
+    quads = max(8, int(round(math.sqrt(max(triangles - spheres * per, 128) / 2))))
+    scene, _, options = plain_gpu_scene(seed, 64, 64, quads, spheres, sub)
+    return scene, options
+
+
+def orbit_camera(width, height, angle=0.0, radius=15.0):
+    """The plain_gpu camera moved round the scene centre (0, 0.6, -2) by `angle` radians."""
+    camera = T.PerspectiveCamera(50, width / height, 0.1, 200)
+    camera.position = [radius * math.sin(angle), 3.5, -2 + radius * math.cos(angle)]
+    camera.lookAt(0, 0.6, -2)
+    camera.updateMatrixWorld()
+    return camera
+
+
 def compiled(kind, seed=0, build_bvh=None, **kw):
     """(CompiledScene, CompiledCamera, options) of a named workload."""
     maker = {'plain_cpu': plain_cpu_scene, 'plain_gpu': plain_gpu_scene}[kind]

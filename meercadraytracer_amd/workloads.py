@@ -1,0 +1,273 @@
+"""Drivers of the batched and served workloads over the C-ABI (used by bench.py).
+
+  batched_form   256 tile jobs drawn from 16 seeded scenes (triangle counts log-uniform in
+                 [10k, 2M], plain_gpu generator), tile sizes uniform over {32,64,128,256}^2,
+                 samples log-uniform in [1, 64]; every pass of every job goes to one
+                 mcr_render_batch call (one wavefront run, bounce iterations replayed from a
+                 captured graph), with the path-slot count picked by `autotune` on a sample.
+  served_mix     Poisson arrivals over 32 resident scenes (50k-4M triangles) of requests with a
+                 resolution uniform over {320x240, 640x480, 1280x720, 1920x1080} and samples
+                 log-uniform in [4, 256]; 30% of arrivals are camera moves that cancel an
+                 in-flight render of the same scene and submit the moved view. Reports
+                 time-to-first-preview and time-to-target-samples percentiles and Mrays/s.
+Log-uniform draws are exp(U[ln a, ln b]) rounded to an integer.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import abi, native
+from .compile import compile_camera, compile_scene
+from .raytrace import trace_settings
+from .scenes import orbit_camera, sized_gpu_scene
+from .types import Tile
+
+RESOLUTIONS = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
+TILE_SIZES = [32, 64, 128, 256]
+
+
+def log_uniform(rng, a, b, size=None):
+    return np.rint(np.exp(rng.uniform(math.log(a), math.log(b), size))).astype(np.int64)
+
+
+def resident_scenes(ctx, seed, count, lo, hi, log=print):
+    """`count` plain_gpu scenes with log-uniform triangle counts, BVHs built on the GPU."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = log_uniform(rng, lo, hi, count)
+    out = []
+    t0 = time.time()
+    for i, n in enumerate(sizes):
+        scene, options = sized_gpu_scene(seed * 1000 + i, int(n))
+        sc = compile_scene(scene, ctx.build_bvh)
+        gpu = native.Scene(ctx)
+        gpu.set_geometry(sc.geometry, 1)
+        gpu.set_shading(sc.shading, 1)
+        out.append((gpu, int(sc.geometry.triangleCount), options))
+    log(f'{count} scenes, {sum(n for _, n, _ in out):,} triangles, built in {time.time() - t0:.1f} s')
+    return out
+
+
+# ---- batched form --------------------------------------------------------------------------------
+def batched_jobs(rng, scenes, n_jobs=256, width=1920, height=1080):
+    """(views, jobs, job_views, floats, samples): every pass of every tile job."""
+    views = []
+    for gpu, _, options in scenes:
+        cam = compile_camera(orbit_camera(width, height, float(rng.uniform(-0.5, 0.5))))
+        views.append((gpu, cam, trace_settings(options)))
+    jobs, jv, off, samples = [], [], 0, 0
+    for _ in range(n_jobs):
+        v = int(rng.integers(len(scenes)))
+        tw, th = int(rng.choice(TILE_SIZES)), int(rng.choice(TILE_SIZES))
+        x, y = int(rng.integers(0, width - tw + 1)), int(rng.integers(0, height - th + 1))
+        spp = int(log_uniform(rng, 1, 64))
+        for p in range(spp):
+            jobs.append(abi.TileJob(Tile(x, y, tw, th).to_c(), width, height, p, 1, off))
+            jv.append(v)
+            off += tw * th * 4
+            samples += tw * th
+    return views, jobs, jv, off, samples
+
+
+def run_batch(ctx, views, jobs, jv, out_ptr, capacity, graph):
+    ctx.set_capacity(capacity)
+    ctx.set_graph(graph)
+    cnt = abi.Counters()
+    t = time.perf_counter()
+    native.render_batch(ctx, views, jobs, jv, out=out_ptr, out_device=True, counters=cnt)
+    return time.perf_counter() - t, cnt
+
+
+def autotune(ctx, views, jobs, jv, out_ptr, log=print,
+             capacities=(1 << 19, 1 << 20, 1 << 21, 1 << 22), graphs=(True, False)):
+    """Times each (path slots, graph replay) choice on the first quarter of the jobs and returns
+    the fastest, with the trial table."""
+    sample = jobs[:max(1, len(jobs) // 4)]
+    trials = []
+    for cap in capacities:
+        for g in graphs:
+            run_batch(ctx, views, sample, jv[:len(sample)], out_ptr, cap, g)  # allocate, capture
+            dt, cnt = run_batch(ctx, views, sample, jv[:len(sample)], out_ptr, cap, g)
+            rate = (cnt.closest_queries + cnt.any_queries) / dt / 1e6
+            trials.append({'slots': cap, 'graph': g, 'mrays_s': round(rate, 1)})
+    best = max(trials, key=lambda t: t['mrays_s'])
+    log(f'autotune: {best} of {len(trials)} trials')
+    return best, trials
+
+
+def batched_form(ctx, seed=1, n_jobs=256, n_scenes=16, log=print, tune=True):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scenes = resident_scenes(ctx, seed, n_scenes, 10_000, 2_000_000, log)
+    views, jobs, jv, floats, samples = batched_jobs(rng, scenes, n_jobs)
+    out = native.DeviceBuffer(ctx, floats * 4)
+    try:
+        best, trials = (autotune(ctx, views, jobs, jv, out.ptr, log) if tune
+                        else ({'slots': 1 << 21, 'graph': True}, []))
+        run_batch(ctx, views, jobs, jv, out.ptr, best['slots'], best['graph'])  # warm
+        dt, cnt = run_batch(ctx, views, jobs, jv, out.ptr, best['slots'], best['graph'])
+    finally:
+        out.free()
+        ctx.set_capacity(1 << 21)
+        ctx.set_graph(True)
+        for gpu, _, _ in scenes:
+            gpu.release()
+    rays = cnt.closest_queries + cnt.any_queries
+    return {'jobs': n_jobs, 'tile_passes': len(jobs), 'scenes': n_scenes,
+            'triangles': int(sum(n for _, n, _ in scenes)),
+            'samples': samples, 'rays': int(rays), 'seconds': round(dt, 4),
+            'mrays_s': round(rays / dt / 1e6, 2), 'paths_per_s': round(samples / dt, 1),
+            'slots': best['slots'], 'graph': best['graph'], 'autotune': trials}
+
+
+def batched_equivalent(ctx, scenes, specs, slots=1 << 21, log=print):
+    """The plain batched form of a served run's content: every request in `specs` (scene index,
+    width, height, samples, camera angle), each as 64-px tiles x passes, all in one
+    mcr_render_batch call. Results are overwritten pass after pass (only the time matters)."""
+    views, jobs, jv = [], [], []
+    for i, (k, w, h, spp, angle) in enumerate(specs):
+        gpu, _, options = scenes[k]
+        views.append((gpu, compile_camera(orbit_camera(w, h, angle)), trace_settings(options)))
+        tiles = [(x, y, min(64, w - x), min(64, h - y)) for y in range(0, h, 64) for x in range(0, w, 64)]
+        for p in range(spp):
+            for (x, y, tw, th) in tiles:
+                jobs.append(abi.TileJob(Tile(x, y, tw, th).to_c(), w, h, p, 1, (y * w + x * th) * 4))
+                jv.append(i)
+    out = native.DeviceBuffer(ctx, 1920 * 1080 * 16)
+    try:
+        n = max(1, len(jobs) // 16)
+        run_batch(ctx, views, jobs[:n], jv[:n], out.ptr, slots, False)  # allocate
+        dt, cnt = run_batch(ctx, views, jobs, jv, out.ptr, slots, False)
+    finally:
+        out.free()
+        ctx.set_capacity(1 << 21)
+        ctx.set_graph(True)
+    rays = cnt.closest_queries + cnt.any_queries
+    log(f'batched equivalent: {len(jobs)} tile passes of {len(specs)} requests in {dt:.2f} s')
+    return {'requests': len(specs), 'tile_passes': len(jobs), 'slots': slots, 'seconds': round(dt, 3),
+            'mrays_s': round(rays / dt / 1e6, 2)}
+
+
+def autotune_served(ctx, scenes, target_ms=50.0, preview_iterations=12, candidates=(1 << 21, 1 << 20, 1 << 19, 1 << 18),
+                    log=print):
+    """Path slots for the served form: the most slots whose bounce iteration is short enough that a
+    preview (about `preview_iterations` iterations deep, plus one step of queueing) lands within
+    `target_ms`. Iteration times are measured on two full-HD passes of the largest scene."""
+    k = max(range(len(scenes)), key=lambda i: scenes[i][1])
+    spec = [(k, 1920, 1080, 2, 0.0)]
+    trials = []
+    for slots in candidates:
+        ctx.set_timing(True)
+        try:
+            batched_equivalent(ctx, scenes, spec, slots=slots, log=lambda *_: None)
+            t = ctx.last_timing()
+        finally:
+            ctx.set_timing(False)
+        per = t['total_ms'] / max(1, t['closest_launches'])
+        trials.append({'slots': slots, 'ms_per_iteration': round(per, 3),
+                       'predicted_ttfp_ms': round((preview_iterations + 2) * per, 1)})
+    ok = [t for t in trials if t['predicted_ttfp_ms'] <= target_ms]
+    best = max(ok, key=lambda t: t['slots']) if ok else min(trials, key=lambda t: t['slots'])
+    log(f'served autotune: {best}')
+    return best, trials
+
+
+# ---- served form ---------------------------------------------------------------------------------
+def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_frac=0.3,
+               budget=200 << 30, slots=0, window=0, max_requests=1024, drain_seconds=60.0, log=print):
+    """Runs the arrival process for `seconds`, waits up to `drain_seconds` for the admitted
+    requests, cancels what is left, and returns the latency and throughput figures."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    own = scenes is None
+    if own:
+        scenes = resident_scenes(ctx, seed, n_scenes, 50_000, 4_000_000, log)
+    srv = native.Server(ctx, hbm_budget=budget, slots=slots, window=window, max_requests=max_requests)
+    try:
+        for gpu, _, _ in scenes:
+            srv.add_scene(gpu)
+        live = []          # (rid, scene index, w, h, spp, angle)
+        record = {}        # rid -> kind
+        t_next = 0.0
+        stats0 = srv.stats()
+        t0 = time.perf_counter()
+        submitted = restarts = 0
+        while True:
+            t_next += float(rng.exponential(1.0 / rate))
+            if t_next > seconds:
+                break
+            delay = t0 + t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            live = [x for x in live if srv.status(x[0])['state'] in (abi.REQ_QUEUED, abi.REQ_RUNNING)]
+            if live and rng.random() < restart_frac:
+                old = live.pop(int(rng.integers(len(live))))
+                srv.cancel(old[0])
+                k, w, h, spp, angle = old[1:]
+                angle += float(rng.uniform(-0.05, 0.05))
+                kind = 'restart'
+                restarts += 1
+            else:
+                k = int(rng.integers(len(scenes)))
+                w, h = RESOLUTIONS[int(rng.integers(len(RESOLUTIONS)))]
+                spp = int(log_uniform(rng, 4, 256))
+                angle = float(rng.uniform(-0.6, 0.6))
+                kind = 'new'
+            gpu, _, options = scenes[k]
+            cam = compile_camera(orbit_camera(w, h, angle))
+            rid = srv.submit(gpu, cam, trace_settings(options), w, h, spp, 64, 4)
+            live.append((rid, k, w, h, spp, angle))
+            record[rid] = (kind, (k, w, h, spp, angle))
+            submitted += 1
+        t_arrivals = time.perf_counter() - t0
+        deadline = time.perf_counter() + drain_seconds
+        for rid in record:
+            left = (deadline - time.perf_counter()) * 1e3
+            if left <= 0:
+                break
+            try:
+                srv.wait(rid, left)
+            except native.NativeError as e:
+                if e.code != abi.MCR_E_TIMEOUT:
+                    raise
+                break
+        t_end = time.perf_counter() - t0
+        stats1 = srv.stats()
+        for rid in record:
+            if srv.status(rid)['state'] in (abi.REQ_QUEUED, abi.REQ_RUNNING):
+                srv.cancel(rid)
+        ttfp, ttt, cancelled_before_preview, unfinished, wait, iters = [], [], 0, 0, [], []
+        for rid in record:
+            st = srv.status(rid)
+            if st['first_preview_ms'] >= 0:
+                ttfp.append(st['first_preview_ms'] - st['submit_ms'])
+                wait.append(st['admit_ms'] - st['submit_ms'])
+                iters.append(st['preview_iterations'])
+            elif st['state'] == abi.REQ_CANCELLED:
+                cancelled_before_preview += 1
+            if st['state'] == abi.REQ_DONE:
+                ttt.append(st['done_ms'] - st['submit_ms'])
+            elif st['state'] != abi.REQ_CANCELLED:
+                unfinished += 1
+        rays = (stats1['closest_queries'] + stats1['any_queries']) - (stats0['closest_queries'] + stats0['any_queries'])
+        wall = max(stats1['uptime_ms'] - stats0['uptime_ms'], 1e-9)
+        device = max(stats1['device_ms'] - stats0['device_ms'], 1e-9)
+
+        def pct(a, q):
+            return round(float(np.percentile(a, q)), 2) if a else None
+        completed_specs = [record[rid][1] for rid in record if srv.status(rid)['state'] == abi.REQ_DONE]
+        return {'completed_specs': completed_specs, 'rate_per_s': rate, 'arrival_seconds': round(t_arrivals, 2), 'submitted': submitted,
+                'restarts': restarts, 'previews': len(ttfp), 'completed': len(ttt),
+                'cancelled_before_preview': cancelled_before_preview, 'unfinished_at_drain': unfinished,
+                'ttfp_ms_p50': pct(ttfp, 50), 'ttfp_ms_p99': pct(ttfp, 99),
+                'time_to_target_ms_p50': pct(ttt, 50), 'time_to_target_ms_p99': pct(ttt, 99),
+                'admit_wait_ms_p50': pct(wait, 50), 'preview_iterations_p50': pct(iters, 50),
+                'preview_iterations_p99': pct(iters, 99),
+                'iterations': int(stats1['iterations'] - stats0['iterations']),
+                'ms_per_iteration': round(device / max(1, stats1['iterations'] - stats0['iterations']), 3),
+                'mrays_s': round(rays / (wall / 1e3) / 1e6, 2),
+                'mrays_s_device_busy': round(rays / (device / 1e3) / 1e6, 2), 'wall_seconds': round(t_end, 2),
+                'paths': int(stats1['paths'] - stats0['paths'])}
+    finally:
+        srv.close()
+        if own:
+            for gpu, _, _ in scenes:
+                gpu.release()

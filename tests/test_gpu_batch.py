@@ -104,3 +104,19 @@ def test_graph_replay_counts_equal_direct(views):
         assert np.array_equal(g[0].view(np.uint32), got[0][0].view(np.uint32))
         assert g[1:] == got[0][1:]
     r.release()
+
+
+def test_call_split_into_runs(views, monkeypatch):
+    """A call with more samples than one run holds is traced as consecutive runs; results are the
+    same as one run."""
+    ctx = native.default_context()
+    res = [_resident(sc) for sc, *_ in views]
+    vlist = [(res[i], cam, st) for i, (sc, cam, st, w, h) in enumerate(views)]
+    jobs, jv, meta = _jobs(views)
+    whole = native.render_batch(ctx, vlist, jobs, jv)
+    monkeypatch.setenv('MCR_RUN_SAMPLES', '4000')
+    split = native.render_batch(ctx, vlist, jobs, jv)
+    monkeypatch.delenv('MCR_RUN_SAMPLES')
+    assert np.array_equal(whole.view(np.uint32), split.view(np.uint32))
+    for r in res:
+        r.release()
