@@ -286,6 +286,11 @@ int mcr_server_stats_get(mcr_server *server, mcr_server_stats *out);
 /* sum may be NULL (preview tiles). image is width*height*4 bytes, straight alpha. */
 int mcr_encode_tile(mcr_ctx *ctx, uint8_t *image, uint32_t width, mcr_tile tile,
                     const float *data, float *sum, uint32_t count, int srgb, int device_ptrs);
+/* As mcr_encode_tile (device pointers) with the running sum kept in FP64 (width*height*4
+ * doubles). No counterpart in the reference, whose Float32Array sum (progressive.ts:200) stops
+ * resolving late samples of very long renders: used for renders beyond 1024 samples. */
+int mcr_encode_tile64(mcr_ctx *ctx, uint8_t *image, uint32_t width, mcr_tile tile,
+                      const float *data, double *sum, uint32_t count, int srgb);
 
 /* ---- ray queries: intersectClosest / intersectAny (src/bvh.ts:202-332) over ray batches --- */
 /* rays: n*6 doubles (origin, direction). Outputs per ray: triangle (-1 = none), t, u, v, front.

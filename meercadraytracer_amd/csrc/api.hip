@@ -529,7 +529,20 @@ int mcr_encode_tile(mcr_ctx *c, uint8_t *image, uint32_t width, mcr_tile tile, c
   if (count < 1) return fail(MCR_E_INVALID, "count must be at least 1");
   std::lock_guard<std::mutex> lock(c->mu);
   HIP_TRY(hipSetDevice(c->device));
-  launch_encode(c->stream, image, width, tile, data, sum, count, srgb);
+  launch_encode(c->stream, image, width, tile, data, sum, nullptr, count, srgb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCR_OK;
+}
+
+int mcr_encode_tile64(mcr_ctx *c, uint8_t *image, uint32_t width, mcr_tile tile, const float *data,
+                      double *sum, uint32_t count, int srgb) {
+  if (!c || !image || !data || !sum) return fail(MCR_E_INVALID, "null argument");
+  if (tile.width <= 0 || tile.height <= 0) return MCR_OK;
+  if (count < 1) return fail(MCR_E_INVALID, "count must be at least 1");
+  std::lock_guard<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  launch_encode(c->stream, image, width, tile, data, nullptr, sum, count, srgb);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCR_OK;

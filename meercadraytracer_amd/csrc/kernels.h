@@ -31,6 +31,6 @@ void launch_probe_any_count(hipStream_t s, const DevScene &sc, const double *ray
                             int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
 void launch_encode_batch(hipStream_t s, const EncodeItem *items, uint32_t n, uint32_t blocks);
 void launch_encode(hipStream_t s, uint8_t *image, uint32_t width, const mcr_tile &tile,
-                   const float *data, float *sum, uint32_t count, int srgb);
+                   const float *data, float *sum, double *sum64, uint32_t count, int srgb);
 
 #endif

@@ -181,7 +181,13 @@ struct mcr_server {
 
 namespace {
 
-uint64_t request_bytes(const mcr_request &r) { return (uint64_t)r.width * r.height * 20; }
+// Renders beyond this many samples keep their running sum in FP64.
+const uint32_t WIDE_SUM_SAMPLES = 1024;
+
+uint64_t sum_bytes(const mcr_request &r) {
+  return (uint64_t)r.width * r.height * (r.samples > WIDE_SUM_SAMPLES ? 32 : 16);
+}
+uint64_t request_bytes(const mcr_request &r) { return (uint64_t)r.width * r.height * 4 + sum_bytes(r); }
 
 // Frees a terminal request's device memory and slot once nothing of it is in flight.
 void retire(mcr_server *s, SrvRequest &r) {
@@ -196,7 +202,7 @@ void retire(mcr_server *s, SrvRequest &r) {
   } else {
     hipFree(r.sum);  // the image stays readable until released
     r.sum = nullptr;
-    s->bytes_used -= (uint64_t)r.r.width * r.r.height * 16;
+    s->bytes_used -= sum_bytes(r.r);
     r.bytes = (uint64_t)r.r.width * r.r.height * 4;
   }
   s->view_free.push_back(r.view);
@@ -207,14 +213,15 @@ void retire(mcr_server *s, SrvRequest &r) {
 int admit(mcr_server *s, SrvRequest &r) {
   uint64_t npx = (uint64_t)r.r.width * r.r.height;
   HIP_TRY(hipMalloc(&r.image, npx * 4));
-  hipError_t e = hipMalloc(&r.sum, npx * 16);
+  uint64_t sb = sum_bytes(r.r);
+  hipError_t e = hipMalloc(&r.sum, sb);
   if (e != hipSuccess) {
     hipFree(r.image);
     r.image = nullptr;
     return mcr_fail(MCR_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
   HIP_TRY(hipMemsetAsync(r.image, 0, npx * 4, s->stream));
-  HIP_TRY(hipMemsetAsync(r.sum, 0, npx * 16, s->stream));
+  HIP_TRY(hipMemsetAsync(r.sum, 0, sb, s->stream));
   r.view = s->view_free.back();
   s->view_free.pop_back();
   s->views[r.view] = make_view(r.r.scene, r.r.camera, r.r.settings);
@@ -399,7 +406,10 @@ int deliver(mcr_server *s, bool &progress) {
           t.samples++;
           if (t.samples == r.r.samples) r.finished_tiles++;
         }
-        s->enc_host[nenc++] = EncodeItem{r.image, preview ? nullptr : r.sum, s->arena_dev + data, r.r.width,
+        bool wide = r.r.samples > WIDE_SUM_SAMPLES;
+        s->enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
+                                         preview || !wide ? nullptr : reinterpret_cast<double *>(r.sum),
+                                         s->arena_dev + data, r.r.width,
                                          t.t.x, t.t.y, t.t.width, t.t.height, preview ? 1u : t.samples,
                                          r.r.srgb, 0};
         if (!t.covered) {

@@ -1012,13 +1012,23 @@ __device__ inline uint8_t to_uint8_clamp(double v) {
 
 // One pixel of writeTile: sum += sample (the array keeps the float, the frame the double), mean,
 // un-premultiply, sRGB encode, ToUint8Clamp(v * 255 + 0.5).
+// sum64, when given instead of sum, keeps the running sum in FP64 (renders beyond 1024 samples,
+// where a float sum would round away the late samples' contribution).
 MCR_INLINE void encode_pixel(uint8_t *image, uint32_t width, int32_t tx, int32_t ty, int32_t tw,
-                             const float *data, float *sum, uint32_t count, int srgb, uint32_t i) {
+                             const float *data, float *sum, double *sum64, uint32_t count, int srgb,
+                             uint32_t i) {
   uint32_t x = i % (uint32_t)tw, y = i / (uint32_t)tw;
   size_t s = (size_t)i * 4;
   size_t d = (((size_t)ty + y) * width + tx + x) * 4;
   double c[4] = {data[s], data[s + 1], data[s + 2], data[s + 3]};
-  if (sum) {
+  if (sum64) {
+    double scale = 1.0 / count;
+    for (int k = 0; k < 4; k++) {
+      double v = sum64[d + k] + c[k];
+      sum64[d + k] = v;
+      c[k] = v * scale;
+    }
+  } else if (sum) {
     double scale = 1.0 / count;
     for (int k = 0; k < 4; k++) {
       double v = (double)sum[d + k] + c[k];
@@ -1038,11 +1048,11 @@ MCR_INLINE void encode_pixel(uint8_t *image, uint32_t width, int32_t tx, int32_t
 }
 
 __global__ __launch_bounds__(256) void k_encode(uint8_t *image, uint32_t width, mcr_tile tile,
-                                                const float *data, float *sum, uint32_t count,
-                                                int srgb) {
+                                                const float *data, float *sum, double *sum64,
+                                                uint32_t count, int srgb) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (uint32_t)tile.width * (uint32_t)tile.height) return;
-  encode_pixel(image, width, tile.x, tile.y, tile.width, data, sum, count, srgb, i);
+  encode_pixel(image, width, tile.x, tile.y, tile.width, data, sum, sum64, count, srgb, i);
 }
 
 // Many tiles at once: block b works on the item whose block range holds b. No two items of one
@@ -1057,7 +1067,7 @@ __global__ __launch_bounds__(256) void k_encode_batch(const EncodeItem *items, u
   const EncodeItem &it = items[lo];
   uint32_t i = (blockIdx.x - it.block_start) * blockDim.x + threadIdx.x;
   if (i >= (uint32_t)it.tw * (uint32_t)it.th) return;
-  encode_pixel(it.image, it.width, it.tx, it.ty, it.tw, it.data, it.sum, it.count, it.srgb, i);
+  encode_pixel(it.image, it.width, it.tx, it.ty, it.tw, it.data, it.sum, it.sum64, it.count, it.srgb, i);
 }
 
 // ---- launch wrappers ----------------------------------------------------------------------------
@@ -1114,8 +1124,8 @@ void launch_encode_batch(hipStream_t s, const EncodeItem *items, uint32_t n, uin
   if (n && blocks) hipLaunchKernelGGL(k_encode_batch, dim3(blocks), dim3(256), 0, s, items, n);
 }
 void launch_encode(hipStream_t s, uint8_t *image, uint32_t width, const mcr_tile &tile,
-                   const float *data, float *sum, uint32_t count, int srgb) {
+                   const float *data, float *sum, double *sum64, uint32_t count, int srgb) {
   uint64_t npx = (uint64_t)tile.width * (uint64_t)tile.height;
   hipLaunchKernelGGL(k_encode, dim3(blocks_for(npx)), dim3(256), 0, s, image, width, tile, data, sum,
-                     count, srgb);
+                     sum64, count, srgb);
 }

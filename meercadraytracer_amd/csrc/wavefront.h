@@ -112,6 +112,7 @@ struct JobDev {  // one tile job, device side
 struct EncodeItem {
   uint8_t *image;
   float *sum;
+  double *sum64;             // FP64 running sum instead of sum (renders beyond 1024 samples)
   const float *data;
   uint32_t width;
   int32_t tx, ty, tw, th;

@@ -66,6 +66,7 @@ def lib():
             'mcr_server_release': (C.c_int, [P, C.c_uint64]),
             'mcr_server_stats_get': (C.c_int, [P, P]),
             'mcr_encode_tile': (C.c_int, [P, P, C.c_uint32, abi.Tile, P, P, C.c_uint32, C.c_int, C.c_int]),
+            'mcr_encode_tile64': (C.c_int, [P, P, C.c_uint32, abi.Tile, P, P, C.c_uint32, C.c_int]),
             'mcr_intersect_closest': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P, P,
                                                 P, C.c_int, P]),
             'mcr_intersect_any_count': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, P, C.c_int, P]),
@@ -266,6 +267,11 @@ class Scene:
             self.release()
         except Exception:
             pass
+
+
+def encode_tile64_device(ctx, image_ptr, width, tile, data_ptr, sum_ptr, count, srgb):
+    """writeTile with the running sum in FP64 (renders beyond 1024 samples)."""
+    check(lib().mcr_encode_tile64(ctx.handle, image_ptr, width, tile.to_c(), data_ptr, sum_ptr, count, int(srgb)))
 
 
 def render_batch(ctx, views, jobs, job_views, out=None, out_device=False, counters=None):

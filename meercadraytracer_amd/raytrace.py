@@ -13,6 +13,9 @@ from .compile import compile_camera, compile_scene
 from .types import TraceSettings
 
 
+WIDE_SUM_SAMPLES = 1024  # renders with more passes keep their running sum in FP64
+
+
 def trace_settings(options=None):  # raytrace.ts:45-56
     o = options or {}
     return TraceSettings(
@@ -45,7 +48,8 @@ def raytrace_compiled(scene, camera, width, height, options=None, ctx=None, coun
     """raytrace.ts:69-82: `samples` passes of one full-image tile, each accumulated into the FP32
     sum and encoded as writeTile does. Passes are traced in batches (one device run streams all
     the batch's samples through the wavefront); each pass keeps its own output so the passes are
-    still summed in order, as the reference sums them."""
+    still summed in order, as the reference sums them. Beyond WIDE_SUM_SAMPLES passes the running
+    sum is kept in FP64 (mcr_encode_tile64)."""
     o = options or {}
     settings = trace_settings(o)
     samples = max(1, math.floor(o.get('samples', 16)))
@@ -57,7 +61,8 @@ def raytrace_compiled(scene, camera, width, height, options=None, ctx=None, coun
     npx = width * height
     batch = max(1, min(samples, pass_batch_bytes // (npx * 16)))
     data = native.DeviceBuffer(ctx, batch * npx * 16)
-    acc = native.DeviceBuffer(ctx, npx * 16)
+    wide = samples > WIDE_SUM_SAMPLES
+    acc = native.DeviceBuffer(ctx, npx * (32 if wide else 16))
     image = native.DeviceBuffer(ctx, npx * 4)
     acc.zero()
     image.zero()
@@ -68,7 +73,10 @@ def raytrace_compiled(scene, camera, width, height, options=None, ctx=None, coun
         passes = range(p0, min(samples, p0 + batch))
         render_passes(gpu, camera, settings, width, height, passes, data.ptr, cnt)
         for i, p in enumerate(passes):
-            native.encode_tile_device(ctx, image.ptr, width, t, data.ptr + i * npx * 16, acc.ptr, p + 1, srgb)
+            if wide:
+                native.encode_tile64_device(ctx, image.ptr, width, t, data.ptr + i * npx * 16, acc.ptr, p + 1, srgb)
+            else:
+                native.encode_tile_device(ctx, image.ptr, width, t, data.ptr + i * npx * 16, acc.ptr, p + 1, srgb)
     out = image.download(np.uint8, npx * 4)
     for b in (data, acc, image):
         b.free()

@@ -48,6 +48,66 @@ def resident_scenes(ctx, seed, count, lo, hi, log=print):
     return out
 
 
+# ---- precision form: the probe-ray set ----------------------------------------------------------
+def camera_rays(camera, xs, ys, width, height):
+    """cameraRay (tracer.ts:705-727) for sub-pixel positions, in FP64 numpy: origin on the near
+    plane, unit direction towards the far plane. Returns n x 6."""
+    p = np.asarray(camera.projectionInverse, np.float64).reshape(4, 4).T  # column-major
+    m = np.asarray(camera.world, np.float64).reshape(4, 4).T
+    nx, ny = xs / width * 2 - 1, 1 - ys / height * 2
+
+    def unproject(z):
+        v = np.stack([nx, ny, np.full_like(nx, z), np.ones_like(nx)])
+        h = p @ v
+        w = (m @ np.vstack([h[:3] / h[3], np.ones_like(nx)]))[:3]
+        return w.T
+    a, b = unproject(-1.0), unproject(1.0)
+    d = b - a
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.hstack([a, d])
+
+
+def probe_rays(geometry, camera, width, height, n, seed, closest):
+    """The precision probe set: n/2 primary rays through uniform random sub-pixel positions and n/2
+    secondary rays leaving their hits (offset by the geometry epsilon along the geometric normal,
+    towards the viewer) in cosine-distributed directions. Origins and directions are rounded to
+    float32, so every consumer traces identical rays. closest(rays) -> (tri, t, ..., front) gives the
+    primary hits. Returns n x 6 float32 (primaries first)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    half = n // 2
+    prim = camera_rays(camera, rng.uniform(0, width, half), rng.uniform(0, height, half), width, height)
+    prim = prim.astype(np.float32)
+    sec = []
+    got = 0
+    src = prim
+    while got < n - half:
+        tri, t, _, _, front = closest(src.astype(np.float64))[:5]
+        ok = np.flatnonzero(tri >= 0)
+        r = src[ok].astype(np.float64)
+        tr = np.asarray(geometry.triangles, np.float64).reshape(-1, 9)[tri[ok]]
+        g = np.cross(tr[:, 3:6], tr[:, 6:9])
+        g /= np.linalg.norm(g, axis=1, keepdims=True)
+        g *= np.where(front[ok] != 0, 1.0, -1.0)[:, None]
+        # the normal facing the incoming ray
+        g *= np.where((g * r[:, 3:6]).sum(1) > 0, -1.0, 1.0)[:, None]
+        hit = r[:, :3] + r[:, 3:6] * t[ok][:, None]
+        u1, u2 = rng.random(len(ok)), rng.random(len(ok))
+        phi, rad = 2 * np.pi * u1, np.sqrt(u2)
+        lx, ly, lz = rad * np.cos(phi), rad * np.sin(phi), np.sqrt(np.maximum(0, 1 - u2))
+        sgn = np.where(g[:, 2] >= 0, 1.0, -1.0)  # Duff et al. 2017 basis
+        a = -1.0 / (sgn + g[:, 2])
+        bb = g[:, 0] * g[:, 1] * a
+        t1 = np.stack([1 + sgn * g[:, 0] ** 2 * a, sgn * bb, -sgn * g[:, 0]], 1)
+        t2 = np.stack([bb, sgn + g[:, 1] ** 2 * a, -g[:, 1]], 1)
+        d = t1 * lx[:, None] + t2 * ly[:, None] + g * lz[:, None]
+        o = hit + g * float(geometry.epsilon)
+        sec.append(np.hstack([o, d]).astype(np.float32))
+        got += len(ok)
+        src = camera_rays(camera, rng.uniform(0, width, half), rng.uniform(0, height, half), width,
+                          height).astype(np.float32)
+    return np.vstack([prim, np.vstack(sec)[:n - half]])
+
+
 # ---- batched form --------------------------------------------------------------------------------
 def batched_jobs(rng, scenes, n_jobs=256, width=1920, height=1080):
     """(views, jobs, job_views, floats, samples): every pass of every tile job."""
