@@ -163,6 +163,7 @@ typedef struct mcr_counters {
   uint64_t paths;            /* camera samples traced */
   uint64_t closest_node_visits, closest_triangle_tests;  /* split by query kind, for the */
   uint64_t any_node_visits, any_triangle_tests;          /* per-kernel algorithmic bytes   */
+  uint64_t shade_waves, mixed_view_waves;  /* batched runs: shade waves, and those mixing views */
 } mcr_counters;
 
 typedef struct mcr_ctx mcr_ctx;
@@ -263,6 +264,7 @@ typedef struct mcr_request_status {
 } mcr_request_status;
 typedef struct mcr_server_stats {
   uint64_t closest_queries, any_queries, paths, iterations;
+  uint64_t shade_waves, mixed_view_waves;  /* shade waves, and those holding several requests */
   uint64_t bytes_in_use, requests_live, requests_queued;
   double device_ms;                /* time spent waiting for the device */
   double uptime_ms;

@@ -119,6 +119,7 @@ class Counters(C.Structure):
         ('triangle_tests', C.c_uint64), ('paths', C.c_uint64),
         ('closest_node_visits', C.c_uint64), ('closest_triangle_tests', C.c_uint64),
         ('any_node_visits', C.c_uint64), ('any_triangle_tests', C.c_uint64),
+        ('shade_waves', C.c_uint64), ('mixed_view_waves', C.c_uint64),
     ]
 
     def as_dict(self):
@@ -160,7 +161,8 @@ class RequestStatus(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [('closest_queries', C.c_uint64), ('any_queries', C.c_uint64), ('paths', C.c_uint64),
-                ('iterations', C.c_uint64), ('bytes_in_use', C.c_uint64), ('requests_live', C.c_uint64),
+                ('iterations', C.c_uint64), ('shade_waves', C.c_uint64), ('mixed_view_waves', C.c_uint64),
+                ('bytes_in_use', C.c_uint64), ('requests_live', C.c_uint64),
                 ('requests_queued', C.c_uint64), ('device_ms', C.c_double), ('uptime_ms', C.c_double)]
 
     def as_dict(self):

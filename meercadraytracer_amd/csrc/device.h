@@ -71,6 +71,7 @@ struct DevScene {
 struct DevCounters {  // accumulated per wave, flushed with one atomic per field
   unsigned long long closest_queries, any_queries, node_visits, leaf_visits, triangle_tests, paths;
   unsigned long long closest_node_visits, closest_triangle_tests, any_node_visits, any_triangle_tests;
+  unsigned long long shade_waves, mixed_view_waves;  // shade waves, and those holding several views
 };
 
 __device__ inline bool ref_is_leaf(int32_t ref) { return (uint32_t)ref & MCR_LEAF_BIT; }

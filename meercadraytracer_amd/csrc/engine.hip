@@ -173,7 +173,7 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
   launch_extend(sm, e.mix, e.views, w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(live)), e.spill,
                 TRAVERSE_THREADS, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[1], sm));
-  launch_shade(sm, e.mix, e.views, w, q, live);
+  launch_shade(sm, e.mix, e.views, w, q, live, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[2], sm));
   uint64_t max_shadow = (uint64_t)live * max_lights;
   if (max_shadow)
@@ -218,7 +218,7 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
       for (int q = 0; q < 2; q++) {
         launch_extend(cs, e.mix, e.views, e.w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(cap)), e.spill,
                       TRAVERSE_THREADS, e.counters);
-        launch_shade(cs, e.mix, e.views, e.w, q, cap);
+        launch_shade(cs, e.mix, e.views, e.w, q, cap, e.counters);
         if (max_lights)
           launch_shadow(cs, e.mix, e.views, e.w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of((uint64_t)cap * max_lights)),
                         e.spill, TRAVERSE_THREADS, e.seen, e.seen_meta, e.counters);
@@ -272,6 +272,8 @@ void engine_add_counters(const DevCounters &hc, uint64_t paths, mcr_counters *co
   counters->closest_triangle_tests += hc.closest_triangle_tests;
   counters->any_node_visits += hc.any_node_visits;
   counters->any_triangle_tests += hc.any_triangle_tests;
+  counters->shade_waves += hc.shade_waves;
+  counters->mixed_view_waves += hc.mixed_view_waves;
 }
 
 int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<JobDev> &jobs,

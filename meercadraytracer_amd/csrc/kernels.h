@@ -16,7 +16,7 @@ void launch_refill(hipStream_t s, bool mix, const ViewDev *views, const Wavefron
 void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
 void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
-                  uint32_t max_live);
+                  uint32_t max_live, DevCounters *cnt);
 void launch_shadow(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
                    DevCounters *cnt);

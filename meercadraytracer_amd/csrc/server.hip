@@ -794,6 +794,8 @@ int mcr_server_stats_get(mcr_server *s, mcr_server_stats *out) {
   out->any_queries = s->counters.any_queries;
   out->paths = s->paths;
   out->iterations = s->iterations;
+  out->shade_waves = s->counters.shade_waves;
+  out->mixed_view_waves = s->counters.mixed_view_waves;
   out->bytes_in_use = s->bytes_used;
   out->requests_live = s->running.size();
   out->requests_queued = s->admit_queue.size();

@@ -668,7 +668,8 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
 }
 
 template <bool MIX>
-__global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views, Wavefront w, int q) {
+__global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views, Wavefront w, int q,
+                                               DevCounters *cnt) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = w.counts[q];
   if (gid == 0) {
@@ -682,9 +683,22 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
   size_t P = w.capacity;
   int slot = live ? w.queue[q][gid] : 0;
   uint32_t nshadow = 0, nl = 0;
+  uint32_t view = live ? (uint32_t)w.i[I_VIEW * P + slot] : 0;
+  if (MIX) {  // how often a wave holds samples of several views
+    uint64_t lanes = __ballot(live);
+    if (lanes) {
+      int leader = __ffsll((unsigned long long)lanes) - 1;
+      uint32_t v0 = __shfl(view, leader, 64);
+      bool mixed = __ballot(live && view != v0) != 0;
+      if ((int)(threadIdx.x & 63) == leader) {
+        atomicAdd(&cnt->shade_waves, 1ull);
+        if (mixed) atomicAdd(&cnt->mixed_view_waves, 1ull);
+      }
+    }
+  }
   if (live) {
     const Control &ctl = *w.ctl;
-    with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
+    with_view<MIX>(views, view, [&](const ViewDev &V) {
       nshadow = shade_slot(V, w, ctl, slot);
       nl = V.light_slots;
     });
@@ -1095,8 +1109,8 @@ void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefron
   MCR_LAUNCH_MIX(k_extend, grid, s, views, w, q, spill, spill_stride, cnt);
 }
 void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
-                  uint32_t max_live) {
-  MCR_LAUNCH_MIX(k_shade, blocks_for(max_live ? max_live : 1), s, views, w, q);
+                  uint32_t max_live, DevCounters *cnt) {
+  MCR_LAUNCH_MIX(k_shade, blocks_for(max_live ? max_live : 1), s, views, w, q, cnt);
 }
 void launch_shadow(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,

@@ -204,31 +204,25 @@ def batched_equivalent(ctx, scenes, specs, slots=1 << 21, log=print):
     rays = cnt.closest_queries + cnt.any_queries
     log(f'batched equivalent: {len(jobs)} tile passes of {len(specs)} requests in {dt:.2f} s')
     return {'requests': len(specs), 'tile_passes': len(jobs), 'slots': slots, 'seconds': round(dt, 3),
-            'mrays_s': round(rays / dt / 1e6, 2)}
+            'mrays_s': round(rays / dt / 1e6, 2),
+            'mixed_view_wave_frac': round(cnt.mixed_view_waves / max(1, cnt.shade_waves), 4)}
 
 
-def autotune_served(ctx, scenes, target_ms=50.0, preview_iterations=12, candidates=(1 << 21, 1 << 20, 1 << 19, 1 << 18),
-                    log=print):
-    """Path slots for the served form: the most slots whose bounce iteration is short enough that a
-    preview (about `preview_iterations` iterations deep, plus one step of queueing) lands within
-    `target_ms`. Iteration times are measured on two full-HD passes of the largest scene."""
-    k = max(range(len(scenes)), key=lambda i: scenes[i][1])
-    spec = [(k, 1920, 1080, 2, 0.0)]
+def autotune_served(ctx, scenes, target_ms=50.0, rate=8.0, seconds=3.0,
+                    candidates=(1 << 21, 1 << 20, 1 << 19, 1 << 18), log=print):
+    """Path slots for the served form: a short run of the mix at each candidate slot count, from the
+    most slots down; the first whose p99 time-to-first-preview meets `target_ms` wins (more slots
+    give more throughput, fewer give shorter bounce iterations and so earlier previews)."""
     trials = []
     for slots in candidates:
-        ctx.set_timing(True)
-        try:
-            batched_equivalent(ctx, scenes, spec, slots=slots, log=lambda *_: None)
-            t = ctx.last_timing()
-        finally:
-            ctx.set_timing(False)
-        per = t['total_ms'] / max(1, t['closest_launches'])
-        trials.append({'slots': slots, 'ms_per_iteration': round(per, 3),
-                       'predicted_ttfp_ms': round((preview_iterations + 2) * per, 1)})
-    ok = [t for t in trials if t['predicted_ttfp_ms'] <= target_ms]
-    best = max(ok, key=lambda t: t['slots']) if ok else min(trials, key=lambda t: t['slots'])
-    log(f'served autotune: {best}')
-    return best, trials
+        r = served_mix(ctx, rate, seconds, seed=7, scenes=scenes, slots=slots, drain_seconds=2.0,
+                       log=lambda *_: None)
+        trials.append({'slots': slots, 'ttfp_ms_p99': r['ttfp_ms_p99'], 'mrays_s': r['mrays_s'],
+                       'ms_per_iteration': r['ms_per_iteration']})
+        log(f'served autotune: {trials[-1]}')
+        if r['ttfp_ms_p99'] is not None and r['ttfp_ms_p99'] <= target_ms:
+            return trials[-1], trials
+    return trials[-1], trials
 
 
 # ---- served form ---------------------------------------------------------------------------------
@@ -322,6 +316,8 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
                 'admit_wait_ms_p50': pct(wait, 50), 'preview_iterations_p50': pct(iters, 50),
                 'preview_iterations_p99': pct(iters, 99),
                 'iterations': int(stats1['iterations'] - stats0['iterations']),
+                'mixed_view_wave_frac': round((stats1['mixed_view_waves'] - stats0['mixed_view_waves']) /
+                                              max(1, stats1['shade_waves'] - stats0['shade_waves']), 4),
                 'ms_per_iteration': round(device / max(1, stats1['iterations'] - stats0['iterations']), 3),
                 'mrays_s': round(rays / (wall / 1e3) / 1e6, 2),
                 'mrays_s_device_busy': round(rays / (device / 1e3) / 1e6, 2), 'wall_seconds': round(t_end, 2),
