@@ -48,6 +48,13 @@ struct BNode {
   uint32_t bkey[6], ckey[6];
 };
 
+// What k_decide chose for the node in an active slot; k_create makes the children from it.
+struct Decision {
+  int32_t kind;     // 0 SAH leaf, 1 binned split, 2 half split
+  int32_t axis, split, nl;
+  uint32_t ckey[2][6], bkey[2][6];
+};
+
 // Counter words in device memory.
 enum { C_NODES = 0, C_NEXT = 1, C_LEAVES = 2, C_BOUNDS = 3, C_ERROR = 4, C_CHUNKS = 5, C_COUNT = 8 };
 
@@ -124,6 +131,37 @@ __global__ __launch_bounds__(256) void k_node_bounds(BNode *nodes, const int32_t
   else if (threadIdx.x < 12) nd.ckey[threadIdx.x - 6] = k[threadIdx.x];
 }
 
+// Bounds of one large node (the root) over many blocks: per-block LDS reduction, then one global
+// atomic per key; the node's keys start at +inf / -inf (k_root).
+__global__ __launch_bounds__(256) void k_node_bounds_wide(BNode *nodes, int32_t node, Prims P,
+                                                          uint32_t first, uint32_t n) {
+  __shared__ uint32_t k[12];
+  if (threadIdx.x < 12) k[threadIdx.x] = ((threadIdx.x / 3) % 2 == 0) ? KEY_POS_INF : KEY_NEG_INF;
+  __syncthreads();
+  uint32_t m[12];
+  for (int j = 0; j < 12; j++) m[j] = ((j / 3) % 2 == 0) ? KEY_POS_INF : KEY_NEG_INF;
+  for (uint32_t p = first + blockIdx.x * blockDim.x + threadIdx.x; p < first + n; p += gridDim.x * blockDim.x) {
+    for (int a = 0; a < 3; a++) {
+      m[a] = min(m[a], fkey(P.get(a, p)));
+      m[3 + a] = max(m[3 + a], fkey(P.get(3 + a, p)));
+      uint32_t c = fkey(P.get(6 + a, p));
+      m[6 + a] = min(m[6 + a], c);
+      m[9 + a] = max(m[9 + a], c);
+    }
+  }
+  for (int j = 0; j < 12; j++) {
+    if ((j / 3) % 2 == 0) atomicMin(&k[j], m[j]);
+    else atomicMax(&k[j], m[j]);
+  }
+  __syncthreads();
+  BNode &nd = nodes[node];
+  if (threadIdx.x < 12) {
+    uint32_t *dst = threadIdx.x < 6 ? &nd.bkey[threadIdx.x] : &nd.ckey[threadIdx.x - 6];
+    if ((threadIdx.x / 3) % 2 == 0) atomicMin(dst, k[threadIdx.x]);
+    else atomicMax(dst, k[threadIdx.x]);
+  }
+}
+
 __global__ void k_bins_init(uint32_t *bins, uint32_t words) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= words) return;
@@ -150,41 +188,19 @@ __device__ inline int bin_of(double c, double lo, double scale) {  // bvh.ts:102
   return (int)js_min(BINS - 1, jm_floor((c - lo) * scale));
 }
 
-// Binning over chunks of active nodes: LDS bins, then one global atomic per non-empty word.
-__global__ __launch_bounds__(256) void k_bin(const int4 *chunks, const int32_t *active, BNode *nodes,
-                                             Prims P, uint32_t *bins) {
-  __shared__ uint32_t lb[BIN_WORDS];
-  int4 ch = chunks[blockIdx.x];  // (slot, start, len, 0)
-  const BNode &nd = nodes[active[ch.x]];
+// Binning over chunks of active nodes: LDS bins, then one global atomic per non-empty word. A
+// block takes BIN_GROUP consecutive chunks and merges into global memory only when the node changes
+// (measured on 1.38M triangles: grouping 8 chunks gained nothing, the LDS atomics dominate).
+const int BIN_GROUP = 1;
+
+__device__ inline void bins_lds_init(uint32_t *lb) {
   for (int i = threadIdx.x; i < BIN_WORDS; i += blockDim.x) {
     int f = i % NB;
     lb[i] = f == 0 ? 0u : ((f - 1) / 3 % 2 == 0 ? KEY_POS_INF : KEY_NEG_INF);
   }
-  __syncthreads();
-  NodeGeom g = node_geom(nd);
-  if ((int)threadIdx.x < ch.z) {
-    uint32_t p = (uint32_t)(ch.y + threadIdx.x);
-    uint32_t kb[6], kc[3];
-    for (int a = 0; a < 3; a++) {
-      kb[a] = fkey(P.get(a, p));
-      kb[3 + a] = fkey(P.get(3 + a, p));
-      kc[a] = fkey(P.get(6 + a, p));
-    }
-    for (int a = 0; a < 3; a++) {
-      if (!g.valid[a]) continue;
-      int b = bin_of((double)P.get(6 + a, p), g.lo[a], g.scale[a]);
-      uint32_t *w = lb + (a * BINS + b) * NB;
-      atomicAdd(&w[0], 1u);
-      for (int j = 0; j < 3; j++) {
-        atomicMin(&w[1 + j], kb[j]);
-        atomicMax(&w[4 + j], kb[3 + j]);
-        atomicMin(&w[7 + j], kc[j]);
-        atomicMax(&w[10 + j], kc[j]);
-      }
-    }
-  }
-  __syncthreads();
-  uint32_t *gb = bins + (size_t)ch.x * BIN_WORDS;
+}
+
+__device__ inline void bins_flush(const uint32_t *lb, uint32_t *gb) {
   for (int i = threadIdx.x; i < BIN_WORDS; i += blockDim.x) {
     int base = i - i % NB;
     if (lb[base] == 0) continue;
@@ -192,6 +208,53 @@ __global__ __launch_bounds__(256) void k_bin(const int4 *chunks, const int32_t *
     if (f == 0) atomicAdd(&gb[i], lb[i]);
     else if ((f - 1) / 3 % 2 == 0) atomicMin(&gb[i], lb[i]);
     else atomicMax(&gb[i], lb[i]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin(const int4 *chunks, uint32_t nchunks, const int32_t *active,
+                                             BNode *nodes, Prims P, uint32_t *bins) {
+  __shared__ uint32_t lb[BIN_WORDS];
+  uint32_t c0 = blockIdx.x * BIN_GROUP, c1 = min(c0 + BIN_GROUP, nchunks);
+  int cur = -1;
+  for (uint32_t c = c0; c < c1; c++) {
+    int4 ch = chunks[c];  // (slot, start, len, 0); uniform over the block
+    if (ch.x != cur) {
+      if (cur >= 0) {
+        __syncthreads();
+        bins_flush(lb, bins + (size_t)cur * BIN_WORDS);
+      }
+      __syncthreads();
+      bins_lds_init(lb);
+      __syncthreads();
+      cur = ch.x;
+    }
+    const BNode &nd = nodes[active[ch.x]];
+    NodeGeom g = node_geom(nd);
+    if ((int)threadIdx.x < ch.z) {
+      uint32_t p = (uint32_t)(ch.y + threadIdx.x);
+      uint32_t kb[6], kc[3];
+      for (int a = 0; a < 3; a++) {
+        kb[a] = fkey(P.get(a, p));
+        kb[3 + a] = fkey(P.get(3 + a, p));
+        kc[a] = fkey(P.get(6 + a, p));
+      }
+      for (int a = 0; a < 3; a++) {
+        if (!g.valid[a]) continue;
+        int b = bin_of((double)P.get(6 + a, p), g.lo[a], g.scale[a]);
+        uint32_t *w = lb + (a * BINS + b) * NB;
+        atomicAdd(&w[0], 1u);
+        for (int j = 0; j < 3; j++) {
+          atomicMin(&w[1 + j], kb[j]);
+          atomicMax(&w[4 + j], kb[3 + j]);
+          atomicMin(&w[7 + j], kc[j]);
+          atomicMax(&w[10 + j], kc[j]);
+        }
+      }
+    }
+  }
+  if (cur >= 0) {
+    __syncthreads();
+    bins_flush(lb, bins + (size_t)cur * BIN_WORDS);
   }
 }
 
@@ -219,117 +282,232 @@ __device__ inline void union_bins(const uint32_t *B, int from, int to, uint32_t 
   }
 }
 
-// Split decision for every active node (bvh.ts:82-171), children creation.
-__global__ void k_decide(const int32_t *active, uint32_t nact, BNode *nodes, const uint32_t *bins,
-                         uint32_t *counters, int32_t *next_active, int32_t *leaf_list,
-                         int32_t *bounds_list) {
-  uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= nact) return;
+// Scans within 16-lane segments (one axis's bins): inclusive prefix (from bin 0) and suffix (to
+// bin 15) of min, max and sum. min/max of float keys and integer sums do not depend on the order
+// they are taken in, so they equal the reference's sequential sweeps.
+__device__ inline uint32_t pre_min(uint32_t v, int l) {
+  for (int d = 1; d < 16; d <<= 1) { uint32_t o = __shfl_up(v, d, 16); if (l >= d) v = min(v, o); }
+  return v;
+}
+__device__ inline uint32_t pre_max(uint32_t v, int l) {
+  for (int d = 1; d < 16; d <<= 1) { uint32_t o = __shfl_up(v, d, 16); if (l >= d) v = max(v, o); }
+  return v;
+}
+__device__ inline int pre_sum(int v, int l) {
+  for (int d = 1; d < 16; d <<= 1) { int o = __shfl_up(v, d, 16); if (l >= d) v += o; }
+  return v;
+}
+__device__ inline uint32_t suf_min(uint32_t v, int l) {
+  for (int d = 1; d < 16; d <<= 1) { uint32_t o = __shfl_down(v, d, 16); if (l + d < 16) v = min(v, o); }
+  return v;
+}
+__device__ inline uint32_t suf_max(uint32_t v, int l) {
+  for (int d = 1; d < 16; d <<= 1) { uint32_t o = __shfl_down(v, d, 16); if (l + d < 16) v = max(v, o); }
+  return v;
+}
+__device__ inline int suf_sum(int v, int l) {
+  for (int d = 1; d < 16; d <<= 1) { int o = __shfl_down(v, d, 16); if (l + d < 16) v += o; }
+  return v;
+}
+__device__ inline uint32_t wave_min(uint32_t v) {
+  for (int d = 32; d > 0; d >>= 1) v = min(v, (uint32_t)__shfl_xor(v, d, 64));
+  return v;
+}
+__device__ inline uint32_t wave_max(uint32_t v) {
+  for (int d = 32; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+  return v;
+}
+
+// Split decision for every active node (bvh.ts:82-171) and children creation: one wave per node,
+// lane 16 * axis + bin holds one bin. Lane i of an axis computes the reference's cost of the split
+// after bin i (left = bins 0..i, right = bins i+1..15) with the same FP64 expressions; the winner
+// is the first strict minimum in (axis, split) order, as the sequential loop finds it.
+__global__ __launch_bounds__(256) void k_decide(const int32_t *active, uint32_t nact, BNode *nodes,
+                                                const uint32_t *bins, uint32_t *counters,
+                                                Decision *decisions) {
+  uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slot >= nact) return;  // uniform over the wave
+  int lane = (int)(threadIdx.x & 63);
+  int axis = lane >> 4, bin = lane & 15;
   int32_t ni = active[slot];
   BNode &nd = nodes[ni];
   int n = nd.n;
-  double nx0 = fdecode(nd.bkey[0]), ny0 = fdecode(nd.bkey[1]), nz0 = fdecode(nd.bkey[2]);
-  double nx1 = fdecode(nd.bkey[3]), ny1 = fdecode(nd.bkey[4]), nz1 = fdecode(nd.bkey[5]);
   const uint32_t *NBINS = bins + (size_t)slot * BIN_WORDS;
-  double bestCost = JM_INF;
-  int bestAxis = -1, bestSplit = -1;
-  for (int axis = 0; axis < 3; axis++) {
+  bool in_axis = axis < 3;
+  bool axis_ok = false;
+  if (in_axis) {
     double lo = fdecode(nd.ckey[axis]), extent = (double)fdecode(nd.ckey[3 + axis]) - lo;
-    if (!(extent > 0)) continue;
-    const uint32_t *B = NBINS + axis * BINS * NB;
-    double rightArea[BINS];
-    int rightCount[BINS];
-    double rx0 = JM_INF, ry0 = JM_INF, rz0 = JM_INF, rx1 = -JM_INF, ry1 = -JM_INF, rz1 = -JM_INF;
-    int rc = 0;
-    for (int i = BINS - 1; i > 0; i--) {
-      const uint32_t *w = B + i * NB;
-      rc += (int)w[0];
-      rx0 = js_min(rx0, fdecode(w[1]));
-      ry0 = js_min(ry0, fdecode(w[2]));
-      rz0 = js_min(rz0, fdecode(w[3]));
-      rx1 = js_max(rx1, fdecode(w[4]));
-      ry1 = js_max(ry1, fdecode(w[5]));
-      rz1 = js_max(rz1, fdecode(w[6]));
-      rightCount[i] = rc;
-      rightArea[i] = rc > 0 ? area(rx0, ry0, rz0, rx1, ry1, rz1) : 0;
+    axis_ok = extent > 0;
+  }
+  // This lane's bin (empty bins hold +inf / -inf keys and a zero count).
+  const uint32_t *w = NBINS + (in_axis ? (axis * BINS + bin) * NB : 0);
+  int cnt = in_axis ? (int)w[0] : 0;
+  uint32_t kb[6], kc[6];
+  for (int j = 0; j < 3; j++) {
+    kb[j] = in_axis ? w[1 + j] : KEY_POS_INF;
+    kb[3 + j] = in_axis ? w[4 + j] : KEY_NEG_INF;
+    kc[j] = in_axis ? w[7 + j] : KEY_POS_INF;
+    kc[3 + j] = in_axis ? w[10 + j] : KEY_NEG_INF;
+  }
+  int lc = pre_sum(cnt, bin), rc = suf_sum(cnt, bin);
+  uint32_t L[6], R[6];
+  for (int j = 0; j < 3; j++) {
+    L[j] = pre_min(kb[j], bin);
+    L[3 + j] = pre_max(kb[3 + j], bin);
+    R[j] = suf_min(kb[j], bin);
+    R[3 + j] = suf_max(kb[3 + j], bin);
+  }
+  // rightArea[bin] of the reference (bins bin..15), handed to the lane of the split before it.
+  double rArea = rc > 0 ? area(fdecode(R[0]), fdecode(R[1]), fdecode(R[2]), fdecode(R[3]), fdecode(R[4]),
+                               fdecode(R[5]))
+                        : 0;
+  double rAreaNext = __shfl_down(rArea, 1, 16);
+  int rcNext = __shfl_down(rc, 1, 16);
+  double cost = JM_INF;
+  bool valid = in_axis && axis_ok && bin < BINS - 1 && lc != 0 && rcNext != 0;
+  if (valid) {
+    cost = lc * area(fdecode(L[0]), fdecode(L[1]), fdecode(L[2]), fdecode(L[3]), fdecode(L[4]), fdecode(L[5])) +
+           rcNext * rAreaNext;
+    if (!(cost < JM_INF)) valid = false;  // never below the initial Infinity of the reference
+  }
+  // First strict minimum: smallest cost, then smallest (axis, split).
+  double bc = valid ? cost : JM_INF;
+  int bi = valid ? axis * BINS + bin : 1 << 30;
+  for (int d = 32; d > 0; d >>= 1) {
+    double oc = __shfl_xor(bc, d, 64);
+    int oi = __shfl_xor(bi, d, 64);
+    if (oc < bc || (oc == bc && oi < bi)) {
+      bc = oc;
+      bi = oi;
     }
-    double lx0 = JM_INF, ly0 = JM_INF, lz0 = JM_INF, lx1 = -JM_INF, ly1 = -JM_INF, lz1 = -JM_INF;
-    int lc = 0;
-    for (int i = 0; i < BINS - 1; i++) {
-      const uint32_t *w = B + i * NB;
-      lc += (int)w[0];
-      lx0 = js_min(lx0, fdecode(w[1]));
-      ly0 = js_min(ly0, fdecode(w[2]));
-      lz0 = js_min(lz0, fdecode(w[3]));
-      lx1 = js_max(lx1, fdecode(w[4]));
-      ly1 = js_max(ly1, fdecode(w[5]));
-      lz1 = js_max(lz1, fdecode(w[6]));
-      int rc2 = rightCount[i + 1];
-      if (lc == 0 || rc2 == 0) continue;
-      double cost = lc * area(lx0, ly0, lz0, lx1, ly1, lz1) + rc2 * rightArea[i + 1];
-      if (cost < bestCost) {
-        bestCost = cost;
-        bestAxis = axis;
-        bestSplit = i;
+  }
+  int bestAxis = bi < (1 << 30) ? bi / BINS : -1;
+  int bestSplit = bi < (1 << 30) ? bi % BINS : -1;
+  // Children's boxes from the prefix at the split lane and the suffix at the lane after it, and
+  // their centroid bounds by masked reductions over the winning axis.
+  uint32_t childB[2][6], childC[2][6];
+  int nlSplit = 0;
+  if (bestAxis >= 0) {
+    int src = bestAxis * BINS + bestSplit;
+    nlSplit = __shfl(lc, src, 64);
+    for (int j = 0; j < 6; j++) {
+      childB[0][j] = __shfl(L[j], src, 64);
+      childB[1][j] = __shfl(R[j], src + 1, 64);
+    }
+    bool left = axis == bestAxis && bin <= bestSplit, right = axis == bestAxis && bin > bestSplit;
+    for (int j = 0; j < 3; j++) {
+      childC[0][j] = wave_min(left ? kc[j] : KEY_POS_INF);
+      childC[0][3 + j] = wave_max(left ? kc[3 + j] : KEY_NEG_INF);
+      childC[1][j] = wave_min(right ? kc[j] : KEY_POS_INF);
+      childC[1][3 + j] = wave_max(right ? kc[3 + j] : KEY_NEG_INF);
+    }
+  }
+  if (lane != 0) return;
+  Decision &dec = decisions[slot];
+  if (bestAxis >= 0) {
+    double nx0 = fdecode(nd.bkey[0]), ny0 = fdecode(nd.bkey[1]), nz0 = fdecode(nd.bkey[2]);
+    double nx1 = fdecode(nd.bkey[3]), ny1 = fdecode(nd.bkey[4]), nz1 = fdecode(nd.bkey[5]);
+    double parentArea = area(nx0, ny0, nz0, nx1, ny1, nz1);
+    double splitCost = 1.0 + (1.0 * bc) / js_max(parentArea, 1e-30);
+    if (splitCost >= 1.0 * n && n <= 16) {
+      dec.kind = 0;
+      return;
+    }
+    if (nlSplit <= 0 || nlSplit >= n) atomicOr(&counters[C_ERROR], 1u);  // cannot happen: lc > 0, rc2 > 0
+    dec.kind = 1;
+    dec.axis = bestAxis;
+    dec.split = bestSplit;
+    dec.nl = nlSplit;
+    for (int side = 0; side < 2; side++)
+      for (int j = 0; j < 6; j++) {
+        dec.bkey[side][j] = childB[side][j];
+        dec.ckey[side][j] = childC[side][j];
+      }
+  } else {
+    dec.kind = 2;
+    dec.axis = 0;
+    dec.split = -1;
+    dec.nl = n >> 1;
+  }
+}
+
+// Exclusive prefix of v over the wave and one atomic per wave on *counter; every lane calls it.
+__device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t v) {
+  uint32_t lane = threadIdx.x & 63, x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  uint32_t total = __shfl(x, 63, 64), base = 0;
+  if (lane == 63 && total) base = atomicAdd(counter, total);
+  base = __shfl(base, 63, 64);
+  return base + x - v;
+}
+
+// Children of every split node (bvh.ts:165-171): node records in pairs, then the next level's
+// active list, the leaf list and the list of nodes whose bounds come from their primitives.
+__global__ __launch_bounds__(256) void k_create(const int32_t *active, uint32_t nact, BNode *nodes,
+                                                const Decision *decisions, uint32_t *counters,
+                                                int32_t *next_active, int32_t *leaf_list,
+                                                int32_t *bounds_list) {
+  uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  bool in = slot < nact;
+  Decision d;
+  d.kind = 0;
+  if (in) d = decisions[slot];
+  int32_t ni = in ? active[slot] : 0;
+  bool split = in && d.kind != 0;
+  uint32_t base = wave_reserve(&counters[C_NODES], split ? 2u : 0u);
+  bool leaf[2] = {false, false};
+  if (in) {
+    BNode &nd = nodes[ni];
+    if (!split) {
+      nd.flags |= NF_LEAF | NF_SAHLEAF;
+      nd.child = -1;
+    } else {
+      int n = nd.n, nl = d.nl;
+      nd.child = (int32_t)base;
+      nd.axis = d.axis;
+      nd.split = d.split;
+      nd.nl = nl;
+      nd.flags |= d.kind == 2 ? NF_HALF : NF_SPLIT;
+      for (int side = 0; side < 2; side++) {
+        BNode &c = nodes[base + side];
+        c.first = side == 0 ? nd.first : nd.first + nl;
+        c.n = side == 0 ? nl : n - nl;
+        c.depth = nd.depth + 1;
+        c.parent = ni;
+        c.child = -1;
+        c.flags = (side == 0 ? NF_LEFT : 0) | (d.kind == 2 ? NF_BOUNDS : 0);
+        c.axis = 0;
+        c.split = -1;
+        c.nl = 0;
+        c.F = 0;
+        c.ic = 0;
+        c.rank = 0;
+        c.id = 0;
+        if (d.kind == 1)
+          for (int j = 0; j < 6; j++) {
+            c.bkey[j] = d.bkey[side][j];
+            c.ckey[j] = d.ckey[side][j];
+          }
+        leaf[side] = c.n <= MAX_LEAF || c.depth >= MAX_DEPTH;
+        if (leaf[side]) c.flags |= NF_LEAF;
       }
     }
   }
-  int nl;
-  bool halfSplit = false;
-  if (bestAxis >= 0) {
-    double parentArea = area(nx0, ny0, nz0, nx1, ny1, nz1);
-    double splitCost = 1.0 + (1.0 * bestCost) / js_max(parentArea, 1e-30);
-    if (splitCost >= 1.0 * n && n <= 16) {
-      nd.flags |= NF_LEAF | NF_SAHLEAF;
-      nd.child = -1;
-      return;
-    }
-    const uint32_t *B = NBINS + bestAxis * BINS * NB;
-    nl = 0;
-    for (int i = 0; i <= bestSplit; i++) nl += (int)B[i * NB];
-    if (nl <= 0 || nl >= n) atomicOr(&counters[C_ERROR], 1u);  // cannot happen: lc > 0 and rc2 > 0
-  } else {
-    halfSplit = true;
-    bestAxis = 0;
-    nl = n >> 1;
-  }
-  uint32_t base = atomicAdd(&counters[C_NODES], 2u);
-  nd.child = (int32_t)base;
-  nd.axis = bestAxis;
-  nd.split = bestSplit;
-  nd.nl = nl;
-  nd.flags |= halfSplit ? NF_HALF : NF_SPLIT;
-  for (int side = 0; side < 2; side++) {
-    BNode &c = nodes[base + side];
-    c.first = side == 0 ? nd.first : nd.first + nl;
-    c.n = side == 0 ? nl : n - nl;
-    c.depth = nd.depth + 1;
-    c.parent = ni;
-    c.child = -1;
-    c.flags = side == 0 ? NF_LEFT : 0;
-    c.axis = 0;
-    c.split = -1;
-    c.nl = 0;
-    c.F = 0;
-    c.ic = 0;
-    c.rank = 0;
-    c.id = 0;
-    if (halfSplit) {
-      c.flags |= NF_BOUNDS;
-      uint32_t bi = atomicAdd(&counters[C_BOUNDS], 1u);
-      bounds_list[bi] = (int32_t)(base + side);
-    } else {
-      const uint32_t *B = NBINS + bestAxis * BINS * NB;
-      if (side == 0) union_bins(B, 0, bestSplit + 1, c.bkey, c.ckey);
-      else union_bins(B, bestSplit + 1, BINS, c.bkey, c.ckey);
-    }
-    if (c.n <= MAX_LEAF || c.depth >= MAX_DEPTH) {
-      c.flags |= NF_LEAF;
-      uint32_t li = atomicAdd(&counters[C_LEAVES], 1u);
-      leaf_list[li] = (int32_t)(base + side);
-    } else {
-      uint32_t ai = atomicAdd(&counters[C_NEXT], 1u);
-      next_active[ai] = (int32_t)(base + side);
+  uint32_t nb = split && d.kind == 2 ? 2u : 0u;
+  uint32_t nlv = split ? (uint32_t)leaf[0] + (uint32_t)leaf[1] : 0u;
+  uint32_t nnx = split ? 2u - nlv : 0u;
+  uint32_t bpos = wave_reserve(&counters[C_BOUNDS], nb);
+  uint32_t lpos = wave_reserve(&counters[C_LEAVES], nlv);
+  uint32_t apos = wave_reserve(&counters[C_NEXT], nnx);
+  if (split) {
+    for (int side = 0; side < 2; side++) {
+      int32_t c = (int32_t)(base + side);
+      if (nb) bounds_list[bpos++] = c;
+      if (leaf[side]) leaf_list[lpos++] = c;
+      else next_active[apos++] = c;
     }
   }
 }
@@ -340,41 +518,73 @@ __global__ void k_chunk_count(const int32_t *active, uint32_t nact, const BNode 
   cnt[s] = (uint32_t)((nodes[active[s]].n + CHUNK - 1) / CHUNK);
 }
 
-// Exclusive scan of cnt[0..n) into off, total into *total; one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(const uint32_t *cnt, uint32_t n, uint32_t *off,
-                                               uint32_t *total) {
-  __shared__ uint32_t s[1024];
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < n; base += 1024) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t v = i < n ? cnt[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-      uint32_t t = threadIdx.x >= (uint32_t)d ? s[threadIdx.x - d] : 0;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < n) off[i] = carry + s[threadIdx.x] - v;
-    uint32_t tile_total = s[1023];
-    __syncthreads();
-    carry += tile_total;
+// Exclusive scan over many blocks: each block of 1024 scans its tile and leaves the tile total;
+// one block scans the totals; every tile then adds its offset. total (if given) gets the sum.
+__device__ inline uint32_t tile_scan_1024(uint32_t v, uint32_t &tile_total) {
+  __shared__ uint32_t wsum[16];
+  uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  for (uint32_t i = 0; i < 16; i++) {
+    if (i < wid) pre += wsum[i];
+    tot += wsum[i];
+  }
+  tile_total = tot;
+  __syncthreads();
+  return pre + x - v;
 }
 
-__global__ void k_chunk_emit(const int32_t *active, uint32_t nact, BNode *nodes, const uint32_t *cnt,
-                             const uint32_t *off, int4 *chunks) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nact) return;
-  BNode &nd = nodes[active[s]];
-  nd.chunk_begin = (int32_t)off[s];
-  nd.chunk_count = (int32_t)cnt[s];
-  for (uint32_t c = 0; c < cnt[s]; c++) {
-    int start = nd.first + (int)c * CHUNK;
-    chunks[off[s] + c] = make_int4((int)s, start, min(CHUNK, nd.first + nd.n - start), 0);
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *in, uint32_t n, uint32_t *out,
+                                                     uint32_t *tile_sums) {
+  uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+  uint32_t v = i < n ? in[i] : 0, tot;
+  uint32_t e = tile_scan_1024(v, tot);
+  if (i < n) out[i] = e;
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t *tile_sums, uint32_t ntiles, uint32_t *total) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < ntiles; base += 1024) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < ntiles ? tile_sums[i] : 0, tot;
+    uint32_t e = tile_scan_1024(v, tot);
+    if (i < ntiles) tile_sums[i] = carry + e;
+    carry += tot;
   }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void k_scan_add(uint32_t *out, uint32_t n, const uint32_t *tile_sums) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += tile_sums[i / 1024];
+}
+
+// Chunk c of the level: its node is the last one whose first chunk is <= c.
+__global__ void k_chunk_emit(const int32_t *active, uint32_t nact, BNode *nodes, const uint32_t *cnt,
+                             const uint32_t *off, uint32_t nchunks, int4 *chunks) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  int lo = 0, hi = (int)nact - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  BNode &nd = nodes[active[lo]];
+  uint32_t k = c - off[lo];
+  if (k == 0) {
+    nd.chunk_begin = (int32_t)off[lo];
+    nd.chunk_count = (int32_t)cnt[lo];
+  }
+  int start = nd.first + (int)k * CHUNK;
+  chunks[c] = make_int4(lo, start, min(CHUNK, nd.first + nd.n - start), 0);
 }
 
 // Block-wide exclusive scan of 0/1 flags (256 threads).
@@ -401,7 +611,10 @@ __global__ __launch_bounds__(256) void k_part_classify(const int4 *chunks, const
                                                        uint16_t *lpre, uint32_t *chunk_l) {
   int4 ch = chunks[blockIdx.x];
   const BNode &nd = nodes[active[ch.x]];
-  if (!(nd.flags & NF_SPLIT)) return;  // uniform over the block
+  if (!(nd.flags & NF_SPLIT)) {  // uniform over the block
+    if (threadIdx.x == 0) chunk_l[blockIdx.x] = 0;
+    return;
+  }
   NodeGeom g = node_geom(nd);
   uint32_t isL = 0;
   uint32_t p = (uint32_t)(ch.y + threadIdx.x);
@@ -417,18 +630,15 @@ __global__ __launch_bounds__(256) void k_part_classify(const int4 *chunks, const
   if (threadIdx.x == 0) chunk_l[blockIdx.x] = total;
 }
 
-// Partition step 2: per split node, chunk offsets of the left count and the front length F.
-__global__ void k_part_offsets(const int32_t *active, uint32_t nact, BNode *nodes, const uint32_t *chunk_l,
-                               uint32_t *chunk_off, const uint8_t *cls, uint32_t *counters) {
+// Partition step 2: per split node, the left count from the level-wide scan of chunk lefts
+// (chunk_off[c] = lefts in chunks before c, chunk_off[nchunks] = all) and the front length F.
+__global__ void k_part_offsets(const int32_t *active, uint32_t nact, BNode *nodes, const uint32_t *chunk_off,
+                               const uint8_t *cls, uint32_t *counters) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nact) return;
   BNode &nd = nodes[active[s]];
   if (!(nd.flags & NF_SPLIT)) return;
-  uint32_t acc = 0;
-  for (int c = 0; c < nd.chunk_count; c++) {
-    chunk_off[nd.chunk_begin + c] = acc;
-    acc += chunk_l[nd.chunk_begin + c];
-  }
+  uint32_t acc = chunk_off[nd.chunk_begin + nd.chunk_count] - chunk_off[nd.chunk_begin];
   if ((int)acc != nd.nl) atomicOr(&counters[C_ERROR], 2u);
   nd.F = nd.nl + (cls[nd.first + nd.nl] ? 1 : 0);
 }
@@ -442,7 +652,7 @@ __global__ __launch_bounds__(256) void k_part_fb(const int4 *chunks, const int32
   const BNode &nd = nodes[active[ch.x]];
   if (!(nd.flags & NF_SPLIT) || (int)threadIdx.x >= ch.z) return;
   int p = ch.y + (int)threadIdx.x;
-  int L = (int)(chunk_off[blockIdx.x] + lpre[p]);  // lefts in [first, p)
+  int L = (int)(chunk_off[blockIdx.x] - chunk_off[nd.chunk_begin] + lpre[p]);  // lefts in [first, p)
   int last = nd.first + nd.n - 1;
   if (p < nd.first + nd.F) {
     if (cls[p]) fr[nd.first + ((p - nd.first) - L)] = p - nd.first;
@@ -470,7 +680,7 @@ __global__ __launch_bounds__(256) void k_scatter(const int4 *chunks, const int32
     copy_prim(A, (uint32_t)p, B, (uint32_t)p);
     return;
   }
-  int L = (int)(chunk_off[blockIdx.x] + lpre[p]);
+  int L = (int)(chunk_off[blockIdx.x] - chunk_off[nd.chunk_begin] + lpre[p]);
   int first = nd.first, last = nd.first + nd.n - 1;
   int np;
   if (p < first + nd.F) {
@@ -544,6 +754,10 @@ __global__ void k_root(BNode *nodes, uint32_t n, int32_t *list, uint32_t *counte
   r.axis = 0;
   r.split = -1;
   r.nl = r.F = r.ic = r.rank = r.id = 0;
+  for (int k = 0; k < 6; k++) {
+    r.bkey[k] = k < 3 ? KEY_POS_INF : KEY_NEG_INF;
+    r.ckey[k] = k < 3 ? KEY_POS_INF : KEY_NEG_INF;
+  }
   list[0] = 0;
   for (int k = 0; k < C_COUNT; k++) counters[k] = 0;
   counters[C_NODES] = 1;
@@ -551,14 +765,24 @@ __global__ void k_root(BNode *nodes, uint32_t n, int32_t *list, uint32_t *counte
 
 inline uint32_t nblocks(uint64_t n, uint32_t b = 256) { return (uint32_t)((n + b - 1) / b); }
 
+// Exclusive scan of in[0..n) into out[0..n); the sum goes to *total (device) when given.
+void launch_scan(hipStream_t sm, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *total,
+                 uint32_t *tile_sums) {
+  uint32_t tiles = nblocks(n, 1024);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(1024), 0, sm, in, n, out, tile_sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, sm, tile_sums, tiles, total);
+  if (tiles > 1) hipLaunchKernelGGL(k_scan_add, dim3(nblocks(n)), dim3(256), 0, sm, out, n, tile_sums);
+}
+
 struct Layout {
   Prims P[2];
   BNode *nodes;
   int32_t *active[2];
   int32_t *leaf_list, *bounds_list;
   int4 *chunks;
-  uint32_t *chunk_cnt, *chunk_off_node, *chunk_l, *chunk_off;
+  uint32_t *chunk_cnt, *chunk_off_node, *chunk_l, *chunk_off, *tile_sums;
   uint32_t *bins;
+  Decision *decisions;
   uint8_t *cls;
   uint16_t *lpre;
   int32_t *fr, *bl;
@@ -590,8 +814,10 @@ size_t layout(char *base, uint32_t n, Layout &L) {
   L.chunk_cnt = (uint32_t *)take(sizeof(uint32_t) * L.max_active);
   L.chunk_off_node = (uint32_t *)take(sizeof(uint32_t) * L.max_active);
   L.chunk_l = (uint32_t *)take(sizeof(uint32_t) * L.max_chunks);
-  L.chunk_off = (uint32_t *)take(sizeof(uint32_t) * L.max_chunks);
+  L.chunk_off = (uint32_t *)take(sizeof(uint32_t) * (L.max_chunks + 1));
+  L.tile_sums = (uint32_t *)take(sizeof(uint32_t) * (L.max_chunks / 1024 + 2));
   L.bins = (uint32_t *)take(sizeof(uint32_t) * BIN_WORDS * L.max_active);
+  L.decisions = (Decision *)take(sizeof(Decision) * L.max_active);
   L.cls = (uint8_t *)take(N);
   L.lpre = (uint16_t *)take(sizeof(uint16_t) * N);
   L.fr = (int32_t *)take(sizeof(int32_t) * N);
@@ -635,7 +861,8 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
   for (int b = 0; b < 2; b++) L.P[b].stride = ws.capacity;
   hipLaunchKernelGGL(k_prim_init, dim3(nblocks(n)), dim3(256), 0, sm, tris, n, L.P[0]);
   hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, sm, L.nodes, n, L.bounds_list, L.counters);
-  hipLaunchKernelGGL(k_node_bounds, dim3(1), dim3(256), 0, sm, L.nodes, L.bounds_list, L.P[0]);
+  hipLaunchKernelGGL(k_node_bounds_wide, dim3(std::min<uint32_t>(1024, nblocks(n))), dim3(256), 0, sm, L.nodes, 0,
+                     L.P[0], 0u, n);
   std::vector<uint32_t> level_start = {0, 1};
   uint32_t nact = 0;
   int cur = 0;
@@ -655,10 +882,7 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
     int32_t *active = L.active[act], *next = L.active[act ^ 1];
     // chunks of the level's nodes
     hipLaunchKernelGGL(k_chunk_count, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_cnt);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, sm, L.chunk_cnt, nact, L.chunk_off_node,
-                       L.counters + C_CHUNKS);
-    hipLaunchKernelGGL(k_chunk_emit, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_cnt,
-                       L.chunk_off_node, L.chunks);
+    launch_scan(sm, L.chunk_cnt, nact, L.chunk_off_node, L.counters + C_CHUNKS, L.tile_sums);
     BTRY(hipMemsetAsync(L.counters + C_NEXT, 0, sizeof(uint32_t) * 3, sm));
     BTRY(hipMemcpyAsync(ws.host, L.counters, sizeof(uint32_t) * C_COUNT, hipMemcpyDeviceToHost, sm));
     BTRY(hipStreamSynchronize(sm));
@@ -667,15 +891,21 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
       err = "builder: chunk list overflow";
       return MCR_E_OVERFLOW;
     }
+    hipLaunchKernelGGL(k_chunk_emit, dim3(nblocks(nchunks)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_cnt,
+                       L.chunk_off_node, nchunks, L.chunks);
     uint32_t words = (uint32_t)BIN_WORDS * nact;
     hipLaunchKernelGGL(k_bins_init, dim3(nblocks(words)), dim3(256), 0, sm, L.bins, words);
-    hipLaunchKernelGGL(k_bin, dim3(nchunks), dim3(256), 0, sm, L.chunks, active, L.nodes, L.P[cur], L.bins);
-    hipLaunchKernelGGL(k_decide, dim3(nblocks(nact, 64)), dim3(64), 0, sm, active, nact, L.nodes, L.bins,
+    hipLaunchKernelGGL(k_bin, dim3(nblocks(nchunks, BIN_GROUP)), dim3(256), 0, sm, L.chunks, nchunks, active, L.nodes,
+                       L.P[cur], L.bins);
+    hipLaunchKernelGGL(k_decide, dim3(nblocks(nact, 4)), dim3(256), 0, sm, active, nact, L.nodes, L.bins,
+                       L.counters, L.decisions);
+    hipLaunchKernelGGL(k_create, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.decisions,
                        L.counters, next, L.leaf_list, L.bounds_list);
     hipLaunchKernelGGL(k_part_classify, dim3(nchunks), dim3(256), 0, sm, L.chunks, active, L.nodes, L.P[cur],
                        L.cls, L.lpre, L.chunk_l);
-    hipLaunchKernelGGL(k_part_offsets, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_l,
-                       L.chunk_off, L.cls, L.counters);
+    launch_scan(sm, L.chunk_l, nchunks, L.chunk_off, L.chunk_off + nchunks, L.tile_sums);
+    hipLaunchKernelGGL(k_part_offsets, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_off,
+                       L.cls, L.counters);
     hipLaunchKernelGGL(k_part_fb, dim3(nchunks), dim3(256), 0, sm, L.chunks, active, L.nodes, L.cls, L.lpre,
                        L.chunk_off, L.fr, L.bl);
     hipLaunchKernelGGL(k_scatter, dim3(nchunks), dim3(256), 0, sm, L.chunks, active, L.nodes, L.P[cur],
