@@ -122,6 +122,32 @@ def measure_traffic(args, kernel, launches):
             'fetch_scale': round(res['FETCH_SIZE'][1], 4), 'write_scale': round(res['WRITE_SIZE'][1], 4)}
 
 
+def c0(ctx):
+    """The CPU reference configuration (plain_cpu: 10,242 triangles, 256x256, 16 spp, 4 bounces,
+    GI off, ACES, sRGB) on the GPU through raytrace_compiled, and the whole frame on the host's
+    cores with the oracle (16 passes of centre-first 64x64 tiles, pass-major)."""
+    import oracle
+    from meercadraytracer_amd import abi, scenes
+    from meercadraytracer_amd.raytrace import raytrace_compiled, trace_settings
+    sc, cam, o = scenes.compiled('plain_cpu', 0, build_bvh=ctx.build_bvh)
+    raytrace_compiled(sc, cam, 256, 256, o, ctx=ctx)  # warm
+    cnt = abi.Counters()
+    t = time.perf_counter()
+    raytrace_compiled(sc, cam, 256, 256, o, ctx=ctx, counters=cnt)
+    dt = time.perf_counter() - t
+    gpu_rays = cnt.closest_queries + cnt.any_queries
+    tiles = oracle.make_tiles(256, 256, 64)
+    jobs = [abi.TileJob(abi.Tile(*tl), 256, 256, p, 1, 0) for p in range(16) for tl in tiles]
+    threads = os.cpu_count() or 1
+    secs, c = oracle.render_items_mt(sc, cam, trace_settings(o), jobs, threads)
+    cpu_rays = c['closest_queries'] + c['any_queries']
+    return {'workload': 'plain_cpu (C0): 10,242 triangles, 256x256, 16 spp, 4 bounces, GI off, ACES, sRGB',
+            'gpu': {'seconds': round(dt, 4), 'rays': int(gpu_rays), 'mrays_s': round(gpu_rays / dt / 1e6, 2),
+                    'note': 'raytrace_compiled: scene upload, 16 passes, encode and readback'},
+            'cpu': {'seconds': round(secs, 3), 'rays': int(cpu_rays), 'mrays_s': round(cpu_rays / secs / 1e6, 2),
+                    'cores': threads, 'kind': 'port'}}
+
+
 def served(args, ctx, workloads):
     """The served mix at each arrival rate over the same 32 resident scenes. The top-level
     figures are those of the highest rate (the one meant to keep the device saturated)."""
@@ -170,7 +196,7 @@ def main():
     ap.add_argument('--served-rates', default='2,8', help='request arrival rates (per second) to sweep')
     ap.add_argument('--served-seconds', type=float, default=12.0, help='arrival window per rate')
     ap.add_argument('--served-slots', type=int, default=0, help='server path slots (0 = default)')
-    ap.add_argument('--only', choices=['plain', 'batched', 'served'], default=None,
+    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0'], default=None,
                     help='run one workload and print its JSON object only')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB (for scaling HBM counters)')
@@ -201,6 +227,9 @@ def main():
         return
     if args.only == 'served':
         print(json.dumps(served(args, ctx, workloads)), flush=True)
+        return
+    if args.only == 'c0':
+        print(json.dumps(c0(ctx)), flush=True)
         return
     t0 = time.time()
     scene, camera, opts = scenes.plain_gpu_scene(0, W, H)
