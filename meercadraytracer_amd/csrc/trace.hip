@@ -703,11 +703,25 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
       nl = V.light_slots;
     });
   }
-  // Enqueue this lane's shadow rays (one atomic per wave).
-  uint32_t e = wave_append(&w.counts[C_SHADOW], nshadow);
-  if (live && nshadow) {
+  // Enqueue the wave's shadow rays with one atomic, light by light inside the wave's block, so a
+  // traversal wave mostly holds rays towards one light from neighbouring paths.
+  uint32_t mask = 0;
+  if (live && nshadow)
     for (uint32_t li = 0; li < nl; li++)
-      if (w.status[(size_t)li * P + slot] == LS_RAY) w.sref[e++] = slot * MCR_MAX_LIGHTS + (int32_t)li;
+      if (w.status[(size_t)li * P + slot] == LS_RAY) mask |= 1u << li;
+  uint64_t bits[MCR_MAX_LIGHTS];
+  uint32_t total = 0;
+  for (int li = 0; li < MCR_MAX_LIGHTS; li++) {
+    bits[li] = __ballot((mask >> li) & 1u);
+    total += (uint32_t)__popcll(bits[li]);
+  }
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 0 && total) base = atomicAdd(&w.counts[C_SHADOW], total);
+  base = __shfl(base, 0, 64);
+  uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
+  for (int li = 0; li < MCR_MAX_LIGHTS; li++) {
+    if ((mask >> li) & 1u) w.sref[base + (uint32_t)__popcll(bits[li] & below)] = slot * MCR_MAX_LIGHTS + li;
+    base += (uint32_t)__popcll(bits[li]);
   }
 }
 
