@@ -196,7 +196,8 @@ def main():
     ap.add_argument('--served-rates', default='2,8', help='request arrival rates (per second) to sweep')
     ap.add_argument('--served-seconds', type=float, default=12.0, help='arrival window per rate')
     ap.add_argument('--served-slots', type=int, default=0, help='server path slots (0 = default)')
-    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0'], default=None,
+    ap.add_argument('--no-precision', action='store_true', help='skip the precision-form probe measurement')
+    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0', 'precision'], default=None,
                     help='run one workload and print its JSON object only')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB (for scaling HBM counters)')
@@ -317,6 +318,12 @@ def main():
             roofline['traffic_detail'] = t
     for b in (data, acc, image):
         b.free()
+    if not args.no_precision or args.only == 'precision':
+        log('precision form (1M probe rays, FP64 binary vs FP32 wide) ...')
+        result['precision'] = workloads.precision_form(ctx, gpu, sc.geometry, cam, W, H, log=log)
+        if args.only == 'precision':
+            print(json.dumps(result['precision']), flush=True)
+            return
     gpu.release()
     if not args.no_batched:
         log('batched form ...')

@@ -27,6 +27,7 @@ extern "C" {
 #define MCR_E_OVERFLOW (-5)    /* a fixed device capacity was exceeded */
 #define MCR_E_CANCELLED (-6)   /* the job's generation was cancelled before it ran */
 #define MCR_E_TIMEOUT (-7)     /* a wait ended before the request finished */
+#define MCR_E_INTERNAL (-8)    /* an internal consistency check failed (a bug, not bad input) */
 
 /* ---- enums restated from src/constants.ts and src/types.ts ------------------------------ */
 enum { MCR_FRONT_SIDE = 0, MCR_BACK_SIDE = 1, MCR_DOUBLE_SIDE = 2 };           /* constants.ts:5-7 */
@@ -306,6 +307,29 @@ int mcr_intersect_closest(mcr_ctx *ctx, mcr_scene *scene, const double *rays, ui
 int mcr_intersect_any_count(mcr_ctx *ctx, mcr_scene *scene, const double *rays, uint32_t n,
                             double t_min, double t_max, uint32_t *count_out, int device_ptrs,
                             mcr_counters *counters);
+
+/* ---- precision form: compressed 8-wide BVH, FP32 watertight closest hit ------------------- */
+/* A second layout of a scene's geometry, built on the device from its binary BVH: 80-byte nodes
+ * of eight children whose boxes are 8-bit grid coordinates over the node, rounded outward, and
+ * FP32 triangles (v0, v0 + e1, v0 + e2 rounded once). Queries test boxes in FP32 with slabs
+ * widened by their rounding bound (never missing a box the ray meets) and triangles with the
+ * watertight test of Woop, Benthin and Wald (2013). It stands beside intersectClosest
+ * (src/bvh.ts:202-282): the same arguments and hit record in FP32, with results that can differ
+ * from the FP64 query where FP32 rounding decides a hit. Rebuilt after each geometry change
+ * (mcr_scene_set_geometry drops it). */
+typedef struct mcr_wide_info {
+  uint32_t node_count, triangle_count, depth, pad0;
+  uint64_t bytes;   /* nodes and triangles */
+  double build_ms;  /* device time of the build */
+} mcr_wide_info;
+int mcr_scene_build_wide(mcr_scene *scene, mcr_wide_info *info_out);
+int mcr_scene_wide_info(const mcr_scene *scene, mcr_wide_info *out);
+/* rays: n*6 floats (origin, direction). Outputs as mcr_intersect_closest, in FP32; triangle ids
+ * are BVH-order indices, as there. */
+int mcr_intersect_closest_wide(mcr_ctx *ctx, mcr_scene *scene, const float *rays, uint32_t n,
+                               float t_min, float t_max, int cull, int32_t *tri_out, float *t_out,
+                               float *u_out, float *v_out, uint8_t *front_out, int device_ptrs,
+                               mcr_counters *counters);
 
 /* ---- device buffers, for hosts without a GPU allocator of their own ----------------------- */
 int mcr_device_alloc(mcr_ctx *ctx, uint64_t bytes, void **out);

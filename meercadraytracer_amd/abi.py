@@ -134,6 +134,14 @@ class Timing(C.Structure):
     ]
 
 
+class WideInfo(C.Structure):
+    _fields_ = [('node_count', C.c_uint32), ('triangle_count', C.c_uint32), ('depth', C.c_uint32),
+                ('pad0', C.c_uint32), ('bytes', C.c_uint64), ('build_ms', C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != 'pad0'}
+
+
 class ServerConfig(C.Structure):
     _fields_ = [('hbm_budget', C.c_uint64), ('slots', C.c_uint32), ('window', C.c_uint32),
                 ('max_requests', C.c_uint32), ('use_graph', C.c_int32)]
@@ -146,7 +154,7 @@ class Request(C.Structure):
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
-MCR_E_CANCELLED, MCR_E_TIMEOUT = -6, -7
+MCR_E_CANCELLED, MCR_E_TIMEOUT, MCR_E_INTERNAL = -6, -7, -8
 
 
 class RequestStatus(C.Structure):

@@ -33,6 +33,7 @@ namespace {
 int fail(int code, const std::string &msg) { return mcr_fail(code, msg); }
 
 void free_geometry(mcr_scene *s) {
+  wide_free(s);
   hipFree(s->tri);
   hipFree(s->inner);
   hipFree(s->longleaf);

@@ -13,6 +13,7 @@
 #include "builder.h"
 #include "device.h"
 #include "wavefront.h"
+#include "wide.h"
 
 // Records msg as this thread's last error (mcr_last_error) and returns code.
 int mcr_fail(int code, const std::string &msg);
@@ -128,7 +129,16 @@ struct mcr_scene {
   DevMaterial *materials = nullptr;
   DevLight *lights = nullptr;
   uint64_t bytes_geometry = 0, bytes_shading = 0;
+  // precision form (wide.hip)
+  bool has_wide = false;
+  uint4 *wide_nodes = nullptr;
+  float4 *wide_tri = nullptr;
+  WideScene wide = {};
+  mcr_wide_info wide_info = {};
 };
+
+// Frees the scene's wide BVH (wide.hip).
+void wide_free(mcr_scene *s);
 
 // The view of (scene, camera, settings) as the kernels read it.
 ViewDev make_view(const mcr_scene *s, const mcr_camera &cam, const mcr_settings &st);

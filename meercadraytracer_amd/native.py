@@ -69,6 +69,10 @@ def lib():
             'mcr_encode_tile64': (C.c_int, [P, P, C.c_uint32, abi.Tile, P, P, C.c_uint32, C.c_int]),
             'mcr_intersect_closest': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P, P,
                                                 P, C.c_int, P]),
+            'mcr_scene_build_wide': (C.c_int, [P, C.POINTER(abi.WideInfo)]),
+            'mcr_scene_wide_info': (C.c_int, [P, C.POINTER(abi.WideInfo)]),
+            'mcr_intersect_closest_wide': (C.c_int, [P, P, P, C.c_uint32, C.c_float, C.c_float, C.c_int, P, P, P,
+                                                     P, P, C.c_int, P]),
             'mcr_intersect_any_count': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, P, C.c_int, P]),
             'mcr_device_alloc': (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
             'mcr_device_free': (C.c_int, [P, P]),
@@ -252,6 +256,33 @@ class Scene:
         cnt = counters if counters is not None else abi.Counters()
         check(lib().mcr_intersect_closest(self.ctx.handle, self.handle, _p(rays), n, t_min, t_max, int(cull),
                                           _p(tri), _p(t), _p(u), _p(v), _p(front), 0, C.byref(cnt)))
+        return tri, t, u, v, front
+
+    def build_wide(self):
+        """Builds the precision form's 8-wide BVH (mcr_scene_build_wide); returns its info."""
+        info = abi.WideInfo()
+        check(lib().mcr_scene_build_wide(self.handle, C.byref(info)))
+        return info.as_dict()
+
+    def intersect_closest_wide(self, rays, t_min=0.0, t_max=np.inf, cull=True, counters=None, device=None):
+        """FP32 closest hits over the wide BVH. rays: (n, 6) float32 (host), or device=(ptr, n)
+        with output device pointers in device[2:] (tri, t, u, v, front; None to skip)."""
+        cnt = counters if counters is not None else abi.Counters()
+        if device is not None:
+            ptr, n, *outs = device
+            outs = (list(outs) + [None] * 5)[:5]
+            check(lib().mcr_intersect_closest_wide(self.ctx.handle, self.handle, ptr, n, t_min, t_max, int(cull),
+                                                   *outs, 1, C.byref(cnt)))
+            return None
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = len(rays)
+        tri = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        u = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        front = np.empty(n, np.uint8)
+        check(lib().mcr_intersect_closest_wide(self.ctx.handle, self.handle, _p(rays), n, t_min, t_max, int(cull),
+                                               _p(tri), _p(t), _p(u), _p(v), _p(front), 0, C.byref(cnt)))
         return tri, t, u, v, front
 
     def intersect_any_count(self, rays, t_min=0.0, t_max=np.inf, counters=None):

@@ -108,6 +108,62 @@ def probe_rays(geometry, camera, width, height, n, seed, closest):
     return np.vstack([prim, np.vstack(sec)[:n - half]])
 
 
+# ---- precision form --------------------------------------------------------------------------------
+# Record sizes the two query kernels read per visit (for algorithmic bytes): the FP64 path reads a
+# 64-byte binary inner record per node and a 48-byte triangle record; the wide path an 80-byte
+# node and a 48-byte triangle.
+FP64_NODE_BYTES, WIDE_NODE_BYTES, TRI_BYTES = 64, 80, 48
+
+
+def precision_form(ctx, gpu, geometry, camera, width=1920, height=1080, n=1_000_000, seed=1234, repeats=5,
+                   log=None):
+    """Closest-hit throughput of the FP64 binary query and the FP32 wide query on the same resident
+    probe rays (inputs and outputs in HBM), with node visits, triangle tests and the wide BVH's
+    build time. Returns a dict."""
+    import ctypes as C
+    from . import abi, native
+    rays = probe_rays(geometry, camera, width, height, n, seed, lambda r: gpu.intersect_closest(r, cull=True))
+    info = gpu.build_wide()
+    if log:
+        log(f'wide BVH: {info}')
+    r32 = native.DeviceBuffer(ctx, n * 24)
+    r64 = native.DeviceBuffer(ctx, n * 48)
+    tri = native.DeviceBuffer(ctx, n * 4)
+    tbuf = native.DeviceBuffer(ctx, n * 8)
+    r32.upload(rays)
+    r64.upload(rays.astype(np.float64))
+    L = native.lib()
+    ctx.set_timing(True)
+    out = {'rays': n, 'wide_bvh': info}
+
+    def timed(call, node_bytes):
+        call(abi.Counters())  # warm
+        cnt = abi.Counters()
+        ms = 0.0
+        for _ in range(repeats):
+            call(cnt)
+            ms += ctx.last_timing()['closest_ms']
+        c = cnt.as_dict()
+        q = max(1, c['closest_queries'])
+        algo = c['node_visits'] * node_bytes + c['triangle_tests'] * TRI_BYTES
+        return {'mrays_per_s': round(q / (ms / 1e3) / 1e6, 1), 'ms_per_launch': round(ms / repeats, 3),
+                'node_visits_per_ray': round(c['node_visits'] / q, 2),
+                'triangle_tests_per_ray': round(c['triangle_tests'] / q, 2),
+                'algorithmic_gb_per_s': round(algo / (ms / 1e3) / 1e9, 1)}
+
+    out['fp64_binary'] = timed(lambda cnt: native.check(L.mcr_intersect_closest(
+        ctx.handle, gpu.handle, r64.ptr, n, 0.0, float('inf'), 1, tri.ptr, tbuf.ptr, None, None, None, 1,
+        C.byref(cnt))), FP64_NODE_BYTES)
+    out['fp32_wide'] = timed(lambda cnt: native.check(L.mcr_intersect_closest_wide(
+        ctx.handle, gpu.handle, r32.ptr, n, 0.0, float('inf'), 1, tri.ptr, tbuf.ptr, None, None, None, 1,
+        C.byref(cnt))), WIDE_NODE_BYTES)
+    out['wide_speedup'] = round(out['fp32_wide']['mrays_per_s'] / out['fp64_binary']['mrays_per_s'], 2)
+    ctx.set_timing(False)
+    for b in (r32, r64, tri, tbuf):
+        b.free()
+    return out
+
+
 # ---- batched form --------------------------------------------------------------------------------
 def batched_jobs(rng, scenes, n_jobs=256, width=1920, height=1080):
     """(views, jobs, job_views, floats, samples): every pass of every tile job."""

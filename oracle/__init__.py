@@ -38,6 +38,8 @@ def load(libm=False):
         'orc_intersect_closest': (None, [P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P, P, P, P]),
         'orc_intersect_any_count': (None, [P, P, C.c_uint32, C.c_double, C.c_double, P, P]),
         'orc_brute_force': (None, [P, C.c_uint32, P, C.c_uint32, P, P, P]),
+        'orc_wide_brute': (None, [P, P, C.c_uint32, P, C.c_uint32, C.c_float, C.c_float, C.c_int, P, P, P, P, P,
+                                  C.c_uint32]),
         'orc_hash': (C.c_uint32, [C.c_uint32]),
         'orc_seed': (C.c_uint32, [C.c_int32] * 4),
         'orc_random_stream': (None, [C.c_uint32, C.c_uint32, P]),
@@ -122,6 +124,24 @@ def brute_force(triangles, count, rays):
     all_ = np.empty(n, np.uint32)
     lib.orc_brute_force(_p(tri_in), count, _p(rays), n, _p(tri), _p(t), _p(all_))
     return tri, t, all_
+
+
+def wide_brute(geometry, rays, t_min=0.0, t_max=np.inf, cull=True, threads=None):
+    """Precision-form checker: FP32 watertight closest hit over every triangle (orc_wide_brute).
+    rays: (n, 6) float32. Returns tri (BVH order), t, u, v (float32), front."""
+    lib = load()
+    tris = np.ascontiguousarray(geometry.triangles, dtype=np.float32)
+    sides = np.ascontiguousarray(geometry.sides, dtype=np.uint8)
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+    n = len(rays)
+    tri = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    u = np.empty(n, np.float32)
+    v = np.empty(n, np.float32)
+    front = np.empty(n, np.uint8)
+    lib.orc_wide_brute(_p(tris), _p(sides), len(sides), _p(rays), n, t_min, t_max, int(cull), _p(tri), _p(t),
+                       _p(u), _p(v), _p(front), threads or min(16, os.cpu_count() or 1))
+    return tri, t, u, v, front
 
 
 def render_tile(scene, camera, settings, tile, width, height, pass_, block=1, libm=False):

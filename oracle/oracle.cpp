@@ -1249,6 +1249,95 @@ void orc_brute_force(const float *data, uint32_t count, const double *rays, uint
   }
 }
 
+// ---- precision-form checker -------------------------------------------------------------------
+// Not a restatement of the reference: the closest hit among ALL triangles (brute force) under the
+// FP32 watertight ray/triangle test of Woop, Benthin and Wald (JCGT 2(1), 2013), on the FP32
+// vertices the precision form stores: v0, and the FP64 sums v0 + e1, v0 + e2 (as bvh.ts:26-27
+// forms them) rounded once. The wide-BVH query never skips a box the ray meets, so it must return
+// this hit exactly (up to ties in t). Order of operations: vertices minus origin; shear with the
+// dominant axis kz (first largest |d| in x, y, z order), kx = kz + 1, ky = kx + 1 (mod 3), kx and
+// ky swapped when d[kz] < 0; edge functions U = cx*by - cy*bx, V = ax*cy - ay*cx,
+// W = bx*ay - by*ax (products and differences rounded separately, redone in FP64 when one is 0);
+// det = U + V + W; T = U*az + V*bz + W*cz with z = sz * (vertex[kz] - o[kz]); t, u, v = T, V, W
+// times (1 / det).
+static float sel3(const float *p, int k) { return p[k]; }
+
+static bool wt_closest_one(const float *tris, const uint8_t *sides, uint32_t count, const float *ray,
+                           float tmin, float tmax, int cull, int32_t &tri, float &tb, float &ub,
+                           float &vb, uint8_t &fb) {
+  const float *o = ray, *d = ray + 3;
+  tri = -1;
+  tb = tmax;
+  ub = vb = 0;
+  fb = 1;
+  float ad[3] = {fabsf(d[0]), fabsf(d[1]), fabsf(d[2])};
+  int kz = ad[0] >= ad[1] ? (ad[0] >= ad[2] ? 0 : 2) : (ad[1] >= ad[2] ? 1 : 2);
+  int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+  if (d[kz] < 0) std::swap(kx, ky);
+  if (d[kz] == 0) return false;
+  float sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz = 1.0f / d[kz];
+  for (uint32_t k = 0; k < count; k++) {
+    const float *b = tris + (size_t)k * 9;
+    float v[3][3] = {{b[0], b[1], b[2]},
+                     {(float)((double)b[0] + (double)b[3]), (float)((double)b[1] + (double)b[4]),
+                      (float)((double)b[2] + (double)b[5])},
+                     {(float)((double)b[0] + (double)b[6]), (float)((double)b[1] + (double)b[7]),
+                      (float)((double)b[2] + (double)b[8])}};
+    float px[3], py[3], pz[3];
+    for (int j = 0; j < 3; j++) {
+      float rel[3] = {v[j][0] - o[0], v[j][1] - o[1], v[j][2] - o[2]};
+      float z = sel3(rel, kz);
+      px[j] = sel3(rel, kx) - sx * z;
+      py[j] = sel3(rel, ky) - sy * z;
+      pz[j] = z;
+    }
+    float U = px[2] * py[1] - py[2] * px[1];
+    float V = px[0] * py[2] - py[0] * px[2];
+    float W = px[1] * py[0] - py[1] * px[0];
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+      U = (float)((double)px[2] * py[1] - (double)py[2] * px[1]);
+      V = (float)((double)px[0] * py[2] - (double)py[0] * px[2]);
+      W = (float)((double)px[1] * py[0] - (double)py[1] * px[0]);
+    }
+    bool neg = U < 0 || V < 0 || W < 0, pos = U > 0 || V > 0 || W > 0;
+    int side = cull ? sides[k] : 2;
+    if (side == 0 ? neg : (side == 1 ? pos : (neg && pos))) continue;
+    float det = U + V + W;
+    if (det == 0.0f) continue;
+    float T = U * (sz * pz[0]) + V * (sz * pz[1]) + W * (sz * pz[2]);
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (t > tmin && t < tb) {
+      tb = t;
+      ub = V * rcp;
+      vb = W * rcp;
+      tri = (int32_t)k;
+      fb = det > 0 ? 1 : 0;
+    }
+  }
+  return tri >= 0;
+}
+
+void orc_wide_brute(const float *tris, const uint8_t *sides, uint32_t count, const float *rays, uint32_t n,
+                    float tmin, float tmax, int cull, int32_t *tri, float *t, float *u, float *v,
+                    uint8_t *front, uint32_t threads) {
+  threads = std::max<uint32_t>(1, std::min<uint32_t>(threads, 64));
+  std::atomic<uint32_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      uint32_t i0 = next.fetch_add(256);
+      if (i0 >= n) return;
+      for (uint32_t i = i0; i < std::min(n, i0 + 256); i++)
+        wt_closest_one(tris, sides, count, rays + (size_t)i * 6, tmin, tmax, cull, tri[i], t[i], u[i], v[i],
+                       front[i]);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (uint32_t k = 1; k < threads; k++) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+}
+
 uint32_t orc_hash(uint32_t x) { return hash(x); }
 uint32_t orc_seed(int32_t a, int32_t b, int32_t c, int32_t d) {
   Random r;

@@ -37,6 +37,13 @@ void orc_intersect_any_count(const mcr_geometry *g, const double *rays, uint32_t
 void orc_brute_force(const float *triangles, uint32_t count, const double *rays, uint32_t n,
                      int32_t *tri, double *t, uint32_t *all);
 
+/* Precision-form checker (not a reference restatement): closest hit over every triangle under
+ * the FP32 watertight test of Woop, Benthin and Wald (2013) on the precision form's FP32 vertices.
+ * rays: n*6 floats. threads: worker threads to use. */
+void orc_wide_brute(const float *triangles, const uint8_t *sides, uint32_t count, const float *rays,
+                    uint32_t n, float t_min, float t_max, int cull, int32_t *tri, float *t, float *u,
+                    float *v, uint8_t *front, uint32_t threads);
+
 /* random.ts */
 uint32_t orc_hash(uint32_t x);
 uint32_t orc_seed(int32_t a, int32_t b, int32_t c, int32_t d);
