@@ -90,6 +90,8 @@ int engine_zero_done(Engine &e, uint32_t first, uint32_t n);
 int engine_reset(Engine &e);
 // Starts `count` samples from `next` in free slots; see k_refill.
 int engine_refill(Engine &e, int q, uint32_t count, uint32_t free_top, uint32_t next, uint32_t queued);
+// Puts the live slots of queue q (at most `live`) in ascending slot order; between iterations only.
+int engine_sort_queue(Engine &e, int q, uint32_t live);
 // One bounce iteration over the `live` paths of queue q (grids sized from live).
 int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, double *t_closest,
                      double *t_shade, double *t_any);

@@ -164,6 +164,12 @@ int engine_refill(Engine &e, int q, uint32_t count, uint32_t free_top, uint32_t 
   return MCR_OK;
 }
 
+int engine_sort_queue(Engine &e, int q, uint32_t live) {
+  launch_sort_queue(e.stream, e.w, q, live);
+  HIP_TRY(hipGetLastError());
+  return MCR_OK;
+}
+
 int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, double *t_closest,
                      double *t_shade, double *t_any) {
   hipStream_t sm = e.stream;

@@ -13,6 +13,8 @@ void launch_reset_slots(hipStream_t s, const Wavefront &w);
 // mix: the run holds samples of more than one view (see with_view in trace.hip).
 void launch_refill(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t count, uint32_t free_top, uint32_t next, uint32_t queued);
+// Rewrites queue q (holding `live` slots, at most) in ascending slot order.
+void launch_sort_queue(hipStream_t s, const Wavefront &w, int q, uint32_t live);
 void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
 void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,

@@ -513,6 +513,12 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
   uint32_t k = std::min<uint32_t>(free_slots, s->total - next);
   MCR_TRY(engine_refill(e, s->q, k, free_slots, next, live));
   live += k;
+  // Refilled slots come from the free list in the order they went idle (after a drain, every
+  // slot does), and in-place regeneration would keep them scattered for the rest of the run;
+  // re-sorting the queue after a refill puts every wave back on neighbouring slots (coalesced
+  // path-state accesses). Steps without a refill keep the queue's own order: waves requeue in
+  // the order they finish, which groups paths of similar cost.
+  if (k) MCR_TRY(engine_sort_queue(e, s->q, live));
   uint32_t max_lights = 0;
   for (const ViewDev &v : s->views) max_lights = std::max(max_lights, v.light_slots);
   lk.unlock();
@@ -614,7 +620,7 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   memset(s->views.data(), 0, s->views.size() * sizeof(ViewDev));
   for (int i = (int)s->cfg.max_requests - 1; i >= 0; i--) s->view_free.push_back(i);
   // Tile results wait here until encoded: the window plus tiles waiting for an earlier pass.
-  s->arena_floats = std::max<uint64_t>((uint64_t)s->cfg.window * 4 * 4, 1ull << 24);
+  s->arena_floats = std::max<uint64_t>((uint64_t)s->cfg.window * 4 * 16, 1ull << 26);
   HIP_TRY(hipMalloc(&s->arena_dev, s->arena_floats * sizeof(float)));
   s->arena.reset(s->arena_floats);
   HIP_TRY(hipHostMalloc(&s->counters_host, sizeof(DevCounters), hipHostMallocDefault));

@@ -956,6 +956,28 @@ __global__ __launch_bounds__(256) void k_refill(const ViewDev *__restrict__ view
   start_sample_any<MIX>(views, w, *w.ctl, (uint32_t)slot, next + i);
 }
 
+// ---- queue re-sort: live slots of queue q in ascending slot order ---------------------------------
+// Slots that went idle and were refilled come back in free-list order, scattered over the slot
+// range, and in-place regeneration keeps them where they are; a wave over such a queue touches
+// many more lines of the per-slot arrays than one over neighbouring slots. Between a resolve and
+// the next shade the per-slot mode byte is dead, so it marks the live slots here.
+#define MODE_SORT_MARK 0xffu
+__global__ __launch_bounds__(256) void k_sort_mark(Wavefront w, int q) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) w.counts[C_SORT] = 0;
+  if (i < w.counts[q]) w.mode[w.queue[q][i]] = (uint8_t)MODE_SORT_MARK;
+}
+
+__global__ __launch_bounds__(256) void k_sort_gather(Wavefront w, int q) {
+  uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = slot < w.capacity && w.mode[slot] == MODE_SORT_MARK;
+  uint32_t pos = wave_append(&w.counts[C_SORT], live ? 1u : 0u);
+  if (live) {
+    w.queue[q][pos] = (int32_t)slot;
+    w.mode[slot] = (uint8_t)MODE_DONE;
+  }
+}
+
 // Free list = every slot, in descending order, so a refill of k slots takes slots 0 .. k-1.
 __global__ __launch_bounds__(256) void k_reset_slots(Wavefront w) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1117,6 +1139,10 @@ void launch_reset_slots(hipStream_t s, const Wavefront &w) {
 void launch_refill(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t count, uint32_t free_top, uint32_t next, uint32_t queued) {
   MCR_LAUNCH_MIX(k_refill, blocks_for(count ? count : 1), s, views, w, q, count, free_top, next, queued);
+}
+void launch_sort_queue(hipStream_t s, const Wavefront &w, int q, uint32_t live) {
+  hipLaunchKernelGGL(k_sort_mark, dim3(blocks_for(live ? live : 1)), dim3(256), 0, s, w, q);
+  hipLaunchKernelGGL(k_sort_gather, dim3(blocks_for(w.capacity)), dim3(256), 0, s, w, q);
 }
 void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {

@@ -57,7 +57,8 @@ enum CountWord {
   C_NEXT = 5,       // next sample to start (may run past Control::total; the host clamps it)
   C_EXTEND_FETCH = 6,
   C_SHADOW_FETCH = 7,
-  C_WORDS = 8,
+  C_SORT = 8,       // appends of the queue re-sort (k_sort_gather)
+  C_WORDS = 9,
 };
 
 struct JobDev;
