@@ -320,7 +320,7 @@ def main():
         b.free()
     if not args.no_precision or args.only == 'precision':
         log('precision form (1M probe rays, FP64 binary vs FP32 wide) ...')
-        result['precision'] = workloads.precision_form(ctx, gpu, sc.geometry, cam, W, H, log=log)
+        result['precision'] = workloads.precision_form(ctx, gpu, sc.geometry, cam, W, H, settings=st, log=log)
         if args.only == 'precision':
             print(json.dumps(result['precision']), flush=True)
             return

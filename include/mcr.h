@@ -326,6 +326,14 @@ int mcr_scene_build_wide(mcr_scene *scene, mcr_wide_info *info_out);
 int mcr_scene_wide_info(const mcr_scene *scene, mcr_wide_info *out);
 /* rays: n*6 floats (origin, direction). Outputs as mcr_intersect_closest, in FP32; triangle ids
  * are BVH-order indices, as there. */
+#define MCR_ISECT_FP64 0       /* the reference's FP64 arithmetic on its binary BVH (default) */
+#define MCR_ISECT_WIDE_FP32 1  /* the precision form: FP32 on the 8-wide BVH */
+/* How renders on this context (mcr_render_tile/tiles/batch) intersect rays: closest hits
+ * (tracer.ts:113, bvh.ts:202) and shadow rays (tracer.ts:529-542, bvh.ts:289). With
+ * MCR_ISECT_WIDE_FP32 every scene rendered needs mcr_scene_build_wide; shading, accumulation and
+ * sampling stay as in MCR_ISECT_FP64, so images differ only where FP32 intersection does. No
+ * counterpart in the reference, which has one arithmetic. */
+int mcr_set_intersection(mcr_ctx *ctx, int mode);
 int mcr_intersect_closest_wide(mcr_ctx *ctx, mcr_scene *scene, const float *rays, uint32_t n,
                                float t_min, float t_max, int cull, int32_t *tri_out, float *t_out,
                                float *u_out, float *v_out, uint8_t *front_out, int device_ptrs,

@@ -406,6 +406,7 @@ ViewDev make_view(const mcr_scene *s, const mcr_camera &cam, const mcr_settings 
   ViewDev v;
   memset(&v, 0, sizeof(v));
   v.sc = s->d;
+  v.wide = s->wide;
   v.cam = cam;
   v.st = st;
   v.light_slots = std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
@@ -437,6 +438,11 @@ int render_views(mcr_ctx *c, const mcr_view *views, uint32_t nviews, const mcr_t
                  const uint32_t *job_views, uint32_t njobs, float *dout, mcr_counters *counters) {
   std::vector<ViewDev> vd(nviews);
   for (uint32_t v = 0; v < nviews; v++) vd[v] = make_view(views[v].scene, *views[v].camera, *views[v].settings);
+  c->eng.wide = c->intersection == MCR_ISECT_WIDE_FP32;
+  if (c->eng.wide)
+    for (uint32_t v = 0; v < nviews; v++)
+      if (!views[v].scene->has_wide)
+        return fail(MCR_E_NO_SCENE, "the scene has no wide BVH (mcr_scene_build_wide)");
   // MCR_RUN_SAMPLES lowers the run size (tests use it to exercise the split on small calls).
   const char *env = getenv("MCR_RUN_SAMPLES");
   const uint64_t RUN_LIMIT = env ? std::max<uint64_t>(1, strtoull(env, nullptr, 10)) : 1ull << 30;
@@ -491,6 +497,14 @@ int render_common(mcr_ctx *c, const mcr_view *views, uint32_t nviews, const mcr_
 }  // namespace
 
 extern "C" {
+
+int mcr_set_intersection(mcr_ctx *c, int mode) {
+  if (!c) return fail(MCR_E_INVALID, "null argument");
+  if (mode != MCR_ISECT_FP64 && mode != MCR_ISECT_WIDE_FP32) return fail(MCR_E_INVALID, "unknown intersection mode");
+  std::lock_guard<std::mutex> lock(c->mu);
+  c->intersection = mode;
+  return MCR_OK;
+}
 
 int mcr_render_tiles(mcr_ctx *c, mcr_scene *s, const mcr_camera *camera, const mcr_settings *st,
                      const mcr_tile_job *jobs, uint32_t njobs, float *out, int out_device,

@@ -65,11 +65,12 @@ struct Engine {
   Control *ctl_host = nullptr;     // pinned staging
   uint32_t *counts_host = nullptr; // pinned
   bool mix = false;                // the run holds more than one view
+  bool wide = false;               // the run traces with the precision form's FP32 wide BVH
   // Two captured bounce iterations (queue 0 then queue 1), replayed while paths are live; one
-  // graph per kernel variant (single view, mixed views).
+  // graph per kernel variant (single or mixed views, FP64 binary or FP32 wide traversal).
   bool use_graph = true;
-  hipGraphExec_t graph[2] = {};
-  const void *graph_key[2][4] = {};
+  hipGraphExec_t graph[4] = {};
+  const void *graph_key[4][4] = {};
   // timing
   bool timing = false;
   mcr_timing last_timing = {};
@@ -115,6 +116,7 @@ struct mcr_ctx {
   float *out_tmp = nullptr;
   size_t out_tmp_cap = 0;
   BuilderWorkspace builder;
+  int intersection = MCR_ISECT_FP64;  // mcr_set_intersection
   std::mutex mu;
 };
 

@@ -74,7 +74,9 @@ int grow(T *&p, uint32_t &cap, uint32_t n) {
 
 void engine_drop_graphs(Engine &e) {
   // A launch of the graph may still be running.
-  if (e.stream && (e.graph[0] || e.graph[1])) hipStreamSynchronize(e.stream);
+  bool any = false;
+  for (auto &g : e.graph) any |= g != nullptr;
+  if (e.stream && any) hipStreamSynchronize(e.stream);
   for (auto &g : e.graph) {
     if (g) hipGraphExecDestroy(g);
     g = nullptr;
@@ -176,14 +178,14 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
   const Wavefront &w = e.w;
   bool timed = e.timing && t_closest;
   if (timed) HIP_TRY(hipEventRecord(e.ev[0], sm));
-  launch_extend(sm, e.mix, e.views, w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(live)), e.spill,
+  launch_extend(sm, e.mix, e.wide, e.views, w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(live)), e.spill,
                 TRAVERSE_THREADS, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[1], sm));
   launch_shade(sm, e.mix, e.views, w, q, live, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[2], sm));
   uint64_t max_shadow = (uint64_t)live * max_lights;
   if (max_shadow)
-    launch_shadow(sm, e.mix, e.views, w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(max_shadow)), e.spill,
+    launch_shadow(sm, e.mix, e.wide, e.views, w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(max_shadow)), e.spill,
                   TRAVERSE_THREADS, e.seen, e.seen_meta, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[3], sm));
   launch_resolve(sm, e.mix, e.views, w, q, live);
@@ -203,7 +205,7 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
 }
 
 int engine_iteration_pair(Engine &e, uint32_t max_lights) {
-  int v = e.mix ? 1 : 0;
+  int v = (e.mix ? 1 : 0) + (e.wide ? 2 : 0);
   const void *key[4] = {e.w_block, e.views, e.counters,
                         (const void *)(uintptr_t)((uint64_t)e.capacity * 64u + max_lights)};
   hipGraphExec_t &graph = e.graph[v];
@@ -222,11 +224,11 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
     if (err == hipSuccess) {
       uint32_t cap = e.capacity;
       for (int q = 0; q < 2; q++) {
-        launch_extend(cs, e.mix, e.views, e.w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(cap)), e.spill,
+        launch_extend(cs, e.mix, e.wide, e.views, e.w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(cap)), e.spill,
                       TRAVERSE_THREADS, e.counters);
         launch_shade(cs, e.mix, e.views, e.w, q, cap, e.counters);
         if (max_lights)
-          launch_shadow(cs, e.mix, e.views, e.w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of((uint64_t)cap * max_lights)),
+          launch_shadow(cs, e.mix, e.wide, e.views, e.w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of((uint64_t)cap * max_lights)),
                         e.spill, TRAVERSE_THREADS, e.seen, e.seen_meta, e.counters);
         launch_resolve(cs, e.mix, e.views, e.w, q, cap);
       }
@@ -252,7 +254,10 @@ int engine_fetch_counts(Engine &e) {
 }
 
 int engine_check(Engine &e) {
-  if (e.counts_host[C_OVERFLOW])
+  uint32_t o = e.counts_host[C_OVERFLOW];
+  if (o & OVERFLOW_WIDE)
+    return mcr_fail(MCR_E_INTERNAL, "a wide-BVH traversal met a corrupt node or overflowed its stack");
+  if (o)
     return mcr_fail(MCR_E_OVERFLOW,
                     "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
   return MCR_OK;
@@ -261,10 +266,7 @@ int engine_check(Engine &e) {
 int engine_sync(Engine &e) {
   MCR_TRY(engine_fetch_counts(e));
   HIP_TRY(hipStreamSynchronize(e.stream));
-  if (e.counts_host[C_OVERFLOW])
-    return mcr_fail(MCR_E_OVERFLOW,
-                    "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
-  return MCR_OK;
+  return engine_check(e);
 }
 
 void engine_add_counters(const DevCounters &hc, uint64_t paths, mcr_counters *counters) {

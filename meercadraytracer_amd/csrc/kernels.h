@@ -15,11 +15,12 @@ void launch_refill(hipStream_t s, bool mix, const ViewDev *views, const Wavefron
                    uint32_t count, uint32_t free_top, uint32_t next, uint32_t queued);
 // Rewrites queue q (holding `live` slots, at most) in ascending slot order.
 void launch_sort_queue(hipStream_t s, const Wavefront &w, int q, uint32_t live);
-void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+// wide: the run traces with the precision form's FP32 wide BVH (every view's scene has one).
+void launch_extend(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
 void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                   uint32_t max_live, DevCounters *cnt);
-void launch_shadow(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, uint32_t grid,
+void launch_shadow(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
                    DevCounters *cnt);
 void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,

@@ -157,21 +157,15 @@ __device__ inline uint32_t wave_fetch(uint32_t *counter) {
 }
 
 // ---- extend: closest hit for every live slot (bvh.ts:202-282) ------------------------------------
-template <bool MIX>
-__global__ __launch_bounds__(256) void k_extend(const ViewDev *__restrict__ views, Wavefront w, int q,
-                                                int32_t *spill, uint32_t spill_stride,
-                                                DevCounters *cnt) {
-  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+// WIDE: the precision form's FP32 query over the scene's 8-wide BVH (wide.h) instead of the FP64
+// binary traversal; the ray is rounded to FP32 and the hit widened back to FP64 for shading.
+template <bool MIX, bool WIDE, class Stack>
+MCR_INLINE void extend_loop(const ViewDev *__restrict__ views, const Wavefront &w, int q, Stack &st,
+                            DevCounters *cnt) {
   uint32_t n = w.counts[q];
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid == 0) w.counts[C_SHADOW] = 0;  // shade of this iteration appends to it
-  LaneStack st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = 256;
-  st.spill = spill + gid;
-  st.spill_stride = (int)spill_stride;
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
+  bool bad = false;
   size_t P = w.capacity;
   const double *f = w.f;
   // Waves take 64 rays at a time from a shared counter, so no wave idles while others still
@@ -185,7 +179,23 @@ __global__ __launch_bounds__(256) void k_extend(const ViewDev *__restrict__ view
     int flags = w.i[I_FLAGS * P + slot];
     TriHit h;
     with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
-      intersect_closest(V.sc, ox, oy, oz, dx, dy, dz, 0, JM_INF, (flags & PF_CULL) != 0, h, st, tc);
+      if constexpr (WIDE) {
+        WideRay wr;
+        WideHit wh;
+        wh.tri = -1;
+        wh.t = INFINITY;
+        wh.u = wh.v = 0;
+        wh.front = true;
+        if (wide_ray((float)ox, (float)oy, (float)oz, (float)dx, (float)dy, (float)dz, wr))
+          wide_closest(V.wide, wr, 0.0f, INFINITY, (flags & PF_CULL) != 0, wh, st, tc, bad);
+        h.tri = wh.tri;
+        h.t = wh.tri >= 0 ? (double)wh.t : JM_INF;
+        h.u = wh.u;
+        h.v = wh.v;
+        h.front = wh.front;
+      } else {
+        intersect_closest(V.sc, ox, oy, oz, dx, dy, dz, 0, JM_INF, (flags & PF_CULL) != 0, h, st, tc);
+      }
     });
     queries++;
     w.i[I_HTRI * P + slot] = h.tri;
@@ -194,7 +204,34 @@ __global__ __launch_bounds__(256) void k_extend(const ViewDev *__restrict__ view
     w.f[F_HV * P + slot] = h.v;
     w.i[I_FLAGS * P + slot] = h.front ? (flags | PF_FRONT) : (flags & ~PF_FRONT);
   }
+  if (bad) atomicOr(&w.counts[C_OVERFLOW], OVERFLOW_WIDE);
   flush_counts(cnt, tc, queries, 0);
+}
+
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_extend(const ViewDev *__restrict__ views, Wavefront w, int q,
+                                                int32_t *spill, uint32_t spill_stride,
+                                                DevCounters *cnt) {
+  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid == 0) w.counts[C_SHADOW] = 0;  // shade of this iteration appends to it
+  LaneStack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.stride = 256;
+  st.spill = spill + gid;
+  st.spill_stride = (int)spill_stride;
+  extend_loop<MIX, false>(views, w, q, st, cnt);
+}
+
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_extend_wide(const ViewDev *__restrict__ views, Wavefront w, int q,
+                                                        int32_t *spill, uint32_t spill_stride,
+                                                        DevCounters *cnt) {
+  __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid == 0) w.counts[C_SHADOW] = 0;
+  WideStack st{lds_base + threadIdx.x, lds_bits + threadIdx.x, spill + gid, (int)spill_stride, 256, 0};
+  extend_loop<MIX, true>(views, w, q, st, cnt);
 }
 
 // Writes one finished sample of job j (RGBA, doubles rounded to float as Float32Array.set does)
@@ -760,24 +797,17 @@ __device__ inline bool visits_before(int2 a, int2 b) {
   return (a.y & 0x7fffffff) < (b.y & 0x7fffffff);
 }
 
-template <bool MIX>
-__global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ views, Wavefront w,
-                                                int32_t *spill, uint32_t spill_stride,
-                                                double *seen, int2 *seen_meta, DevCounters *cnt) {
-  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+template <bool MIX, bool WIDE, class Stack>
+MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &w, Stack &st,
+                            double *seen, int2 *seen_meta, DevCounters *cnt) {
   uint32_t n = w.counts[C_SHADOW];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nthreads = gridDim.x * blockDim.x;
-  LaneStack st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = 256;
-  st.spill = spill + gid;
-  st.spill_stride = (int)spill_stride;
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
   size_t P = w.capacity;
   const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
-  bool overflow = false;
+  bool overflow = false, bad = false;
   double *T = seen + gid;
   int2 *M = seen_meta + gid;
   const size_t S = nthreads;
@@ -794,7 +824,18 @@ __global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ view
     with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
     const DevScene &sc = V.sc;
     ShadowCollector col{&sc, V.st.shadows == MCR_SHADOWS_CAST, T, M, S, 0, false};
-    bool blocked = occluded(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, col);
+    bool blocked;
+    if constexpr (WIDE) {
+      WideRay wr;
+      // Flags and material come from the triangle's FP64-path record.
+      auto on_hit = [&](int k, int leaf, float t, bool front) {
+        return col(k, leaf, (double)t, front, sc.tri[(size_t)k * 3 + 2]);
+      };
+      blocked = wide_ray((float)ox, (float)oy, (float)oz, (float)dx, (float)dy, (float)dz, wr) &&
+                wide_occluded(V.wide, wr, 0.0f, (float)maxT, st, tc, bad, on_hit);
+    } else {
+      blocked = occluded(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, col);
+    }
     overflow |= col.overflow;
     double depth[3] = {0, 0, 0};
     if (blocked) {
@@ -854,8 +895,35 @@ __global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ view
     w.shadow[2 * RS + idx] = c[2];
     w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
   }
-  if (overflow) atomicOr(&w.counts[C_OVERFLOW], 1u);
+  if (overflow) atomicOr(&w.counts[C_OVERFLOW], OVERFLOW_SEEN);
+  if (bad) atomicOr(&w.counts[C_OVERFLOW], OVERFLOW_WIDE);
   flush_counts(cnt, tc, 0, queries);
+}
+
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ views, Wavefront w,
+                                                int32_t *spill, uint32_t spill_stride,
+                                                double *seen, int2 *seen_meta, DevCounters *cnt) {
+  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  LaneStack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.stride = 256;
+  st.spill = spill + gid;
+  st.spill_stride = (int)spill_stride;
+  shadow_loop<MIX, false>(views, w, st, seen, seen_meta, cnt);
+}
+
+// Five waves per SIMD: the any-hit loop is latency bound, and at the register count the compiler
+// picks unasked only four fit (4.74 -> 4.44 ms per C1 pass). The closest-hit variant would spill.
+template <bool MIX>
+__global__ __launch_bounds__(256, 5) void k_shadow_wide(const ViewDev *__restrict__ views, Wavefront w,
+                                                        int32_t *spill, uint32_t spill_stride,
+                                                        double *seen, int2 *seen_meta, DevCounters *cnt) {
+  __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  WideStack st{lds_base + threadIdx.x, lds_bits + threadIdx.x, spill + gid, (int)spill_stride, 256, 0};
+  shadow_loop<MIX, true>(views, w, st, seen, seen_meta, cnt);
 }
 
 // ---- resolve: finish the vertex's radiance in the reference's summation order ---------------------
@@ -1144,18 +1212,20 @@ void launch_sort_queue(hipStream_t s, const Wavefront &w, int q, uint32_t live) 
   hipLaunchKernelGGL(k_sort_mark, dim3(blocks_for(live ? live : 1)), dim3(256), 0, s, w, q);
   hipLaunchKernelGGL(k_sort_gather, dim3(blocks_for(w.capacity)), dim3(256), 0, s, w, q);
 }
-void launch_extend(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+void launch_extend(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
-  MCR_LAUNCH_MIX(k_extend, grid, s, views, w, q, spill, spill_stride, cnt);
+  if (wide) MCR_LAUNCH_MIX(k_extend_wide, grid, s, views, w, q, spill, spill_stride, cnt);
+  else MCR_LAUNCH_MIX(k_extend, grid, s, views, w, q, spill, spill_stride, cnt);
 }
 void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                   uint32_t max_live, DevCounters *cnt) {
   MCR_LAUNCH_MIX(k_shade, blocks_for(max_live ? max_live : 1), s, views, w, q, cnt);
 }
-void launch_shadow(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, uint32_t grid,
+void launch_shadow(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
                    DevCounters *cnt) {
-  MCR_LAUNCH_MIX(k_shadow, grid, s, views, w, spill, spill_stride, seen, seen_meta, cnt);
+  if (wide) MCR_LAUNCH_MIX(k_shadow_wide, grid, s, views, w, spill, spill_stride, seen, seen_meta, cnt);
+  else MCR_LAUNCH_MIX(k_shadow, grid, s, views, w, spill, spill_stride, seen, seen_meta, cnt);
 }
 void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                     uint32_t max_live) {

@@ -8,7 +8,7 @@
 #define MCR_WAVEFRONT_H
 
 #include "../../include/mcr.h"
-#include "device.h"
+#include "wide.h"
 
 #define MCR_MAX_LIGHTS 16          // punctual lights traced per scene
 #define MCR_SHADOW_HIT_CAP 64      // see-through surfaces remembered per shadow ray (tracer.ts:48)
@@ -52,7 +52,8 @@ enum CountWord {
   C_QUEUE0 = 0,     // live slots in queue 0
   C_QUEUE1 = 1,     // live slots in queue 1
   C_SHADOW = 2,     // shadow rays queued by shade
-  C_OVERFLOW = 3,   // set when a shadow ray met more see-through hits than it can keep
+  C_OVERFLOW = 3,   // OVERFLOW_* bits: a shadow ray met more see-through hits than it can keep, or a
+                    // wide traversal met a corrupt node or a full stack
   C_FREE = 4,       // slots on the free list
   C_NEXT = 5,       // next sample to start (may run past Control::total; the host clamps it)
   C_EXTEND_FETCH = 6,
@@ -60,6 +61,8 @@ enum CountWord {
   C_SORT = 8,       // appends of the queue re-sort (k_sort_gather)
   C_WORDS = 9,
 };
+
+enum OverflowBits { OVERFLOW_SEEN = 1, OVERFLOW_WIDE = 2 };
 
 struct JobDev;
 
@@ -77,6 +80,7 @@ struct Control {
 // share one wavefront run; kernels make the view wave-uniform before they read it.
 struct ViewDev {
   DevScene sc;
+  WideScene wide;            // the precision form's layout, when the run uses it
   mcr_camera cam;
   mcr_settings st;
   uint32_t light_slots;      // min(light count, MCR_MAX_LIGHTS)

@@ -16,11 +16,12 @@
 // = positive side of axis a), so taking hit slots in the order i ^ (the ray's negative-axis bits)
 // runs roughly front to back.
 //
-// Triangle (48 B = 3 x float4): {v0, index} {v1, side} {v2, 0}. v1 and v2 are the reference's FP64
+// Triangle (48 B = 3 x float4): {v0, index} {v1, side} {v2, leaf}. v1 and v2 are the reference's FP64
 // vertex sums v0 + e1 and v0 + e2 (bvh.ts:26-27) rounded once to FP32; the binary node boxes are
 // those sums' FP32 bounds (bvh.ts:28-31), and rounding to nearest is monotone, so every FP32
 // triangle lies inside its leaf's box. index is the triangle's position in BVH order, the id the
-// FP64 queries return.
+// FP64 queries return; leaf is the first triangle of the binary leaf it came from (shadow rays
+// replay see-through hits in the reference's leaf order).
 #ifndef MCR_WIDE_H
 #define MCR_WIDE_H
 
@@ -48,6 +49,7 @@ struct WideScene {
 struct WideChild {
   int32_t inner;
   uint32_t first, count;
+  uint32_t leaf;           // first triangle of the binary leaf a range came from
   float b[6];
 };
 
@@ -324,5 +326,105 @@ WHD bool wide_closest(const WideScene &ws, const WideRay &r, float tmin, float t
   }
   return hit.tri >= 0;
 }
+
+// Shadow-ray query over the wide BVH (tracer.ts:529-594 over bvh.ts:289-332): every triangle hit
+// in (tmin, tmax), no culling, nearer slots first; on_hit(k, leaf, t, front) returns true for an
+// opaque hit, which ends the query (the light is blocked whatever else lies on the segment).
+template <class Stack, class OnHit>
+WHD bool wide_occluded(const WideScene &ws, const WideRay &r, float tmin, float tmax, Stack &st,
+                       TravCount &cnt, bool &bad, OnHit &on_hit) {
+  if (ws.node_count == 0) return false;
+  st.sp = 0;
+  uint32_t node = 0;
+  uint32_t g_base = 0, g_imask = 0, g_hits = 0;
+  for (;;) {
+    if (node >= ws.node_count) {
+      bad = true;
+      return false;
+    }
+    cnt.nodes++;
+    const uint4 *np = ws.nodes + (size_t)node * 5;
+    uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+    uint32_t ih, lh;
+    wide_node_hits(n0, n1, n2, n3, n4, r, tmin, tmax, ih, lh);
+    while (lh) {
+      uint32_t b = (uint32_t)__builtin_ctz(lh);
+      lh &= lh - 1;
+      uint32_t slot = b ^ r.oct;
+      uint32_t first = n1.y + wide_leaf_offset(n1, slot), count = wide_leaf_count(n1, slot);
+      if (first + count > ws.triangle_count) {
+        bad = true;
+        return false;
+      }
+      cnt.leaves++;
+      for (uint32_t k = first; k < first + count; k++) {
+        cnt.tris++;
+        const float4 *tp = ws.tri + (size_t)k * 3;
+        float4 a = tp[0], b4 = tp[1], c = tp[2];
+        float t, u, v;
+        bool front;
+        if (!wide_tri_test(a, b4, c, r, 2, t, u, v, front)) continue;
+        if (t > tmin && t < tmax &&
+            on_hit((int)__float_as_uint(a.w), (int)__float_as_uint(c.w), t, front))
+          return true;
+      }
+    }
+    if (ih) {
+      if (g_hits) {
+        if (st.sp >= MCR_WIDE_STACK_MAX) {
+          bad = true;
+          return false;
+        }
+        st.push(g_base, g_imask | (g_hits << 8));
+      }
+      g_base = n1.x;
+      g_imask = n0.w >> 24;
+      g_hits = ih;
+    }
+    if (!g_hits) {
+      if (st.sp == 0) break;
+      uint32_t bits;
+      st.pop(g_base, bits);
+      g_imask = bits & 0xffu;
+      g_hits = bits >> 8;
+    }
+    uint32_t b = (uint32_t)__builtin_ctz(g_hits);
+    g_hits &= g_hits - 1;
+    uint32_t slot = b ^ r.oct;
+    node = g_base + (uint32_t)__builtin_popcount(g_imask & ((1u << slot) - 1u));
+  }
+  return false;
+}
+
+// Node-group stack of the wide traversals: base and (imask | hits << 8) words, lane-interleaved
+// in LDS (`lanes` threads per block), then in the thread's spill area, two words per entry.
+struct WideStack {
+  uint32_t *lb, *lw;
+  int32_t *spill;
+  int spill_stride, lanes;
+  int sp;
+  __device__ void push(uint32_t base, uint32_t bits) {
+    if (sp < MCR_WIDE_STACK_LDS) {
+      lb[sp * lanes] = base;
+      lw[sp * lanes] = bits;
+    } else {
+      size_t o = (size_t)(2 * (sp - MCR_WIDE_STACK_LDS)) * spill_stride;
+      spill[o] = (int32_t)base;
+      spill[o + spill_stride] = (int32_t)bits;
+    }
+    sp++;
+  }
+  __device__ void pop(uint32_t &base, uint32_t &bits) {
+    sp--;
+    if (sp < MCR_WIDE_STACK_LDS) {
+      base = lb[sp * lanes];
+      bits = lw[sp * lanes];
+    } else {
+      size_t o = (size_t)(2 * (sp - MCR_WIDE_STACK_LDS)) * spill_stride;
+      base = (uint32_t)spill[o];
+      bits = (uint32_t)spill[o + spill_stride];
+    }
+  }
+};
 
 #endif  // MCR_WIDE_H

@@ -35,10 +35,12 @@ __device__ void decode_ref(const WideBuild &a, int32_t ref, WideChild &c) {
       c.first = (uint32_t)l.x;
       c.count = (uint32_t)l.y;
     }
+    c.leaf = c.first;
   } else {
     c.inner = ref;
     c.first = 0;
     c.count = 0;
+    c.leaf = 0;
   }
 }
 
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(256) void k_wide_level(WideBuild a, uint32_t lo, ui
         o[1] = make_float4((float)((double)A.x + (double)A.w), (float)((double)A.y + (double)B.x),
                            (float)((double)A.z + (double)B.y), __uint_as_float(__float_as_uint(C.y) & 0xffu));
         o[2] = make_float4((float)((double)A.x + (double)B.z), (float)((double)A.y + (double)B.w),
-                           (float)((double)A.z + (double)C.x), 0.0f);
+                           (float)((double)A.z + (double)C.x), __uint_as_float(ch[j].leaf));
       }
       off += ch[j].count;
     }
@@ -216,37 +218,6 @@ __global__ __launch_bounds__(256) void k_wide_level(WideBuild a, uint32_t lo, ui
   n[4] = make_uint4(q[8], q[9], q[10], q[11]);
 }
 
-// Node-group stack: base and (imask | hits << 8) words, lane-interleaved in LDS, then in the
-// thread's spill area (two words per entry).
-struct WideStack {
-  uint32_t *lb, *lw;
-  int32_t *spill;
-  int spill_stride;
-  int sp;
-  __device__ void push(uint32_t base, uint32_t bits) {
-    if (sp < MCR_WIDE_STACK_LDS) {
-      lb[sp * 256] = base;
-      lw[sp * 256] = bits;
-    } else {
-      size_t o = (size_t)(2 * (sp - MCR_WIDE_STACK_LDS)) * spill_stride;
-      spill[o] = (int32_t)base;
-      spill[o + spill_stride] = (int32_t)bits;
-    }
-    sp++;
-  }
-  __device__ void pop(uint32_t &base, uint32_t &bits) {
-    sp--;
-    if (sp < MCR_WIDE_STACK_LDS) {
-      base = lb[sp * 256];
-      bits = lw[sp * 256];
-    } else {
-      size_t o = (size_t)(2 * (sp - MCR_WIDE_STACK_LDS)) * spill_stride;
-      base = (uint32_t)spill[o];
-      bits = (uint32_t)spill[o + spill_stride];
-    }
-  }
-};
-
 // Closest hits of caller rays (6 floats each) over the wide BVH.
 __global__ __launch_bounds__(256) void k_wide_closest(WideScene ws, const float *rays, uint32_t n, float tmin,
                                                       float tmax, int cull, int32_t *tri, float *t, float *u,
@@ -260,6 +231,7 @@ __global__ __launch_bounds__(256) void k_wide_closest(WideScene ws, const float 
   st.lw = lds_bits + threadIdx.x;
   st.spill = spill + gid;
   st.spill_stride = (int)spill_stride;
+  st.lanes = 256;
   st.sp = 0;
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;

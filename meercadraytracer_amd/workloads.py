@@ -116,10 +116,12 @@ FP64_NODE_BYTES, WIDE_NODE_BYTES, TRI_BYTES = 64, 80, 48
 
 
 def precision_form(ctx, gpu, geometry, camera, width=1920, height=1080, n=1_000_000, seed=1234, repeats=5,
-                   log=None):
+                   settings=None, render_passes=16, log=None):
     """Closest-hit throughput of the FP64 binary query and the FP32 wide query on the same resident
     probe rays (inputs and outputs in HBM), with node visits, triangle tests and the wide BVH's
-    build time. Returns a dict."""
+    build time; with `settings`, also `render_passes` full-frame passes rendered with each
+    intersection mode (same seeds): throughput, the relative difference of the frame means and the
+    fraction of pixel channels whose means differ by more than 3 sigma. Returns a dict."""
     import ctypes as C
     from . import abi, native
     rays = probe_rays(geometry, camera, width, height, n, seed, lambda r: gpu.intersect_closest(r, cull=True))
@@ -161,6 +163,37 @@ def precision_form(ctx, gpu, geometry, camera, width=1920, height=1080, n=1_000_
     ctx.set_timing(False)
     for b in (r32, r64, tri, tbuf):
         b.free()
+    if settings is not None and render_passes:
+        from .raytrace import render_passes as run_passes
+        npx = width * height
+        data = native.DeviceBuffer(ctx, render_passes * npx * 16)
+        stats = {}
+        try:
+            for name, mode in (('fp64', abi.MCR_ISECT_FP64), ('wide_fp32', abi.MCR_ISECT_WIDE_FP32)):
+                ctx.set_intersection(mode)
+                try:
+                    run_passes(gpu, camera, settings, width, height, range(1), data.ptr, abi.Counters())
+                    cnt = abi.Counters()
+                    t0 = time.perf_counter()
+                    run_passes(gpu, camera, settings, width, height, range(render_passes), data.ptr, cnt)
+                    dt = time.perf_counter() - t0
+                finally:
+                    ctx.set_intersection(abi.MCR_ISECT_FP64)
+                img = data.download(np.float32, render_passes * npx * 4).reshape(render_passes, npx, 4)[..., :3]
+                stats[name] = ((cnt.closest_queries + cnt.any_queries) / dt / 1e6, img.mean(0, dtype=np.float64),
+                               img.var(0, dtype=np.float64, ddof=1))
+                del img
+                if log:
+                    log(f'{name} render: {stats[name][0]:.1f} Mrays/s')
+        finally:
+            data.free()
+        (ra, ma, va), (rb, mb, vb) = stats['fp64'], stats['wide_fp32']
+        se = np.sqrt(va / render_passes + vb / render_passes)
+        diff = np.abs(ma - mb)
+        out['render'] = {
+            'passes': render_passes, 'fp64_mrays_per_s': round(ra, 1), 'wide_fp32_mrays_per_s': round(rb, 1),
+            'frame_mean_rel_diff': float(abs(mb.mean() - ma.mean()) / ma.mean()),
+            'pixel_channels_outside_3_sigma': float(((diff > 3 * se) & (diff > 1e-12)).mean())}
     return out
 
 

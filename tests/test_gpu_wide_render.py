@@ -1,0 +1,110 @@
+"""Renders traced through the precision form (mcr_set_intersection(MCR_ISECT_WIDE_FP32)): closest
+hits and shadow rays over the 8-wide BVH in FP32, shading and accumulation unchanged. Images must
+match the FP64 renders statistically — per pixel channel within 3 sigma of the two Monte Carlo
+estimators at equal samples (at most 1% outside), 4096-sample crop means within 0.5% — and the
+precision form must be deterministic."""
+import numpy as np
+import pytest
+
+import oracle
+from meercadraytracer_amd import abi, native, scenes
+from meercadraytracer_amd.raytrace import trace_settings
+from meercadraytracer_amd.types import Tile
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(kind, seed, **kw):
+    sc, cam, o = scenes.compiled(kind, seed, build_bvh=oracle.build_bvh, **kw)
+    gpu = native.Scene()
+    gpu.set_geometry(sc.geometry, 1)
+    gpu.set_shading(sc.shading, 1)
+    gpu.build_wide()
+    return sc, cam, o, gpu
+
+
+@pytest.fixture(scope='module')
+def c0():
+    s = _scene('plain_cpu', 0, width=256, height=256)
+    yield s
+    s[3].release()
+
+
+@pytest.fixture(scope='module')
+def glass():
+    """A small C1-like scene: heightfield, icospheres with 20% glass, sun and point lights."""
+    s = _scene('plain_gpu', 2, width=96, height=96, quads=40, spheres=16, subdivisions=2)
+    yield s
+    s[3].release()
+
+
+def _passes(gpu, cam, st, tile, w, h, n, mode):
+    ctx = native.default_context()
+    jobs = [abi.TileJob(tile.to_c(), w, h, p, 1, p * tile.width * tile.height * 4) for p in range(n)]
+    ctx.set_intersection(mode)
+    try:
+        out = gpu.render_jobs(cam, st, jobs)
+    finally:
+        ctx.set_intersection(abi.MCR_ISECT_FP64)
+    return out.reshape(n, -1, 4)
+
+
+def _three_sigma(a, b):
+    a, b = a[..., :3].astype(np.float64), b[..., :3].astype(np.float64)
+    n = a.shape[0]
+    se = np.sqrt(a.var(0, ddof=1) / n + b.var(0, ddof=1) / n)
+    diff = np.abs(a.mean(0) - b.mean(0))
+    return ((diff > 3 * se) & (diff > 1e-12)).mean()
+
+
+@pytest.mark.parametrize('fixture,seed', [('c0', 0), ('c0', 1), ('glass', 0), ('glass', 1)])
+def test_wide_render_three_sigma(request, fixture, seed):
+    sc, cam, o, gpu = request.getfixturevalue(fixture)
+    w, h = (256, 256) if fixture == 'c0' else (96, 96)
+    tile = Tile(96, 96, 32, 32) if fixture == 'c0' else Tile(24, 24, 48, 48)
+    st64 = trace_settings(dict(o, globalIllumination=True, maxBounces=4))
+    stw = trace_settings(dict(o, globalIllumination=True, maxBounces=4, seed=seed))
+    n = 256
+    f = _passes(gpu, cam, st64, tile, w, h, n, abi.MCR_ISECT_FP64)
+    g = _passes(gpu, cam, stw, tile, w, h, n, abi.MCR_ISECT_WIDE_FP32)
+    frac = _three_sigma(g, f)
+    print(f'{fixture}, wide seed {seed}: {frac:.4%} of pixel channels outside 3 sigma', flush=True)
+    assert frac <= 0.01
+    if seed == 0:  # same seeds: most paths take the same triangles
+        assert np.abs(g - f).max() < 1.0
+
+
+def test_wide_render_converged_mean(glass):
+    sc, cam, o, gpu = glass
+    st = trace_settings(dict(o, globalIllumination=True, maxBounces=4, seed=3))
+    tile = Tile(40, 40, 16, 16)
+    n = 4096
+    f = _passes(gpu, cam, trace_settings(dict(o, globalIllumination=True, maxBounces=4)), tile, 96, 96, n,
+                abi.MCR_ISECT_FP64)
+    g = _passes(gpu, cam, st, tile, 96, 96, n, abi.MCR_ISECT_WIDE_FP32)
+    mf, mg = f[..., :3].astype(np.float64).mean(), g[..., :3].astype(np.float64).mean()
+    rel = abs(mg - mf) / mf
+    print(f'4096-spp crop mean: fp64 {mf:.6f} wide {mg:.6f} rel {rel:.2e}', flush=True)
+    assert rel <= 0.005
+
+
+def test_wide_render_deterministic(glass):
+    sc, cam, o, gpu = glass
+    st = trace_settings(dict(o, globalIllumination=True, maxBounces=8))
+    tile = Tile(0, 0, 96, 96)
+    a = _passes(gpu, cam, st, tile, 96, 96, 4, abi.MCR_ISECT_WIDE_FP32)
+    b = _passes(gpu, cam, st, tile, 96, 96, 4, abi.MCR_ISECT_WIDE_FP32)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_wide_render_needs_wide_bvh():
+    sc, cam, o = scenes.compiled('plain_cpu', 0, build_bvh=oracle.build_bvh, width=32, height=32)
+    gpu = native.Scene()
+    gpu.set_geometry(sc.geometry, 1)
+    gpu.set_shading(sc.shading, 1)
+    with pytest.raises(native.NativeError) as e:
+        _passes(gpu, cam, trace_settings(o), Tile(0, 0, 32, 32), 32, 32, 1, abi.MCR_ISECT_WIDE_FP32)
+    assert 'no wide BVH' in str(e.value)
+    gpu.release()
+    with pytest.raises(native.NativeError):
+        native.default_context().set_intersection(7)
