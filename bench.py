@@ -241,7 +241,15 @@ def main():
         r = ctx.build_bvh(tris, count)
         build_times.append(time.time() - t)
         return r
+    tc = time.time()
     sc = compile_scene(scene, timed_build)
+    compile_host_s = time.time() - tc
+    # compileGeometry on the device (flatten, build, BVH-order gather, tangents) on the same scene.
+    from meercadraytracer_amd.compile import compile_scene_device
+    compile_scene_device(scene, ctx)  # allocations and first-touch
+    t1 = time.time()
+    compile_scene_device(scene, ctx)
+    compile_device_s = time.time() - t1
     cam = compile_camera(camera)
     st = trace_settings(opts)
     log(f'compiled {sc.geometry.triangleCount} triangles, {sc.geometry.bvh.nodeCount} nodes in '
@@ -303,7 +311,9 @@ def main():
                                  'shadow': round(timing['any_ms'] / args.steps, 2),
                                  'shade': round(timing['shade_ms'] / args.steps, 2),
                                  'device_total': round(timing['total_ms'] / args.steps, 2)},
-                   'bvh_build_ms_incl_copies': round(build_times[0] * 1e3, 1)},
+                   'bvh_build_ms_incl_copies': round(build_times[0] * 1e3, 1),
+                   'compile_geometry_ms': {'host_mirror_with_gpu_build': round(compile_host_s * 1e3, 1),
+                                           'device': round(compile_device_s * 1e3, 1)}},
         'roofline': roofline,
     }
     if not args.no_traffic:

@@ -339,6 +339,56 @@ int mcr_intersect_closest_wide(mcr_ctx *ctx, mcr_scene *scene, const float *rays
                                float *u_out, float *v_out, uint8_t *front_out, int device_ptrs,
                                mcr_counters *counters);
 
+/* ---- scene flattening: compileGeometry (src/compile.ts:212-306, 327-432) on the device ---- */
+/* One drawn group of a mesh (geometry.groups, or one group covering everything): what addMesh
+ * (compile.ts:237-252) derives from its material. Invisible and wireframe groups are left out by
+ * the caller, as addMesh skips them. */
+typedef struct mcr_mesh_group {
+  uint32_t start, count;     /* index range; count 0xffffffff = Infinity */
+  uint32_t material;         /* materialIndex(material) */
+  uint8_t flags;             /* MCR_TRI_*: cast/receive of the mesh, opaque of the material */
+  uint8_t side;              /* material.side */
+  uint8_t flat_shading;      /* material.flatShading */
+  uint8_t vertex_colors;     /* material.vertexColors === true and the geometry has colours */
+} mcr_mesh_group;
+
+/* One mesh, or one instance of an InstancedMesh (world = matrixWorld x instanceMatrix, as
+ * multiply4 forms it, compile.ts:180-189). Positions are the ones getVertexPosition gives for
+ * morphed and skinned meshes. */
+typedef struct mcr_mesh {
+  double world[16];          /* column-major, matrixWorld.elements */
+  const float *positions;    /* vertex_count*3 */
+  const float *normals;      /* vertex_count*3, or NULL */
+  const float *uvs;          /* vertex_count*2, or NULL */
+  const float *colors;       /* vertex_count*color_size, or NULL */
+  const uint32_t *index;     /* index_count entries, or NULL (non-indexed) */
+  const mcr_mesh_group *groups;
+  uint32_t vertex_count, index_count, color_size, group_count;
+  uint32_t draw_start, draw_count;  /* geometry.drawRange; count 0xffffffff = Infinity */
+} mcr_mesh;
+
+/* compileGeometry's output arrays, caller-allocated for triangle_count triangles (from
+ * mcr_compile_count) and 2*triangle_count-1 BVH nodes; colors only when any_colors. */
+typedef struct mcr_geometry_out {
+  uint32_t triangle_count;   /* in: the count mcr_compile_count gave */
+  uint32_t node_count;       /* out */
+  float *triangles, *normals, *uvs, *tangents, *colors;
+  uint32_t *materials;
+  uint8_t *flags, *sides;
+  float *bvh_bounds;
+  int32_t *bvh_nodes;
+  double epsilon;            /* out */
+} mcr_geometry_out;
+
+/* Triangles the meshes produce (addMesh's loops, compile.ts:235-251) and whether any group has
+ * vertex colours (the colors array is then present, white where a group has none). Host only. */
+int mcr_compile_count(const mcr_mesh *meshes, uint32_t mesh_count, uint32_t *triangle_count_out,
+                      int *any_colors_out);
+/* compileGeometry (compile.ts:327-403): flattens the meshes on the device, builds the BVH there
+ * (as mcr_build_bvh), stores every array in BVH order with tangents (writeTangent, :406-432) and
+ * the epsilon. Replaces the main-thread compile (compile.ts:361 calls buildBvh inside it). */
+int mcr_compile_geometry(mcr_ctx *ctx, const mcr_mesh *meshes, uint32_t mesh_count, mcr_geometry_out *out);
+
 /* ---- device buffers, for hosts without a GPU allocator of their own ----------------------- */
 int mcr_device_alloc(mcr_ctx *ctx, uint64_t bytes, void **out);
 int mcr_device_free(mcr_ctx *ctx, void *ptr);

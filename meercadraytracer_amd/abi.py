@@ -134,6 +134,28 @@ class Timing(C.Structure):
     ]
 
 
+class MeshGroup(C.Structure):
+    _fields_ = [('start', C.c_uint32), ('count', C.c_uint32), ('material', C.c_uint32), ('flags', C.c_uint8),
+                ('side', C.c_uint8), ('flat_shading', C.c_uint8), ('vertex_colors', C.c_uint8)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [('world', C.c_double * 16), ('positions', C.c_void_p), ('normals', C.c_void_p),
+                ('uvs', C.c_void_p), ('colors', C.c_void_p), ('index', C.c_void_p), ('groups', C.c_void_p),
+                ('vertex_count', C.c_uint32), ('index_count', C.c_uint32), ('color_size', C.c_uint32),
+                ('group_count', C.c_uint32), ('draw_start', C.c_uint32), ('draw_count', C.c_uint32)]
+
+
+class GeometryOut(C.Structure):
+    _fields_ = [('triangle_count', C.c_uint32), ('node_count', C.c_uint32), ('triangles', C.c_void_p),
+                ('normals', C.c_void_p), ('uvs', C.c_void_p), ('tangents', C.c_void_p), ('colors', C.c_void_p),
+                ('materials', C.c_void_p), ('flags', C.c_void_p), ('sides', C.c_void_p),
+                ('bvh_bounds', C.c_void_p), ('bvh_nodes', C.c_void_p), ('epsilon', C.c_double)]
+
+
+INF_COUNT = 0xffffffff
+
+
 class WideInfo(C.Structure):
     _fields_ = [('node_count', C.c_uint32), ('triangle_count', C.c_uint32), ('depth', C.c_uint32),
                 ('pad0', C.c_uint32), ('bytes', C.c_uint64), ('build_ms', C.c_double)]

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libmcr.so')
-SOURCES = ['api.hip', 'engine.hip', 'server.hip', 'trace.hip', 'bvh_build.hip', 'wide.hip']
+SOURCES = ['api.hip', 'engine.hip', 'server.hip', 'trace.hip', 'bvh_build.hip', 'wide.hip', 'compile.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-ffp-contract=off',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wall', '-Wno-unused-function', '-Wno-unused-value', '-Wno-unused-result']

@@ -329,6 +329,37 @@ def compile_scene(scene, build_bvh=None):  # compile.ts:513-527
     return CompiledScene(compile_geometry(parts, build_bvh), shading)
 
 
+def mesh_descriptor(mesh, mat_index, mat):
+    """The plain arrays addMesh reads (compile.ts:212-251) for mcr_compile_geometry, or None when
+    the mesh contributes nothing (as add_mesh)."""
+    geo = mesh.geometry
+    if geo is None or geo.position is None or not mat.visible or mat.wireframe:
+        return None
+    opaque = mat.transparent is not True and not (mat.alphaTest > 0) and not (getattr(mat, 'transmission', 0) > 0)
+    flags = ((abi.TRI_CAST_SHADOW if mesh.castShadow else 0) |
+             (abi.TRI_RECEIVE_SHADOW if mesh.receiveShadow else 0) | (abi.TRI_OPAQUE if opaque else 0))
+    colors = bool(mat.vertexColors) and geo.color is not None
+    return {'world': list(mesh.matrixWorld), 'positions': geo.position, 'normals': geo.normal, 'uvs': geo.uv,
+            'colors': geo.color, 'index': geo.index, 'draw': None,
+            'groups': [(0, None, mat_index, flags, mat.side, bool(mat.flatShading), colors)]}
+
+
+def compile_scene_device(scene, ctx=None):
+    """compile_scene with compileGeometry on the device (mcr_compile_geometry): flattening, the BVH
+    build, the BVH-order gather and the tangents in one device pass; the same arrays as
+    compile_scene with the GPU builder."""
+    from . import native
+    shading, meshes, index = compile_shading(scene)
+    descs = [mesh_descriptor(m, index.get(id(m.material), 0), m.material) for m in meshes]
+    out = native.compile_meshes(ctx, [d for d in descs if d is not None])
+    geometry = CompiledGeometry(
+        triangleCount=out['count'], triangles=out['triangles'], normals=out['normals'], uvs=out['uvs'],
+        tangents=out['tangents'], colors=out['colors'], materials=out['materials'], flags=out['flags'],
+        sides=out['sides'], bvh=CompiledBvh(out['bounds'], out['nodes'], out['node_count']),
+        epsilon=float(out['epsilon']))
+    return CompiledScene(geometry, shading)
+
+
 def compile_camera(camera, aspect=None):  # compile.ts:536-557
     if camera.parent is None:
         camera.updateMatrixWorld()

@@ -74,6 +74,8 @@ def lib():
             'mcr_scene_wide_info': (C.c_int, [P, C.POINTER(abi.WideInfo)]),
             'mcr_intersect_closest_wide': (C.c_int, [P, P, P, C.c_uint32, C.c_float, C.c_float, C.c_int, P, P, P,
                                                      P, P, C.c_int, P]),
+            'mcr_compile_count': (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
+            'mcr_compile_geometry': (C.c_int, [P, P, C.c_uint32, C.POINTER(abi.GeometryOut)]),
             'mcr_intersect_any_count': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, P, C.c_int, P]),
             'mcr_device_alloc': (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
             'mcr_device_free': (C.c_int, [P, P]),
@@ -303,6 +305,66 @@ class Scene:
             self.release()
         except Exception:
             pass
+
+
+def compile_meshes(ctx, meshes):
+    """compileGeometry on the device (mcr_compile_geometry). meshes: list of dicts with 'world'
+    (16 floats, column-major), 'positions' (v, 3) float32 and optional 'normals', 'uvs', 'colors'
+    ((v, 3|4) float32), 'index' (uint32), 'draw' ((start, count) or None) and 'groups': list of
+    (start, count, material, flags, side, flat_shading, vertex_colors), count None = Infinity.
+    Returns the arrays of a CompiledGeometry as a dict."""
+    ctx = ctx or default_context()
+    keep = []
+    arr = (abi.Mesh * max(1, len(meshes)))()
+    for i, m in enumerate(meshes):
+        d = arr[i]
+        d.world[:] = [float(x) for x in m['world']]
+        for name, dt, width in (('positions', np.float32, 3), ('normals', np.float32, 3), ('uvs', np.float32, 2)):
+            a = m.get(name)
+            if a is not None:
+                a = np.ascontiguousarray(a, dtype=dt).reshape(-1, width)
+                keep.append(a)
+                setattr(d, name, a.ctypes.data)
+                d.vertex_count = len(a)
+        c = m.get('colors')
+        if c is not None:
+            c = np.ascontiguousarray(c, dtype=np.float32)
+            keep.append(c)
+            d.colors, d.color_size = c.ctypes.data, c.shape[1]
+        idx = m.get('index')
+        if idx is not None:
+            idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+            keep.append(idx)
+            d.index, d.index_count = idx.ctypes.data, len(idx)
+        draw = m.get('draw') or (0, None)
+        d.draw_start = draw[0]
+        d.draw_count = abi.INF_COUNT if draw[1] is None else draw[1]
+        groups = (abi.MeshGroup * max(1, len(m['groups'])))()
+        for k, (start, count, material, flags, side, flat, colors) in enumerate(m['groups']):
+            groups[k] = abi.MeshGroup(start, abi.INF_COUNT if count is None else count, material, flags, side,
+                                      int(bool(flat)), int(bool(colors)))
+        keep.append(groups)
+        d.groups, d.group_count = C.cast(groups, C.c_void_p), len(m['groups'])
+    n = C.c_uint32()
+    any_colors = C.c_int()
+    check(lib().mcr_compile_count(C.cast(arr, C.c_void_p), len(meshes), C.byref(n), C.byref(any_colors)))
+    n = n.value
+    nodes = max(1, 2 * n - 1)
+    out = {'triangles': np.empty(n * 9, np.float32), 'normals': np.empty(n * 9, np.float32),
+           'uvs': np.empty(n * 6, np.float32), 'tangents': np.empty(n * 4, np.float32),
+           'colors': np.empty(n * 12 if any_colors.value else 0, np.float32),
+           'materials': np.empty(n, np.uint32), 'flags': np.empty(n, np.uint8), 'sides': np.empty(n, np.uint8),
+           'bounds': np.empty(nodes * 6, np.float32), 'nodes': np.empty(nodes * 3, np.int32)}
+    g = abi.GeometryOut()
+    g.triangle_count = n
+    for f in ('triangles', 'normals', 'uvs', 'tangents', 'colors', 'materials', 'flags', 'sides'):
+        setattr(g, f, out[f].ctypes.data if out[f].size else None)
+    g.bvh_bounds, g.bvh_nodes = out['bounds'].ctypes.data, out['nodes'].ctypes.data
+    check(lib().mcr_compile_geometry(ctx.handle, C.cast(arr, C.c_void_p), len(meshes), C.byref(g)))
+    out['bounds'] = out['bounds'][:g.node_count * 6]
+    out['nodes'] = out['nodes'][:g.node_count * 3]
+    out['count'], out['node_count'], out['epsilon'] = n, g.node_count, g.epsilon
+    return out
 
 
 def encode_tile64_device(ctx, image_ptr, width, tile, data_ptr, sum_ptr, count, srgb):
