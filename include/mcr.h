@@ -50,6 +50,30 @@ enum { MCR_SHADOWS_ALL = 0, MCR_SHADOWS_CAST = 1, MCR_SHADOWS_NONE = 2 };       
 enum { MCR_FOG_NONE = 0, MCR_FOG_LINEAR = 1, MCR_FOG_EXP2 = 2 };                 /* types.ts:115-121 */
 enum { MCR_BACKGROUND_CLEAR = 0, MCR_BACKGROUND_COLOR = 1, MCR_BACKGROUND_SCREEN = 2,
        MCR_BACKGROUND_ENVIRONMENT = 3 };                                         /* types.ts:123-127 */
+enum { MCR_WRAP_REPEAT = 1000, MCR_WRAP_CLAMP = 1001, MCR_WRAP_MIRROR = 1002 };  /* constants.ts:29-31 */
+enum { MCR_COMBINE_MULTIPLY = 0, MCR_COMBINE_MIX = 1, MCR_COMBINE_ADD = 2 };     /* constants.ts:9-11 */
+enum { MCR_ENV_EQUIRECT = 0, MCR_ENV_CUBE = 1 };                                 /* types.ts:23-39 */
+
+/* CompiledTexture (types.ts:8-21): RGBA texels, first row first, 8-bit (is_float 0; srgb marks
+ * sRGB-encoded colour channels) or float32 (is_float 1; half floats decoded as textures.ts:63-105
+ * does). matrix is three.js Texture.matrix, column-major 3x3. */
+typedef struct mcr_texture {
+  uint32_t width, height;
+  const void *data;
+  int32_t is_float, srgb, flip_y, wrap_s, wrap_t, pad0;
+  double matrix[9];
+} mcr_texture;
+
+/* CompiledEnvironment (types.ts:23-39): equirect (texture) or cube (faces +x -x +y -y +z -z,
+ * flip_x). rotation is column-major 3x3, applied to directions before the lookup. */
+typedef struct mcr_environment {
+  int32_t kind;
+  int32_t texture;
+  int32_t faces[6];
+  int32_t flip_x, pad0;
+  double rotation[9];
+  double intensity;
+} mcr_environment;
 
 /* ---- plain-data scene (types.ts) -------------------------------------------------------- */
 
@@ -106,9 +130,10 @@ typedef struct mcr_light {
   double distance, decay, cone_cos, penumbra_cos;
 } mcr_light;
 
-/* Everything of CompiledScene but the geometry (protocol.ts:19-27 ShadingPayload). This build
- * traces no textures or environments: texture_count and environment_count must be 0, every
- * texture index -1, and background_kind CLEAR or COLOR; otherwise MCR_E_UNSUPPORTED. */
+/* Everything of CompiledScene but the geometry (protocol.ts:19-27 ShadingPayload), with the
+ * textures message's arrays (protocol.ts:31-35). Texture and environment indices are checked
+ * against the counts; at most 15 punctual lights when environments are present (one light slot
+ * carries the environment-diffuse ray of tracer.ts:343-353). */
 typedef struct mcr_shading {
   uint32_t material_count, light_count;
   const mcr_material *materials;
@@ -120,6 +145,11 @@ typedef struct mcr_shading {
   int32_t fog_kind, pad0;    /* MCR_FOG_* */
   double fog_color[3];
   double fog_near, fog_far, fog_density;
+  const mcr_texture *textures;           /* texture_count */
+  const mcr_environment *environments;   /* environment_count */
+  int32_t background_texture;            /* MCR_BACKGROUND_SCREEN */
+  int32_t background_environment;        /* MCR_BACKGROUND_ENVIRONMENT */
+  double background_intensity;           /* scene.backgroundIntensity of a screen background */
 } mcr_shading;
 
 /* CompiledCamera (types.ts:181-190), matrices column-major. */

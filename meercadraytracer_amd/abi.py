@@ -24,6 +24,9 @@ MODELS = {'physical': 0, 'phong': 1, 'lambert': 2, 'toon': 3, 'basic': 4}
 LIGHTS = {'ambient': 0, 'hemisphere': 1, 'directional': 2, 'point': 3, 'spot': 4}
 SHADOWS = {'all': 0, 'castShadow': 1, 'none': 2}
 FOGS = {None: 0, 'linear': 1, 'exp2': 2}
+RepeatWrapping, ClampToEdgeWrapping, MirroredRepeatWrapping = 1000, 1001, 1002  # constants.ts:29-31
+MultiplyOperation, MixOperation, AddOperation = 0, 1, 2                           # constants.ts:9-11
+ENV_KINDS = {'equirect': 0, 'cube': 1}
 BACKGROUNDS = {'clear': 0, 'color': 1, 'screen': 2, 'environment': 3}
 
 d3 = C.c_double * 3
@@ -68,6 +71,17 @@ class Light(C.Structure):
     ]
 
 
+class Texture(C.Structure):
+    _fields_ = [('width', C.c_uint32), ('height', C.c_uint32), ('data', C.c_void_p), ('is_float', C.c_int32),
+                ('srgb', C.c_int32), ('flip_y', C.c_int32), ('wrap_s', C.c_int32), ('wrap_t', C.c_int32),
+                ('pad0', C.c_int32), ('matrix', C.c_double * 9)]
+
+
+class Environment(C.Structure):
+    _fields_ = [('kind', C.c_int32), ('texture', C.c_int32), ('faces', C.c_int32 * 6), ('flip_x', C.c_int32),
+                ('pad0', C.c_int32), ('rotation', C.c_double * 9), ('intensity', C.c_double)]
+
+
 class Shading(C.Structure):
     _fields_ = [
         ('material_count', C.c_uint32), ('light_count', C.c_uint32),
@@ -78,6 +92,9 @@ class Shading(C.Structure):
         ('fog_kind', C.c_int32), ('pad0', C.c_int32),
         ('fog_color', d3), ('fog_near', C.c_double), ('fog_far', C.c_double),
         ('fog_density', C.c_double),
+        ('textures', C.c_void_p), ('environments', C.c_void_p),
+        ('background_texture', C.c_int32), ('background_environment', C.c_int32),
+        ('background_intensity', C.c_double),
     ]
 
 

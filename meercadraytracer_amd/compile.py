@@ -17,7 +17,8 @@ import math
 import numpy as np
 
 from . import abi
-from .types import (CompiledBvh, CompiledCamera, CompiledFog, CompiledGeometry, CompiledLight,
+from .types import (CompiledBvh, CompiledCamera, CompiledEnvironment, CompiledFog, CompiledGeometry, CompiledLight,
+                    CompiledTexture,
                     CompiledMaterial, CompiledScene, ShadingPayload)
 
 
@@ -82,7 +83,103 @@ def _volume_attenuation(m):  # compile.ts:86-92
     return (coeff(colour.r), coeff(colour.g), coeff(colour.b))
 
 
-def compile_material(m):  # compile.ts:94-147
+IDENTITY3 = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+
+def _half_to_float(h):  # color.ts:28-35
+    h = np.asarray(h, dtype=np.uint32)
+    sign = np.where(h & 0x8000, -1.0, 1.0)
+    e = (h >> 10) & 0x1f
+    f = (h & 0x3ff).astype(np.float64)
+    sub = sign * 2.0 ** -14 * (f / 1024)
+    inf = np.where(f != 0, np.nan, sign * np.inf)
+    norm = sign * 2.0 ** (e.astype(np.float64) - 15) * (1 + f / 1024)
+    return np.where(e == 0, sub, np.where(e == 0x1f, inf, norm))
+
+
+def _read_pixels(image, type_):
+    """textures.ts:63-105 for RGBA data: uint8 stays 8-bit; float and half float become float32."""
+    data = np.asarray(image['data'])
+    w, h = image['width'], image['height']
+    if type_ == 1016:  # HalfFloatType
+        data = _half_to_float(data).astype(np.float32)
+    elif data.dtype != np.uint8:
+        data = data.astype(np.float32)
+    if data.size != w * h * 4:
+        return None
+    return np.ascontiguousarray(data.reshape(-1))
+
+
+def _euler_matrix_transposed(euler):  # textures.ts:204-217 (XYZ order)
+    if euler is None or tuple(euler) == (0, 0, 0):
+        return IDENTITY3
+    x, y, z = euler
+
+    def mul(a, b):
+        return [a[r] * b[c * 3] + a[3 + r] * b[c * 3 + 1] + a[6 + r] * b[c * 3 + 2] for c in range(3) for r in range(3)]
+    rx = [1, 0, 0, 0, math.cos(x), math.sin(x), 0, -math.sin(x), math.cos(x)]
+    ry = [math.cos(y), 0, -math.sin(y), 0, 1, 0, math.sin(y), 0, math.cos(y)]
+    rz = [math.cos(z), math.sin(z), 0, -math.sin(z), math.cos(z), 0, 0, 0, 1]
+    m = mul(mul(rx, ry), rz)
+    return (m[0], m[3], m[6], m[1], m[4], m[7], m[2], m[5], m[8])
+
+
+class TextureTable:  # textures.ts:134-202
+    def __init__(self):
+        self.textures, self.environments = [], []
+        self._index, self._env_index = {}, {}
+
+    def add(self, texture):
+        if texture is None or not getattr(texture, 'isTexture', False) or texture.isCubeTexture:
+            return -1
+        if id(texture) in self._index:
+            return self._index[id(texture)]
+        if texture.matrixAutoUpdate is not False:
+            texture.updateMatrix()
+        pixels = _read_pixels(texture.image, texture.type)
+        index = -1
+        if pixels is not None:
+            index = len(self.textures)
+            self.textures.append(CompiledTexture(
+                texture.image['width'], texture.image['height'], pixels,
+                srgb=pixels.dtype == np.uint8 and texture.colorSpace == 'srgb', flipY=texture.flipY is True,
+                wrapS=texture.wrapS, wrapT=texture.wrapT, matrix=tuple(texture.matrix)))
+        self._index[id(texture)] = index
+        return index
+
+    def add_environment(self, texture, intensity, rotation):
+        if texture is None or not getattr(texture, 'isTexture', False):
+            return -1
+        matrix = _euler_matrix_transposed(rotation)
+        key = (id(texture), texture.version, intensity, matrix)
+        if key in self._env_index:
+            return self._env_index[key]
+        env = None
+        if texture.isCubeTexture or texture.mapping in (301, 302):
+            faces = []
+            for image in texture.image:
+                pixels = _read_pixels(image, texture.type)
+                if pixels is None:
+                    break
+                faces.append(len(self.textures))
+                self.textures.append(CompiledTexture(image['width'], image['height'], pixels,
+                                                     srgb=pixels.dtype == np.uint8 and texture.colorSpace == 'srgb',
+                                                     wrapS=texture.wrapS, wrapT=texture.wrapT))
+            if len(faces) == 6:
+                env = CompiledEnvironment('cube', faces=tuple(faces), flipX=True, rotation=matrix, intensity=intensity)
+        elif texture.mapping in (303, 304):
+            index = self.add(texture)
+            if index >= 0:
+                env = CompiledEnvironment('equirect', texture=index, rotation=matrix, intensity=intensity)
+        index = -1
+        if env is not None:
+            index = len(self.environments)
+            self.environments.append(env)
+        self._env_index[key] = index
+        return index
+
+
+def compile_material(m, table=None):  # compile.ts:94-147
     if m.isMeshStandardMaterial:
         model = 'physical'
     elif m.isMeshPhongMaterial:
@@ -114,7 +211,22 @@ def compile_material(m):  # compile.ts:94-147
         shininess=getattr(m, 'shininess', 30.0),
         specular=_rgb(spec) if spec is not None else (0.0666, 0.0666, 0.0666),
         envMapIntensity=getattr(m, 'envMapIntensity', 1.0),
-        reflectivity=getattr(m, 'reflectivity', 1.0), combine=getattr(m, 'combine', 0))
+        reflectivity=getattr(m, 'reflectivity', 1.0), combine=getattr(m, 'combine', 0),
+        **_material_maps(m, table if table is not None else TextureTable()))
+
+
+def _material_maps(m, table):  # the texture fields of compile.ts:108-145
+    env = getattr(m, 'envMap', None)
+    ns = getattr(m, 'normalScale', None)
+    return dict(map=table.add(getattr(m, 'map', None)), alphaMap=table.add(getattr(m, 'alphaMap', None)),
+                emissiveMap=table.add(getattr(m, 'emissiveMap', None)),
+                normalMap=table.add(getattr(m, 'normalMap', None)), aoMap=table.add(getattr(m, 'aoMap', None)),
+                roughnessMap=table.add(getattr(m, 'roughnessMap', None)),
+                metalnessMap=table.add(getattr(m, 'metalnessMap', None)),
+                normalScale=(float(ns[0]), float(ns[1])) if ns is not None else (1.0, 1.0),
+                aoMapIntensity=getattr(m, 'aoMapIntensity', 1.0),
+                envMap=table.add_environment(env, getattr(m, 'envMapIntensity', 1.0),
+                                             getattr(m, 'envMapRotation', None)) if env is not None else -1)
 
 
 def compile_light(light):  # compile.ts:149-177
@@ -302,15 +414,31 @@ def compile_shading(scene):  # compile.ts:476-510
     scene.updateMatrixWorld()
     meshes, lights = collect_objects(scene)
     materials, index = [], {}
+    table = TextureTable()
     for mesh in meshes:
         m = mesh.material
         if m is None or id(m) in index:
             continue
         index[id(m)] = len(materials)
-        materials.append(compile_material(m))
-    background = {'kind': 'clear'}
-    if scene.background is not None:
-        background = {'kind': 'color', 'color': _rgb(scene.background)}
+        materials.append(compile_material(m, table))
+    background = {'kind': 'clear'}  # compile.ts:434-449
+    bg = scene.background
+    if bg is not None and getattr(bg, 'isTexture', False):
+        intensity = getattr(scene, 'backgroundIntensity', 1.0)
+        if bg.isCubeTexture or bg.mapping in (301, 302, 303, 304):
+            e = table.add_environment(bg, intensity, getattr(scene, 'backgroundRotation', None))
+            if e >= 0:
+                background = {'kind': 'environment', 'environment': e, 'intensity': intensity}
+        else:
+            t = table.add(bg)
+            if t >= 0:
+                background = {'kind': 'screen', 'texture': t, 'intensity': intensity}
+    elif bg is not None:
+        background = {'kind': 'color', 'color': _rgb(bg)}
+    environment = -1
+    if scene.environment is not None:
+        environment = table.add_environment(scene.environment, getattr(scene, 'environmentIntensity', 1.0),
+                                            getattr(scene, 'environmentRotation', None))
     fog = None
     if scene.fog is not None:
         f = scene.fog
@@ -319,7 +447,8 @@ def compile_shading(scene):  # compile.ts:476-510
         else:
             fog = CompiledFog('linear', _rgb(f.color), f.near, f.far, 0.0)
     compiled_lights = [l for l in (compile_light(x) for x in lights) if l is not None]
-    shading = ShadingPayload(materials=materials, lights=compiled_lights, background=background, fog=fog)
+    shading = ShadingPayload(materials=materials, lights=compiled_lights, background=background, fog=fog,
+                             environment=environment, textures=table.textures, environments=table.environments)
     return shading, meshes, index
 
 

@@ -366,4 +366,59 @@ JM_HD double jm_cos(double x) {
   }
 }
 
+// atan on |x| <= 1: atan(x) = atan(c) + atan((x - c) / (1 + x c)) with c the nearest multiple of
+// 1/16, whose arctangents are tabulated (rounded from a 60-digit evaluation), and the Taylor series
+// of atan on the remainder (|r| <= 1/32, nine terms leave < 1e-26).
+JM_HD double jm_atan_unit(double x) {
+  const double T[17] = {0.0, 0.06241880999595735, 0.12435499454676144, 0.18534794999569476,
+                        0.24497866312686414, 0.3028848683749714, 0.35877067027057225, 0.4124104415973873,
+                        0.4636476090008061, 0.5123894603107377, 0.5585993153435624, 0.6022873461349642,
+                        0.6435011087932844, 0.6823165548747481, 0.7188299996216245, 0.7531512809621944,
+                        0.7853981633974483};
+  double a = jm_fabs(x);
+  int i = (int)(a * 16 + 0.5);
+  double c = i * 0.0625;
+  double r = (a - c) / (1 + a * c);
+  double r2 = r * r;
+  double p = 1.0 / 17;
+  p = 1.0 / 15 - r2 * p;
+  p = 1.0 / 13 - r2 * p;
+  p = 1.0 / 11 - r2 * p;
+  p = 1.0 / 9 - r2 * p;
+  p = 1.0 / 7 - r2 * p;
+  p = 1.0 / 5 - r2 * p;
+  p = 1.0 / 3 - r2 * p;
+  double v = T[i] + (r - r * r2 * p);
+  return x < 0 ? -v : v;
+}
+
+// Math.atan2 with the IEEE special cases (zeros, infinities, NaN).
+JM_HD double jm_atan2(double y, double x) {
+  const double PIO2 = 1.5707963267948966, PIO4 = 0.7853981633974483;
+  if (y != y || x != x) return JM_NAN;
+  bool yneg = jm_signbit(y);
+  double r;
+  if (y == 0) {
+    r = jm_signbit(x) ? JM_PI : 0.0;
+  } else if (x == 0) {
+    r = PIO2;
+  } else if (jm_fabs(x) == JM_INF) {
+    if (jm_fabs(y) == JM_INF) r = x > 0 ? PIO4 : 3 * PIO4;
+    else r = x > 0 ? 0.0 : JM_PI;
+  } else if (jm_fabs(y) == JM_INF) {
+    r = PIO2;
+  } else {
+    double ay = jm_fabs(y), ax = jm_fabs(x);
+    double a = ay <= ax ? jm_atan_unit(ay / ax) : PIO2 - jm_atan_unit(ax / ay);
+    r = x > 0 ? a : JM_PI - a;
+  }
+  return yneg ? -r : r;
+}
+
+// Math.asin: atan2(x, sqrt((1 - x)(1 + x))), NaN outside [-1, 1].
+JM_HD double jm_asin(double x) {
+  if (!(jm_fabs(x) <= 1)) return JM_NAN;
+  return jm_atan2(x, jm_sqrt((1 - x) * (1 + x)));
+}
+
 #endif  // MCR_JSMATH_H

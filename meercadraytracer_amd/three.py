@@ -164,9 +164,65 @@ class Object3D:
 class Scene(Object3D):
     def __init__(self):
         super().__init__()
-        self.background = None  # None or Color
+        self.background = None  # None, a Color or a texture
+        self.backgroundIntensity = 1.0
+        self.backgroundRotation = None  # Euler (x, y, z) or None
         self.fog = None
         self.environment = None
+        self.environmentIntensity = 1.0
+        self.environmentRotation = None
+
+
+# ---- textures --------------------------------------------------------------------------------
+UVMapping, CubeReflectionMapping, CubeRefractionMapping = 300, 301, 302
+EquirectangularReflectionMapping, EquirectangularRefractionMapping = 303, 304
+UnsignedByteType, FloatType, HalfFloatType = 1009, 1015, 1016
+NoColorSpace, SRGBColorSpace = '', 'srgb'
+
+
+class DataTexture:
+    """THREE.DataTexture with RGBA data, rows bottom to top as three.js uploads them (flipY false).
+    type: UnsignedByteType (uint8), FloatType (float32) or HalfFloatType (uint16 half floats)."""
+    isTexture = True
+    isCubeTexture = False
+
+    def __init__(self, data, width, height, type=UnsignedByteType):
+        self.image = {'data': np.asarray(data), 'width': int(width), 'height': int(height)}
+        self.type = type
+        self.colorSpace = NoColorSpace
+        self.flipY = False
+        self.wrapS = self.wrapT = abi.ClampToEdgeWrapping
+        self.mapping = UVMapping
+        self.offset = [0.0, 0.0]
+        self.repeat = [1.0, 1.0]
+        self.center = [0.0, 0.0]
+        self.rotation = 0.0
+        self.matrixAutoUpdate = True
+        self.matrix = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]
+        self.version = 0
+
+    def updateMatrix(self):  # Texture.updateMatrix -> Matrix3.setUvTransform, column-major
+        tx, ty = self.offset
+        sx, sy = self.repeat
+        cx, cy = self.center
+        c, s = math.cos(self.rotation), math.sin(self.rotation)
+        self.matrix = [sx * c, -sy * s, 0.0, sx * s, sy * c, 0.0,
+                       -sx * (c * cx + s * cy) + cx + tx, -sy * (-s * cx + c * cy) + cy + ty, 1.0]
+
+
+class CubeTexture:
+    """THREE.CubeTexture from six RGBA DataTexture-like images (+x, -x, +y, -y, +z, -z)."""
+    isTexture = True
+    isCubeTexture = True
+    isRenderTargetTexture = False
+
+    def __init__(self, faces, type=UnsignedByteType):
+        self.image = list(faces)
+        self.type = type
+        self.colorSpace = NoColorSpace
+        self.mapping = CubeReflectionMapping
+        self.wrapS = self.wrapT = abi.ClampToEdgeWrapping
+        self.version = 0
 
 
 # ---- geometry --------------------------------------------------------------------------------

@@ -62,6 +62,44 @@ class CompiledGeometry:  # types.ts:137-156
 
 
 @dataclass
+class CompiledTexture:  # types.ts:8-21
+    width: int
+    height: int
+    data: np.ndarray          # RGBA texels, first row first: uint8 or float32
+    srgb: bool = False
+    flipY: bool = False
+    wrapS: int = abi.ClampToEdgeWrapping
+    wrapT: int = abi.ClampToEdgeWrapping
+    matrix: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+    def fill(self, t):
+        t.width, t.height = self.width, self.height
+        t.data = self.data.ctypes.data
+        t.is_float = int(self.data.dtype == np.float32)
+        t.srgb, t.flip_y = int(self.srgb), int(self.flipY)
+        t.wrap_s, t.wrap_t = self.wrapS, self.wrapT
+        t.matrix[:] = [float(x) for x in self.matrix]
+
+
+@dataclass
+class CompiledEnvironment:  # types.ts:23-39
+    kind: str                 # 'equirect' | 'cube'
+    texture: int = -1
+    faces: tuple = ()
+    flipX: bool = True
+    rotation: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    intensity: float = 1.0
+
+    def fill(self, e):
+        e.kind = abi.ENV_KINDS[self.kind]
+        e.texture = self.texture
+        e.faces[:] = list(self.faces) if self.faces else [-1] * 6
+        e.flip_x = int(self.flipX)
+        e.rotation[:] = [float(x) for x in self.rotation]
+        e.intensity = self.intensity
+
+
+@dataclass
 class CompiledMaterial:  # types.ts:43-89, defaults as compile.ts:94-147 fills them
     model: str = 'physical'
     side: int = abi.FrontSide
@@ -159,7 +197,7 @@ class CompiledFog:  # types.ts:115-121
 
 
 @dataclass
-class ShadingPayload:  # protocol.ts:19-27 (textures and environments are not traced here)
+class ShadingPayload:  # protocol.ts:19-27, with the textures message's arrays (protocol.ts:31-35)
     materials: List[CompiledMaterial]
     lights: List[CompiledLight]
     background: dict = field(default_factory=lambda: {'kind': 'clear'})
@@ -182,15 +220,30 @@ class ShadingPayload:  # protocol.ts:19-27 (textures and environments are not tr
         s.lights = C.cast(lights, C.c_void_p)
         s.texture_count = len(self.textures)
         s.environment_count = len(self.environments)
+        texs = (abi.Texture * max(1, len(self.textures)))()
+        for i, t in enumerate(self.textures):
+            t.fill(texs[i])
+        envs = (abi.Environment * max(1, len(self.environments)))()
+        for i, e in enumerate(self.environments):
+            e.fill(envs[i])
+        s.textures = C.cast(texs, C.c_void_p)
+        s.environments = C.cast(envs, C.c_void_p)
         s.background_kind = abi.BACKGROUNDS[self.background['kind']]
+        s.background_texture, s.background_environment, s.background_intensity = -1, -1, 1.0
         if self.background['kind'] == 'color':
             s.background_color[:] = self.background['color']
+        elif self.background['kind'] == 'screen':
+            s.background_texture = self.background['texture']
+            s.background_intensity = self.background.get('intensity', 1.0)
+        elif self.background['kind'] == 'environment':
+            s.background_environment = self.background['environment']
+            s.background_intensity = self.background.get('intensity', 1.0)
         s.environment = self.environment
         if self.fog is not None:
             s.fog_kind = abi.FOGS[self.fog.kind]
             s.fog_color[:] = self.fog.color
             s.fog_near, s.fog_far, s.fog_density = self.fog.near, self.fog.far, self.fog.density
-        s._keep = (mats, lights)
+        s._keep = (mats, lights, texs, envs, [t.data for t in self.textures])
         return s
 
 

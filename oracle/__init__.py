@@ -40,6 +40,10 @@ def load(libm=False):
         'orc_brute_force': (None, [P, C.c_uint32, P, C.c_uint32, P, P, P]),
         'orc_wide_brute': (None, [P, P, C.c_uint32, P, C.c_uint32, C.c_float, C.c_float, C.c_int, P, P, P, P, P,
                                   C.c_uint32]),
+        'orc_sample_texture': (None, [P, C.c_double, C.c_double, P]),
+        'orc_sample_environment': (None, [P, P, C.c_double, C.c_double, C.c_double, P]),
+        'orc_atan2': (C.c_double, [C.c_double, C.c_double]),
+        'orc_asin': (C.c_double, [C.c_double]),
         'orc_hash': (C.c_uint32, [C.c_uint32]),
         'orc_seed': (C.c_uint32, [C.c_int32] * 4),
         'orc_random_stream': (None, [C.c_uint32, C.c_uint32, P]),
@@ -142,6 +146,27 @@ def wide_brute(geometry, rays, t_min=0.0, t_max=np.inf, cull=True, threads=None)
     lib.orc_wide_brute(_p(tris), _p(sides), len(sides), _p(rays), n, t_min, t_max, int(cull), _p(tri), _p(t),
                        _p(u), _p(v), _p(front), threads or min(16, os.cpu_count() or 1))
     return tri, t, u, v, front
+
+
+def sample_texture(texture, u, v, libm=False):
+    """textures.ts:247 sampleTexture on a types.CompiledTexture -> RGBA (4 floats)."""
+    t = abi.Texture()
+    texture.fill(t)
+    out = np.zeros(4)
+    load(libm).orc_sample_texture(C.byref(t), u, v, _p(out))
+    return out
+
+
+def sample_environment(env, textures, d, libm=False):
+    """textures.ts:287 sampleEnvironment -> RGB."""
+    e = abi.Environment()
+    env.fill(e)
+    ts = (abi.Texture * max(1, len(textures)))()
+    for i, t in enumerate(textures):
+        t.fill(ts[i])
+    out = np.zeros(3)
+    load(libm).orc_sample_environment(C.byref(e), ts, d[0], d[1], d[2], _p(out))
+    return out
 
 
 def render_tile(scene, camera, settings, tile, width, height, pass_, block=1, libm=False):

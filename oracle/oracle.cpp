@@ -36,6 +36,8 @@ inline double M_pow(double x, double y) { return ::pow(x, y); }
 inline double M_sin(double x) { return ::sin(x); }
 inline double M_cos(double x) { return ::cos(x); }
 inline double M_log2(double x) { return ::log2(x); }
+inline double M_atan2(double y, double x) { return ::atan2(y, x); }
+inline double M_asin(double x) { return ::asin(x); }
 #else
 inline double M_exp(double x) { return jm_exp(x); }
 inline double M_log(double x) { return jm_log(x); }
@@ -43,8 +45,11 @@ inline double M_pow(double x, double y) { return jm_pow(x, y); }
 inline double M_sin(double x) { return jm_sin(x); }
 inline double M_cos(double x) { return jm_cos(x); }
 inline double M_log2(double x) { return jm_log2(x); }
+inline double M_atan2(double y, double x) { return jm_atan2(y, x); }
+inline double M_asin(double x) { return jm_asin(x); }
 #endif
 inline double M_hypot(double x, double y, double z) { return js_hypot3(x, y, z); }
+inline double orOne(double v) { return (v == 0 || v != v) ? 1 : v; }  // `x || 1`
 inline double Mmin(double a, double b) { return js_min(a, b); }
 inline double Mmax(double a, double b) { return js_max(a, b); }
 
@@ -513,6 +518,100 @@ void toneMap(double *c, int mapping, double exposure) {
   }
 }
 
+// ---- textures.ts ----------------------------------------------------------------------------
+// color.ts:21-25: a Float32Array of the linear values of the 256 sRGB levels.
+struct SrgbTable {
+  float v[256];
+  SrgbTable() {
+    for (int i = 0; i < 256; i++) v[i] = (float)srgbToLinear(i / 255.0);
+  }
+};
+const SrgbTable SRGB8;
+
+// textures.ts:229-236
+double wrapCoord(double coordinate, int mode) {
+  if (mode == MCR_WRAP_CLAMP) return coordinate < 0 ? 0 : coordinate > 1 ? 1 : coordinate;
+  if (mode == MCR_WRAP_MIRROR) {
+    double period = coordinate - 2 * jm_floor(coordinate / 2);
+    return period > 1 ? 2 - period : period;
+  }
+  return coordinate - jm_floor(coordinate);
+}
+// textures.ts:238-241 (i is an integer-valued double)
+int64_t texelIndex(double i, int64_t size, int mode) {
+  int64_t k = (int64_t)i;
+  if (mode == MCR_WRAP_CLAMP || mode == MCR_WRAP_MIRROR) return k < 0 ? 0 : k >= size ? size - 1 : k;
+  return ((k % size) + size) % size;
+}
+// textures.ts:255-285
+void sampleTexel(const mcr_texture &texture, double s, double t, double *out) {
+  int64_t w = texture.width, h = texture.height;
+  double row = texture.flip_y ? 1 - t : t;
+  double x = s * w - 0.5, y = row * h - 0.5;
+  double x0 = jm_floor(x), y0 = jm_floor(y);
+  double fx = x - x0, fy = y - y0;
+  int64_t xa = texelIndex(x0, w, texture.wrap_s), xb = texelIndex(x0 + 1, w, texture.wrap_s);
+  int64_t ya = texelIndex(y0, h, texture.wrap_t), yb = texelIndex(y0 + 1, h, texture.wrap_t);
+  int64_t i00 = (ya * w + xa) * 4, i10 = (ya * w + xb) * 4, i01 = (yb * w + xa) * 4, i11 = (yb * w + xb) * 4;
+  double w00 = (1 - fx) * (1 - fy), w10 = fx * (1 - fy), w01 = (1 - fx) * fy, w11 = fx * fy;
+  if (texture.is_float) {
+    const float *d = (const float *)texture.data;
+    for (int c = 0; c < 4; c++)
+      out[c] = (double)d[i00 + c] * w00 + (double)d[i10 + c] * w10 + (double)d[i01 + c] * w01 + (double)d[i11 + c] * w11;
+    return;
+  }
+  const uint8_t *d = (const uint8_t *)texture.data;
+  for (int c = 0; c < 4; c++) {
+    if (texture.srgb && c < 3) {
+      out[c] = (double)SRGB8.v[d[i00 + c]] * w00 + (double)SRGB8.v[d[i10 + c]] * w10 +
+               (double)SRGB8.v[d[i01 + c]] * w01 + (double)SRGB8.v[d[i11 + c]] * w11;
+    } else {
+      out[c] = ((double)d[i00 + c] * w00 + (double)d[i10 + c] * w10 + (double)d[i01 + c] * w01 +
+                (double)d[i11 + c] * w11) / 255;
+    }
+  }
+}
+// textures.ts:247-253
+void sampleTexture(const mcr_texture &texture, double u, double v, double *out) {
+  const double *m = texture.matrix;
+  double tu = m[0] * u + m[3] * v + m[6];
+  double tv = m[1] * u + m[4] * v + m[7];
+  sampleTexel(texture, wrapCoord(tu, texture.wrap_s), wrapCoord(tv, texture.wrap_t), out);
+}
+// textures.ts:287-317
+void sampleEnvironment(const mcr_environment &env, const mcr_texture *textures, double dx, double dy,
+                       double dz, double *out) {
+  const double *r = env.rotation;
+  double x = r[0] * dx + r[3] * dy + r[6] * dz;
+  double y = r[1] * dx + r[4] * dy + r[7] * dz;
+  double z = r[2] * dx + r[5] * dy + r[8] * dz;
+  double tex[4];
+  if (env.kind == MCR_ENV_EQUIRECT) {
+    double length = orOne(M_hypot(x, y, z));
+    double u = M_atan2(z, x) / (2 * PI) + 0.5;
+    double v = M_asin(Mmax(-1, Mmin(1, y / length))) / PI + 0.5;
+    const mcr_texture &texture = textures[env.texture];
+    sampleTexel(texture, wrapCoord(u, texture.wrap_s), Mmax(0, Mmin(1, v)), tex);
+  } else {
+    if (env.flip_x) x = -x;
+    double ax = jm_fabs(x), ay = jm_fabs(y), az = jm_fabs(z);
+    int face;
+    double sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+      face = x > 0 ? 0 : 1; ma = ax; sc = x > 0 ? -z : z; tc = -y;
+    } else if (ay >= az) {
+      face = y > 0 ? 2 : 3; ma = ay; sc = x; tc = y > 0 ? z : -z;
+    } else {
+      face = z > 0 ? 4 : 5; ma = az; sc = z > 0 ? x : -x; tc = -y;
+    }
+    double s2 = (sc / ma + 1) * 0.5, t2 = (tc / ma + 1) * 0.5;
+    sampleTexel(textures[env.faces[face]], Mmax(0, Mmin(1, s2)), Mmax(0, Mmin(1, t2)), tex);
+  }
+  out[0] = tex[0] * env.intensity;
+  out[1] = tex[1] * env.intensity;
+  out[2] = tex[2] * env.intensity;
+}
+
 // ---- brdf.ts --------------------------------------------------------------------------------
 struct DfgTable {  // brdf.ts:8-12
   float v[512];
@@ -604,7 +703,6 @@ void cameraRay(const mcr_camera &camera, double x, double y, double *out) {
   out[3] = dx; out[4] = dy; out[5] = dz;
 }
 
-inline double orOne(double v) { return (v == 0 || v != v) ? 1 : v; }  // `x || 1`
 
 const int MAX_PASS_THROUGH = 64;  // tracer.ts:30
 
@@ -666,16 +764,23 @@ class Tracer {
     int passes = 0;
     bool cull = true;
     double mr = 0, mg = 0, mb = 0;
-    (void)sx; (void)sy;
+    const mcr_environment *lastEnvironment = nullptr;
+    double lastEnvironmentScale = 1;
 
     for (;;) {
       if (!intersectClosest(geometry.triangles, geometry.sides, bvh_, ox, oy, oz, dx, dy, dz, 0, INF,
                             cull, hit, stack_, cnt_)) {
         if (!surfaceSeen) {
-          background(out);
+          background(dx, dy, dz, sx, sy, out);
           return;
         }
-        // lastEnvironment is always null without environments (tracer.ts:119-124).
+        if (lastEnvironment) {  // tracer.ts:119-124
+          double env[3];
+          sampleEnvironment(*lastEnvironment, s_.textures, dx, dy, dz, env);
+          lr += tr * env[0] * lastEnvironmentScale;
+          lg += tg * env[1] * lastEnvironmentScale;
+          lb += tb * env[2] * lastEnvironmentScale;
+        }
         break;
       }
       if (mr > 0 || mg > 0 || mb > 0) {
@@ -694,14 +799,27 @@ class Tracer {
       double face = hit.front ? 1 : -1;
       gx = (gx / gl) * face; gy = (gy / gl) * face; gz = (gz / gl) * face;
 
+      const float *uvs = geometry.uvs + (size_t)k * 6;
+      double u = uvs[0] * w0 + uvs[2] * w1 + uvs[4] * w2;
+      double v = uvs[1] * w0 + uvs[3] * w1 + uvs[5] * w2;
+      double texel[4];
+
       double cr = material.color[0], cg = material.color[1], cb = material.color[2];
       double alpha = material.opacity;
+      if (material.map >= 0) {
+        sampleTexture(s_.textures[material.map], u, v, texel);
+        cr *= texel[0]; cg *= texel[1]; cb *= texel[2]; alpha *= texel[3];
+      }
       if (material.vertex_colors && geometry.colors) {
         const float *colors = geometry.colors + (size_t)k * 12;
         cr *= colors[0] * w0 + colors[4] * w1 + colors[8] * w2;
         cg *= colors[1] * w0 + colors[5] * w1 + colors[9] * w2;
         cb *= colors[2] * w0 + colors[6] * w1 + colors[10] * w2;
         alpha *= colors[3] * w0 + colors[7] * w1 + colors[11] * w2;
+      }
+      if (material.alpha_map >= 0) {
+        sampleTexture(s_.textures[material.alpha_map], u, v, texel);
+        alpha *= texel[1];
       }
       bool cutOut = material.alpha_test > 0 && alpha < material.alpha_test;
       if (cutOut || (material.transparent && random.next() >= alpha)) {
@@ -721,17 +839,42 @@ class Tracer {
       double nz = (nrm[2] * w0 + nrm[5] * w1 + nrm[8] * w2) * face;
       double nl = orOne(M_hypot(nx, ny, nz));
       nx /= nl; ny /= nl; nz /= nl;
+      if (material.normal_map >= 0) {  // tracer.ts:191-209
+        sampleTexture(s_.textures[material.normal_map], u, v, texel);
+        double mx = (texel[0] * 2 - 1) * material.normal_scale[0];
+        double my = (texel[1] * 2 - 1) * material.normal_scale[1];
+        double mz = texel[2] * 2 - 1;
+        const float *tg4 = geometry.tangents + (size_t)k * 4;
+        double ax = tg4[0], ay = tg4[1], az = tg4[2];
+        double along = ax * nx + ay * ny + az * nz;
+        ax -= nx * along; ay -= ny * along; az -= nz * along;
+        double al = orOne(M_hypot(ax, ay, az));
+        ax = (ax / al) * face; ay = (ay / al) * face; az = (az / al) * face;
+        double sign = tg4[3];
+        double bx = (ny * az - nz * ay) * sign, by = (nz * ax - nx * az) * sign, bz = (nx * ay - ny * ax) * sign;
+        nx = ax * mx + bx * my + nx * mz;
+        ny = ay * mx + by * my + ny * mz;
+        nz = az * mx + bz * my + nz * mz;
+        nl = orOne(M_hypot(nx, ny, nz));
+        nx /= nl; ny /= nl; nz /= nl;
+      }
 
       double vx = -dx, vy = -dy, vz = -dz;
       double dotNV = Mmin(Mmax(nx * vx + ny * vy + nz * vz, 0), 1);
       double outR = 0, outG = 0, outB = 0;
       double er = material.emissive[0], eg = material.emissive[1], eb = material.emissive[2];
+      if (material.emissive_map >= 0 && (er > 0 || eg > 0 || eb > 0)) {
+        sampleTexture(s_.textures[material.emissive_map], u, v, texel);
+        er *= texel[0]; eg *= texel[1]; eb *= texel[2];
+      }
       bool receive = (geometry.flags[k] & MCR_TRI_RECEIVE_SHADOW) != 0;
       bool canContinue = bounce < settings_.max_bounces;
       double specR = 0, specG = 0, specB = 0;
       double diffR = 0, diffG = 0, diffB = 0;
       double transR = 0, transG = 0, transB = 0;
       double roughness = 1;
+      const mcr_environment *environment = nullptr;
+      double environmentScale = 1;
 
       if (material.model == MCR_MODEL_BASIC) {
         outR = cr; outG = cg; outB = cb;
@@ -749,9 +892,21 @@ class Tracer {
           }
         }
         double ao = 1;
+        if (material.ao_map >= 0) {
+          sampleTexture(s_.textures[material.ao_map], u, v, texel);
+          ao = (texel[0] - 1) * material.ao_map_intensity + 1;
+        }
         if (material.model == MCR_MODEL_PHYSICAL) {
           double metalness = material.metalness;
           roughness = material.roughness;
+          if (material.roughness_map >= 0) {
+            sampleTexture(s_.textures[material.roughness_map], u, v, texel);
+            roughness *= texel[1];
+          }
+          if (material.metalness_map >= 0) {
+            sampleTexture(s_.textures[material.metalness_map], u, v, texel);
+            metalness *= texel[2];
+          }
           double shadeRoughness = Mmin(Mmax(roughness, 0.045), 1);
           double sr = 0.04, sg = 0.04, sb = 0.04, f90 = 1;
           if (material.physical) {
@@ -818,7 +973,24 @@ class Tracer {
             transG = transmission * (1 - fg) * cg;
             transB = transmission * (1 - fbl) * cb;
           }
-          // No environment (tracer.ts:336-353 need one).
+          if (material.env_map >= 0) environment = &s_.environments[material.env_map];
+          else if (s_.environment >= 0) {
+            environment = &s_.environments[s_.environment];
+            environmentScale = material.env_map_intensity;
+          }
+          // tracer.ts:341-353: environment light on the diffuse layer, one cosine-weighted direction.
+          if (environment && !settings_.global_illumination) {
+            cosineDirection(nx, ny, nz);
+            double sx2 = sx_, sy2 = sy_, sz2 = sz_;
+            if (sx2 * gx + sy2 * gy + sz2 * gz > 0 &&
+                transmittance(px + gx * eps, py + gy * eps, pz + gz * eps, sx2, sy2, sz2, INF, false)) {
+              double env[3];
+              sampleEnvironment(*environment, s_.textures, sx2, sy2, sz2, env);
+              outR += env[0] * environmentScale * diffR * shadowColor_[0];
+              outG += env[1] * environmentScale * diffG * shadowColor_[1];
+              outB += env[2] * environmentScale * diffB * shadowColor_[2];
+            }
+          }
           if (!settings_.global_illumination) diffR = diffG = diffB = 0;
         } else {
           for (uint32_t li = 0; li < s_.light_count; li++) {
@@ -856,6 +1028,21 @@ class Tracer {
           }
         }
       }
+      // tracer.ts:392-406: environment map on lambert, phong, toon and basic materials.
+      if (material.model != MCR_MODEL_PHYSICAL && material.env_map >= 0) {
+        const mcr_environment &envm = s_.environments[material.env_map];
+        double dn = dx * nx + dy * ny + dz * nz;
+        double e[3];
+        sampleEnvironment(envm, s_.textures, dx - 2 * dn * nx, dy - 2 * dn * ny, dz - 2 * dn * nz, e);
+        double sr2 = material.reflectivity;
+        if (material.combine == MCR_COMBINE_MULTIPLY) {
+          outR *= 1 - sr2 + sr2 * e[0]; outG *= 1 - sr2 + sr2 * e[1]; outB *= 1 - sr2 + sr2 * e[2];
+        } else if (material.combine == MCR_COMBINE_MIX) {
+          outR += (e[0] - outR) * sr2; outG += (e[1] - outG) * sr2; outB += (e[2] - outB) * sr2;
+        } else if (material.combine == MCR_COMBINE_ADD) {
+          outR += e[0] * sr2; outG += e[1] * sr2; outB += e[2] * sr2;
+        }
+      }
       lr += tr * (outR + er);
       lg += tg * (outG + eg);
       lb += tb * (outB + eb);
@@ -873,6 +1060,8 @@ class Tracer {
         wr = specR; wg = specG; wb = specB;
         reflection(dx, dy, dz, nx, ny, nz, roughness);
         if (sx_ * gx + sy_ * gy + sz_ * gz <= 0) break;
+        lastEnvironment = environment;
+        lastEnvironmentScale = environmentScale;
       } else if (pick < ws + wt) {
         p = wt / total;
         wr = transR; wg = transG; wb = transB;
@@ -891,11 +1080,15 @@ class Tracer {
         } else {
           sx_ = dx; sy_ = dy; sz_ = dz;
         }
+        lastEnvironment = environment;
+        lastEnvironmentScale = environmentScale;
       } else {
         p = wd / total;
         wr = diffR; wg = diffG; wb = diffB;
         cosineDirection(nx, ny, nz);
         if (sx_ * gx + sy_ * gy + sz_ * gz <= 0) break;
+        lastEnvironment = environment;
+        lastEnvironmentScale = environmentScale;
       }
       tr *= wr / p; tg *= wg / p; tb *= wb / p;
       double side = sx_ * gx + sy_ * gy + sz_ * gz > 0 ? eps : -eps;
@@ -983,9 +1176,8 @@ class Tracer {
     return shadowColor_[0] + shadowColor_[1] + shadowColor_[2] > 0;
   }
 
-  // tracer.ts:544-594 (no texture maps in this build)
+  // tracer.ts:544-594
   bool attenuateShadow(int triangle, double t, double u, double v, bool front) {
-    (void)u; (void)v;
     int flags = g_.flags[triangle];
     if (shadowCastOnly_ && (flags & MCR_TRI_CAST_SHADOW) == 0) return true;
     if (flags & MCR_TRI_OPAQUE) {
@@ -995,6 +1187,22 @@ class Tracer {
     const mcr_material &material = s_.materials[g_.materials[triangle]];
     double alpha = material.opacity;
     double cr = material.color[0], cg = material.color[1], cb = material.color[2];
+    if (material.map >= 0 || material.alpha_map >= 0) {
+      const float *uvs = g_.uvs + (size_t)triangle * 6;
+      double w0 = 1 - u - v;
+      double tu = uvs[0] * w0 + uvs[2] * u + uvs[4] * v;
+      double tv = uvs[1] * w0 + uvs[3] * u + uvs[5] * v;
+      double texel[4];
+      if (material.map >= 0) {
+        sampleTexture(s_.textures[material.map], tu, tv, texel);
+        alpha *= texel[3];
+        cr *= texel[0]; cg *= texel[1]; cb *= texel[2];
+      }
+      if (material.alpha_map >= 0) {
+        sampleTexture(s_.textures[material.alpha_map], tu, tv, texel);
+        alpha *= texel[1];
+      }
+    }
     if (material.alpha_test > 0 && alpha < material.alpha_test) return true;
     if (!material.transparent) alpha = 1;
     double eps = g_.epsilon;
@@ -1072,11 +1280,30 @@ class Tracer {
     if (s_.fog_kind == MCR_FOG_LINEAR) return smoothstep(s_.fog_near, s_.fog_far, depth);
     return 1 - M_exp(-s_.fog_density * s_.fog_density * depth * depth);
   }
-  // tracer.ts:659-691 (clear and colour backgrounds)
-  void background(double *out) {
+  // tracer.ts:659-691
+  void background(double dx, double dy, double dz, double sx, double sy, double *out) {
     if (s_.background_kind == MCR_BACKGROUND_COLOR) {
       out[0] = s_.background_color[0]; out[1] = s_.background_color[1]; out[2] = s_.background_color[2];
       out[3] = 1;
+      return;
+    }
+    if (s_.background_kind == MCR_BACKGROUND_SCREEN || s_.background_kind == MCR_BACKGROUND_ENVIRONMENT) {
+      double c[3];
+      bool srgb;
+      if (s_.background_kind == MCR_BACKGROUND_SCREEN) {
+        const mcr_texture &texture = s_.textures[s_.background_texture];
+        double texel[4];
+        sampleTexture(texture, sx, 1 - sy, texel);
+        for (int i = 0; i < 3; i++) c[i] = texel[i] * s_.background_intensity;
+        srgb = texture.srgb != 0;
+      } else {
+        const mcr_environment &env = s_.environments[s_.background_environment];
+        sampleEnvironment(env, s_.textures, dx, dy, dz, c);
+        srgb = s_.textures[env.kind == MCR_ENV_CUBE ? env.faces[0] : env.texture].srgb != 0;
+      }
+      if (!srgb && settings_.tone_mapping != MCR_NO_TONE_MAPPING)
+        toneMap(c, settings_.tone_mapping, settings_.tone_mapping_exposure);
+      out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = 1;
       return;
     }
     double a = settings_.clear_alpha;
@@ -1101,12 +1328,25 @@ class Tracer {
 };
 
 bool supported(const mcr_shading &s) {
-  if (s.texture_count != 0 || s.environment_count != 0 || s.environment >= 0) return false;
-  if (s.background_kind != MCR_BACKGROUND_CLEAR && s.background_kind != MCR_BACKGROUND_COLOR) return false;
+  auto tex_ok = [&](int32_t i) { return i == -1 || (i >= 0 && (uint32_t)i < s.texture_count && s.textures); };
+  auto env_ok = [&](int32_t i) { return i == -1 || (i >= 0 && (uint32_t)i < s.environment_count && s.environments); };
+  if (!env_ok(s.environment)) return false;
+  if (s.background_kind == MCR_BACKGROUND_SCREEN && (s.background_texture < 0 || !tex_ok(s.background_texture)))
+    return false;
+  if (s.background_kind == MCR_BACKGROUND_ENVIRONMENT &&
+      (s.background_environment < 0 || !env_ok(s.background_environment)))
+    return false;
+  for (uint32_t i = 0; i < s.environment_count; i++) {
+    const mcr_environment &e = s.environments[i];
+    if (e.kind == MCR_ENV_EQUIRECT ? (e.texture < 0 || !tex_ok(e.texture)) : false) return false;
+    if (e.kind == MCR_ENV_CUBE)
+      for (int f = 0; f < 6; f++)
+        if (e.faces[f] < 0 || !tex_ok(e.faces[f])) return false;
+  }
   for (uint32_t i = 0; i < s.material_count; i++) {
     const mcr_material &m = s.materials[i];
-    if (m.map >= 0 || m.alpha_map >= 0 || m.emissive_map >= 0 || m.normal_map >= 0 || m.ao_map >= 0 ||
-        m.roughness_map >= 0 || m.metalness_map >= 0 || m.env_map >= 0)
+    if (!tex_ok(m.map) || !tex_ok(m.alpha_map) || !tex_ok(m.emissive_map) || !tex_ok(m.normal_map) ||
+        !tex_ok(m.ao_map) || !tex_ok(m.roughness_map) || !tex_ok(m.metalness_map) || !env_ok(m.env_map))
       return false;
   }
   return true;
@@ -1337,6 +1577,16 @@ void orc_wide_brute(const float *tris, const uint8_t *sides, uint32_t count, con
   work();
   for (auto &th : pool) th.join();
 }
+
+void orc_sample_texture(const mcr_texture *texture, double u, double v, double *out4) {
+  sampleTexture(*texture, u, v, out4);
+}
+void orc_sample_environment(const mcr_environment *env, const mcr_texture *textures, double dx, double dy,
+                            double dz, double *out3) {
+  sampleEnvironment(*env, textures, dx, dy, dz, out3);
+}
+double orc_atan2(double y, double x) { return M_atan2(y, x); }
+double orc_asin(double x) { return M_asin(x); }
 
 uint32_t orc_hash(uint32_t x) { return hash(x); }
 uint32_t orc_seed(int32_t a, int32_t b, int32_t c, int32_t d) {

@@ -44,6 +44,13 @@ void orc_wide_brute(const float *triangles, const uint8_t *sides, uint32_t count
                     uint32_t n, float t_min, float t_max, int cull, int32_t *tri, float *t, float *u,
                     float *v, uint8_t *front, uint32_t threads);
 
+/* textures.ts:247-253 sampleTexture and :287-317 sampleEnvironment, for known-answer tests. */
+void orc_sample_texture(const mcr_texture *texture, double u, double v, double *out4);
+void orc_sample_environment(const mcr_environment *env, const mcr_texture *textures, double dx, double dy,
+                            double dz, double *out3);
+double orc_atan2(double y, double x);
+double orc_asin(double x);
+
 /* random.ts */
 uint32_t orc_hash(uint32_t x);
 uint32_t orc_seed(int32_t a, int32_t b, int32_t c, int32_t d);
