@@ -39,6 +39,10 @@ void free_geometry(mcr_scene *s) {
   hipFree(s->longleaf);
   hipFree(s->normals);
   hipFree(s->colors);
+  hipFree(s->tangents);
+  hipFree(s->uvs);
+  s->tangents = nullptr;
+  s->uvs = nullptr;
   s->tri = nullptr;
   s->inner = nullptr;
   s->longleaf = nullptr;
@@ -59,19 +63,57 @@ int ensure_out_tmp(mcr_ctx *c, size_t floats) {
 }
 
 bool shading_supported(const mcr_shading *s, std::string &why) {
-  if (s->texture_count || s->environment_count || s->environment >= 0) {
-    why = "textures and environments are not traced by this build";
+  auto tex_ok = [&](int32_t i) { return i == -1 || (i >= 0 && (uint32_t)i < s->texture_count); };
+  auto env_ok = [&](int32_t i) { return i == -1 || (i >= 0 && (uint32_t)i < s->environment_count); };
+  if ((s->texture_count && !s->textures) || (s->environment_count && !s->environments)) {
+    why = "texture or environment array missing";
     return false;
   }
-  if (s->background_kind != MCR_BACKGROUND_CLEAR && s->background_kind != MCR_BACKGROUND_COLOR) {
-    why = "screen and environment backgrounds are not traced by this build";
+  for (uint32_t i = 0; i < s->texture_count; i++) {
+    const mcr_texture &t = s->textures[i];
+    if (!t.data || t.width == 0 || t.height == 0 || (uint64_t)t.width * t.height > (1ull << 28)) {
+      why = "texture " + std::to_string(i) + " has no data or a bad size";
+      return false;
+    }
+    int modes[2] = {t.wrap_s, t.wrap_t};
+    for (int m : modes)
+      if (m != MCR_WRAP_REPEAT && m != MCR_WRAP_CLAMP && m != MCR_WRAP_MIRROR) {
+        why = "texture " + std::to_string(i) + " has an unknown wrap mode";
+        return false;
+      }
+  }
+  for (uint32_t i = 0; i < s->environment_count; i++) {
+    const mcr_environment &e = s->environments[i];
+    bool ok = e.kind == MCR_ENV_EQUIRECT ? (e.texture >= 0 && tex_ok(e.texture)) : e.kind == MCR_ENV_CUBE;
+    if (e.kind == MCR_ENV_CUBE)
+      for (int f = 0; f < 6; f++) ok = ok && e.faces[f] >= 0 && tex_ok(e.faces[f]);
+    if (!ok) {
+      why = "environment " + std::to_string(i) + " names a missing texture";
+      return false;
+    }
+  }
+  if (!env_ok(s->environment)) {
+    why = "scene environment out of range";
+    return false;
+  }
+  if (s->background_kind == MCR_BACKGROUND_SCREEN && !(s->background_texture >= 0 && tex_ok(s->background_texture))) {
+    why = "screen background names a missing texture";
+    return false;
+  }
+  if (s->background_kind == MCR_BACKGROUND_ENVIRONMENT &&
+      !(s->background_environment >= 0 && env_ok(s->background_environment))) {
+    why = "environment background names a missing environment";
+    return false;
+  }
+  if (s->background_kind < MCR_BACKGROUND_CLEAR || s->background_kind > MCR_BACKGROUND_ENVIRONMENT) {
+    why = "unknown background kind";
     return false;
   }
   for (uint32_t i = 0; i < s->material_count; i++) {
     const mcr_material &m = s->materials[i];
-    if (m.map >= 0 || m.alpha_map >= 0 || m.emissive_map >= 0 || m.normal_map >= 0 || m.ao_map >= 0 ||
-        m.roughness_map >= 0 || m.metalness_map >= 0 || m.env_map >= 0) {
-      why = "material " + std::to_string(i) + " uses a texture or environment map";
+    if (!tex_ok(m.map) || !tex_ok(m.alpha_map) || !tex_ok(m.emissive_map) || !tex_ok(m.normal_map) ||
+        !tex_ok(m.ao_map) || !tex_ok(m.roughness_map) || !tex_ok(m.metalness_map) || !env_ok(m.env_map)) {
+      why = "material " + std::to_string(i) + " names a missing texture or environment";
       return false;
     }
     if (m.model < 0 || m.model > 4) {
@@ -79,20 +121,19 @@ bool shading_supported(const mcr_shading *s, std::string &why) {
       return false;
     }
   }
-  uint32_t punctual = 0;
   for (uint32_t i = 0; i < s->light_count; i++) {
     int t = s->lights[i].type;
     if (t < 0 || t > 4) {
       why = "light " + std::to_string(i) + " has an unknown type";
       return false;
     }
-    punctual++;
   }
-  if (s->light_count > MCR_MAX_LIGHTS) {
-    why = "more than " + std::to_string(MCR_MAX_LIGHTS) + " lights";
+  // One light slot carries the environment-diffuse ray when the scene has environments.
+  uint32_t cap = s->environment_count ? MCR_MAX_LIGHTS - 1 : MCR_MAX_LIGHTS;
+  if (s->light_count > cap) {
+    why = "more than " + std::to_string(cap) + " lights";
     return false;
   }
-  (void)punctual;
   return true;
 }
 
@@ -132,6 +173,14 @@ int mcr_create(int device, mcr_ctx **out) {
   float lut[512];
   for (int i = 0; i < 512; i++) lut[i] = half_to_float_js(MCR_DFG_LUT_HALF[i]);
   e = mcr_upload_dfg(lut);
+  if (e == hipSuccess) {
+    float srgb[256];  // color.ts:21-25 with the portable pow
+    for (int i = 0; i < 256; i++) {
+      double v = i / 255.0;
+      srgb[i] = (float)(v < 0.04045 ? v * 0.0773993808 : jm_pow(v * 0.9478672986 + 0.0521327014, 2.4));
+    }
+    e = mcr_upload_srgb8(srgb);
+  }
   if (e != hipSuccess) {
     mcr_destroy(c);
     return fail(MCR_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
@@ -210,6 +259,9 @@ void mcr_scene_release(mcr_scene *s) {
   free_geometry(s);
   hipFree(s->materials);
   hipFree(s->lights);
+  hipFree(s->texels);
+  hipFree(s->textures);
+  hipFree(s->environments);
   delete s;
 }
 
@@ -306,13 +358,24 @@ int mcr_scene_set_geometry(mcr_scene *s, const mcr_geometry *g, uint32_t version
     HIP_TRY(dalloc(&s->colors, (size_t)n * 12));
     HIP_TRY(hipMemcpy(s->colors, g->colors, (size_t)n * 12 * sizeof(float), hipMemcpyHostToDevice));
   }
+  // uvs and tangents, read only by textured materials (tracer.ts:151-209)
+  HIP_TRY(dalloc(&s->uvs, (size_t)n * 6));
+  HIP_TRY(dalloc(&s->tangents, (size_t)n * 4));
+  if (n && g->uvs) HIP_TRY(hipMemcpy(s->uvs, g->uvs, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice));
+  else if (n) HIP_TRY(hipMemset(s->uvs, 0, (size_t)n * 6 * sizeof(float)));
+  if (n && g->tangents)
+    HIP_TRY(hipMemcpy(s->tangents, g->tangents, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice));
+  else if (n) HIP_TRY(hipMemset(s->tangents, 0, (size_t)n * 4 * sizeof(float)));
   d.tri = s->tri;
   d.inner = s->inner;
   d.longleaf = s->longleaf;
   d.normals = s->normals;
   d.colors = s->colors;
+  d.uvs = s->uvs;
+  d.tangents = s->tangents;
   s->bytes_geometry = tri.size() * sizeof(float4) + inner.size() * sizeof(float4) +
-                      longleaf.size() * sizeof(int2) + (size_t)n * 9 * 4 + (s->colors ? (size_t)n * 48 : 0);
+                      longleaf.size() * sizeof(int2) + (size_t)n * (9 + 6 + 4) * 4 +
+                      (s->colors ? (size_t)n * 48 : 0);
   s->geometry_version = version;
   s->has_geometry = true;
   return MCR_OK;
@@ -352,6 +415,44 @@ int mcr_scene_set_shading(mcr_scene *s, const mcr_shading *sh, uint32_t version)
     d.transmission = m.transmission;
     d.thickness = m.thickness;
     d.shininess = m.shininess;
+    d.map = m.map;
+    d.alpha_map = m.alpha_map;
+    d.emissive_map = m.emissive_map;
+    d.normal_map = m.normal_map;
+    d.ao_map = m.ao_map;
+    d.roughness_map = m.roughness_map;
+    d.metalness_map = m.metalness_map;
+    d.env_map = m.env_map;
+    d.combine = m.combine;
+    d.textured = m.map >= 0 || m.alpha_map >= 0 || m.emissive_map >= 0 || m.normal_map >= 0 || m.ao_map >= 0 ||
+                 m.roughness_map >= 0 || m.metalness_map >= 0;
+    d.normal_scale[0] = m.normal_scale[0];
+    d.normal_scale[1] = m.normal_scale[1];
+    d.ao_map_intensity = m.ao_map_intensity;
+    d.env_map_intensity = m.env_map_intensity;
+    d.reflectivity = m.reflectivity;
+  }
+  // Textures: every texel array in one allocation, each 256-byte aligned.
+  std::vector<DevTexture> texs(std::max<uint32_t>(1, sh->texture_count));
+  memset(texs.data(), 0, texs.size() * sizeof(DevTexture));
+  std::vector<size_t> toff(sh->texture_count);
+  size_t tbytes = 0;
+  for (uint32_t i = 0; i < sh->texture_count; i++) {
+    const mcr_texture &t = sh->textures[i];
+    toff[i] = tbytes;
+    tbytes += ((size_t)t.width * t.height * (t.is_float ? 16 : 4) + 255) / 256 * 256;
+  }
+  std::vector<DevEnvironment> envs(std::max<uint32_t>(1, sh->environment_count));
+  memset(envs.data(), 0, envs.size() * sizeof(DevEnvironment));
+  for (uint32_t i = 0; i < sh->environment_count; i++) {
+    const mcr_environment &e = sh->environments[i];
+    DevEnvironment &d = envs[i];
+    d.kind = e.kind;
+    d.texture = e.texture;
+    for (int k = 0; k < 6; k++) d.faces[k] = e.faces[k];
+    d.flip_x = e.flip_x;
+    for (int k = 0; k < 9; k++) d.rotation[k] = e.rotation[k];
+    d.intensity = e.intensity;
   }
   std::vector<DevLight> lights(std::max<uint32_t>(1, sh->light_count));
   memset(lights.data(), 0, lights.size() * sizeof(DevLight));
@@ -373,15 +474,52 @@ int mcr_scene_set_shading(mcr_scene *s, const mcr_shading *sh, uint32_t version)
   }
   hipFree(s->materials);
   hipFree(s->lights);
+  hipFree(s->texels);
+  hipFree(s->textures);
+  hipFree(s->environments);
   s->materials = nullptr;
   s->lights = nullptr;
+  s->texels = nullptr;
+  s->textures = nullptr;
+  s->environments = nullptr;
   HIP_TRY(dalloc(&s->materials, mats.size()));
   HIP_TRY(dalloc(&s->lights, lights.size()));
+  HIP_TRY(hipMalloc(&s->texels, std::max<size_t>(tbytes, 256)));
+  for (uint32_t i = 0; i < sh->texture_count; i++) {
+    const mcr_texture &t = sh->textures[i];
+    DevTexture &d = texs[i];
+    size_t b = (size_t)t.width * t.height * (t.is_float ? 16 : 4);
+    HIP_TRY(hipMemcpy((char *)s->texels + toff[i], t.data, b, hipMemcpyHostToDevice));
+    d.data = (char *)s->texels + toff[i];
+    d.width = (int32_t)t.width;
+    d.height = (int32_t)t.height;
+    d.is_float = t.is_float ? 1 : 0;
+    d.srgb = t.srgb && !t.is_float ? 1 : 0;
+    d.flip_y = t.flip_y ? 1 : 0;
+    d.wrap_s = t.wrap_s;
+    d.wrap_t = t.wrap_t;
+    for (int k = 0; k < 9; k++) d.matrix[k] = t.matrix[k];
+  }
+  HIP_TRY(dalloc(&s->textures, texs.size()));
+  HIP_TRY(dalloc(&s->environments, envs.size()));
+  HIP_TRY(hipMemcpy(s->textures, texs.data(), texs.size() * sizeof(DevTexture), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->environments, envs.data(), envs.size() * sizeof(DevEnvironment), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->materials, mats.data(), mats.size() * sizeof(DevMaterial), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->lights, lights.data(), lights.size() * sizeof(DevLight), hipMemcpyHostToDevice));
   DevScene &d = s->d;
   d.materials = s->materials;
   d.lights = s->lights;
+  d.textures = s->textures;
+  d.environments = s->environments;
+  d.texture_count = sh->texture_count;
+  s->shadow_tex = false;
+  for (uint32_t i = 0; i < sh->material_count; i++)
+    s->shadow_tex |= sh->materials[i].map >= 0 || sh->materials[i].alpha_map >= 0;
+  d.environment_count = sh->environment_count;
+  d.environment = sh->environment;
+  d.background_texture = sh->background_texture;
+  d.background_environment = sh->background_environment;
+  d.background_intensity = sh->background_intensity;
   d.material_count = sh->material_count;
   d.light_count = sh->light_count;
   d.background_kind = sh->background_kind;
@@ -393,7 +531,8 @@ int mcr_scene_set_shading(mcr_scene *s, const mcr_shading *sh, uint32_t version)
   d.fog_near = sh->fog_near;
   d.fog_far = sh->fog_far;
   d.fog_density = sh->fog_density;
-  s->bytes_shading = mats.size() * sizeof(DevMaterial) + lights.size() * sizeof(DevLight);
+  s->bytes_shading = mats.size() * sizeof(DevMaterial) + lights.size() * sizeof(DevLight) + tbytes +
+                     texs.size() * sizeof(DevTexture) + envs.size() * sizeof(DevEnvironment);
   s->shading_version = version;
   s->has_shading = true;
   return MCR_OK;
@@ -410,6 +549,8 @@ ViewDev make_view(const mcr_scene *s, const mcr_camera &cam, const mcr_settings 
   v.cam = cam;
   v.st = st;
   v.light_slots = std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
+  v.shadow_slots = v.light_slots + (s->d.environment_count ? 1 : 0);
+  v.shadow_tex = s->shadow_tex ? 1 : 0;
   return v;
 }
 

@@ -54,6 +54,7 @@ struct Engine {
   int32_t *spill = nullptr;        // traversal stack overflow, per traversal thread
   double *seen = nullptr;          // see-through shadow hits, per traversal thread
   int2 *seen_meta = nullptr;
+  double2 *seen_uv = nullptr;      // their barycentrics, for textured see-through materials
   DevCounters *counters = nullptr;
   ViewDev *views = nullptr;
   uint32_t views_cap = 0;
@@ -66,11 +67,12 @@ struct Engine {
   uint32_t *counts_host = nullptr; // pinned
   bool mix = false;                // the run holds more than one view
   bool wide = false;               // the run traces with the precision form's FP32 wide BVH
+  bool tex = false;                // a view of the run has textured see-through materials
   // Two captured bounce iterations (queue 0 then queue 1), replayed while paths are live; one
   // graph per kernel variant (single or mixed views, FP64 binary or FP32 wide traversal).
   bool use_graph = true;
-  hipGraphExec_t graph[4] = {};
-  const void *graph_key[4][4] = {};
+  hipGraphExec_t graph[8] = {};
+  const void *graph_key[8][4] = {};
   // timing
   bool timing = false;
   mcr_timing last_timing = {};
@@ -130,11 +132,17 @@ struct mcr_scene {
   int2 *longleaf = nullptr;
   float *normals = nullptr;
   float *colors = nullptr;
+  float *tangents = nullptr;
+  float *uvs = nullptr;
+  void *texels = nullptr;              // every texture's texels, one allocation
+  DevTexture *textures = nullptr;
+  DevEnvironment *environments = nullptr;
   DevMaterial *materials = nullptr;
   DevLight *lights = nullptr;
   uint64_t bytes_geometry = 0, bytes_shading = 0;
   // precision form (wide.hip)
   bool has_wide = false;
+  bool shadow_tex = false;         // a material with map or alphaMap: shadow rays sample textures
   uint4 *wide_nodes = nullptr;
   float4 *wide_tri = nullptr;
   WideScene wide = {};

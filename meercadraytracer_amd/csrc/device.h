@@ -41,6 +41,29 @@ struct DevMaterial {
   double attenuation[3];
   double shininess;
   double specular[3];
+  // maps and environments (textures.ts, tracer.ts:151-262, 336-406); -1 = none
+  int map, alpha_map, emissive_map, normal_map;
+  int ao_map, roughness_map, metalness_map, env_map;
+  int combine, textured;  // textured: one of the maps above is set (uvs are read)
+  double normal_scale[2];
+  double ao_map_intensity, env_map_intensity, reflectivity;
+};
+
+// CompiledTexture (types.ts:8-21) in HBM: texels are uint8 RGBA (is_float 0) or float RGBA.
+struct DevTexture {
+  const void *data;
+  int32_t width, height;
+  int32_t is_float, srgb, flip_y, wrap_s, wrap_t, pad;
+  double matrix[9];
+};
+
+// CompiledEnvironment (types.ts:23-39).
+struct DevEnvironment {
+  int32_t kind, texture;
+  int32_t faces[6];
+  int32_t flip_x, pad;
+  double rotation[9];
+  double intensity;
 };
 
 struct DevLight {
@@ -57,6 +80,14 @@ struct DevScene {
   const float *colors;      // 12 per triangle or null
   const DevMaterial *materials;
   const DevLight *lights;
+  const float *tangents;    // 4 per triangle (normal maps)
+  const float *uvs;         // 6 per triangle
+  const DevTexture *textures;
+  const DevEnvironment *environments;
+  uint32_t texture_count, environment_count;
+  int32_t environment;      // scene.environment, or -1
+  int32_t background_texture, background_environment, pad_bg;
+  double background_intensity;
   uint32_t triangle_count, inner_count, material_count, light_count;
   int32_t root_ref;         // child ref of node 0, or -1 for an empty BVH
   float root_box[6];
@@ -336,7 +367,7 @@ MCR_INLINE void intersect_any(const DevScene &s, double ox, double oy, double oz
 // before it. So this traversal takes the nearer child first and stops at the first opaque hit;
 // see-through hits are handed to on_hit with their leaf's first triangle, so the caller can replay
 // them in the reference's order (leaves by descending first triangle, then ascending index).
-// on_hit(k, leaf_first, t, front) returns true for an opaque hit.
+// on_hit(k, leaf_first, t, u, v, front, record) returns true for an opaque hit.
 template <bool EXACT, class OnHit>
 MCR_INLINE bool occluded_impl(const DevScene &s, double ox, double oy, double oz, double dx,
                                      double dy, double dz, double ix, double iy, double iz,
@@ -358,7 +389,7 @@ MCR_INLINE bool occluded_impl(const DevScene &s, double ox, double oy, double oz
         float4 a = r[0], b = r[1], c = r[2];
         double t, u, v, det;
         if (!mt_test(a, b, c, ox, oy, oz, dx, dy, dz, 2, t, u, v, det)) continue;
-        if (t > tmin && t < tmax && on_hit(k, first, t, det > 0, c)) return true;
+        if (t > tmin && t < tmax && on_hit(k, first, t, u, v, det > 0, c)) return true;
       }
     } else {
       cnt.nodes++;

@@ -89,6 +89,7 @@ int engine_init(Engine &e, int device, hipStream_t stream) {
   HIP_TRY(hipMalloc(&e.spill, sizeof(int32_t) * (size_t)(MCR_STACK_MAX - MCR_STACK_LDS) * TRAVERSE_THREADS));
   HIP_TRY(hipMalloc(&e.seen, sizeof(double) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
   HIP_TRY(hipMalloc(&e.seen_meta, sizeof(int2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
+  HIP_TRY(hipMalloc(&e.seen_uv, sizeof(double2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
   HIP_TRY(hipMalloc(&e.counters, sizeof(DevCounters)));
   HIP_TRY(hipMalloc(&e.ctl, sizeof(Control)));
   HIP_TRY(hipHostMalloc(&e.ctl_host, sizeof(Control), hipHostMallocDefault));
@@ -106,6 +107,7 @@ void engine_free(Engine &e) {
   hipFree(e.spill);
   hipFree(e.seen);
   hipFree(e.seen_meta);
+  hipFree(e.seen_uv);
   hipFree(e.counters);
   hipFree(e.views);
   hipFree(e.jobs);
@@ -185,8 +187,8 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
   if (timed) HIP_TRY(hipEventRecord(e.ev[2], sm));
   uint64_t max_shadow = (uint64_t)live * max_lights;
   if (max_shadow)
-    launch_shadow(sm, e.mix, e.wide, e.views, w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(max_shadow)), e.spill,
-                  TRAVERSE_THREADS, e.seen, e.seen_meta, e.counters);
+    launch_shadow(sm, e.mix, e.wide, e.tex, e.views, w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(max_shadow)), e.spill,
+                  TRAVERSE_THREADS, e.seen, e.seen_meta, e.seen_uv, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[3], sm));
   launch_resolve(sm, e.mix, e.views, w, q, live);
   HIP_TRY(hipGetLastError());
@@ -205,7 +207,7 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
 }
 
 int engine_iteration_pair(Engine &e, uint32_t max_lights) {
-  int v = (e.mix ? 1 : 0) + (e.wide ? 2 : 0);
+  int v = (e.mix ? 1 : 0) + (e.wide ? 2 : 0) + (e.tex ? 4 : 0);
   const void *key[4] = {e.w_block, e.views, e.counters,
                         (const void *)(uintptr_t)((uint64_t)e.capacity * 64u + max_lights)};
   hipGraphExec_t &graph = e.graph[v];
@@ -228,8 +230,8 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
                       TRAVERSE_THREADS, e.counters);
         launch_shade(cs, e.mix, e.views, e.w, q, cap, e.counters);
         if (max_lights)
-          launch_shadow(cs, e.mix, e.wide, e.views, e.w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of((uint64_t)cap * max_lights)),
-                        e.spill, TRAVERSE_THREADS, e.seen, e.seen_meta, e.counters);
+          launch_shadow(cs, e.mix, e.wide, e.tex, e.views, e.w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of((uint64_t)cap * max_lights)),
+                        e.spill, TRAVERSE_THREADS, e.seen, e.seen_meta, e.seen_uv, e.counters);
         launch_resolve(cs, e.mix, e.views, e.w, q, cap);
       }
       err = hipStreamEndCapture(cs, &g);
@@ -290,7 +292,11 @@ int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<J
   uint32_t nviews = (uint32_t)views.size(), njobs = (uint32_t)jobs.size();
   MCR_TRY(engine_reserve(e, total, nviews, njobs, false));
   uint32_t max_lights = 0;
-  for (const ViewDev &v : views) max_lights = std::max(max_lights, v.light_slots);
+  e.tex = false;
+  for (const ViewDev &v : views) {
+    max_lights = std::max(max_lights, v.shadow_slots);
+    e.tex |= v.shadow_tex != 0;
+  }
   e.mix = nviews > 1;
   hipStream_t sm = e.stream;
   MCR_TRY(engine_set_control(e, out, njobs, total, false));

@@ -9,6 +9,7 @@
 #include "wavefront.h"
 
 hipError_t mcr_upload_dfg(const float *lut512);
+hipError_t mcr_upload_srgb8(const float *table256);
 void launch_reset_slots(hipStream_t s, const Wavefront &w);
 // mix: the run holds samples of more than one view (see with_view in trace.hip).
 void launch_refill(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
@@ -20,8 +21,8 @@ void launch_extend(hipStream_t s, bool mix, bool wide, const ViewDev *views, con
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
 void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                   uint32_t max_live, DevCounters *cnt);
-void launch_shadow(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w, uint32_t grid,
-                   int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
+void launch_shadow(hipStream_t s, bool mix, bool wide, bool tex, const ViewDev *views, const Wavefront &w, uint32_t grid,
+                   int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta, double2 *seen_uv,
                    DevCounters *cnt);
 void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                     uint32_t max_live);

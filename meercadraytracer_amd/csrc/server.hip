@@ -520,7 +520,11 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
   // the order they finish, which groups paths of similar cost.
   if (k) MCR_TRY(engine_sort_queue(e, s->q, live));
   uint32_t max_lights = 0;
-  for (const ViewDev &v : s->views) max_lights = std::max(max_lights, v.light_slots);
+  e.tex = false;
+  for (const ViewDev &v : s->views) {
+    max_lights = std::max(max_lights, v.shadow_slots);
+    e.tex |= v.shadow_tex != 0;
+  }
   lk.unlock();
   auto t_dev = Clock::now();
   int rc = MCR_OK;

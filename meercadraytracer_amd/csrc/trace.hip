@@ -12,9 +12,11 @@
 #include "device.h"
 #include "kernels.h"
 #include "shading.h"
+#include "texture.h"
 #include "wavefront.h"
 
 __constant__ float g_dfg[512];
+__constant__ float g_srgb8[256];
 
 // ---- helpers ----------------------------------------------------------------------------------
 // Exclusive prefix of v over the active lanes of the wave, plus the wave total.
@@ -86,15 +88,13 @@ __device__ inline void count_done(uint32_t *done, int job, bool active) {
 }
 
 // ---- ray generation: tracer.ts:62-80 + 705-713 --------------------------------------------------
-// Starts sample s of job j (its pixel, pass and block) in path slot `slot`. A sample carries its
-// own seed and output position, so which slot runs it, and when, never changes a result.
-MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, const JobDev &j,
-                                    const ViewDev &V) {
-  const mcr_camera &cam = V.cam;
+// The screen position (sx, sy) of sample s of job j: its pixel, block centre or jittered position,
+// and the random stream after the jitter (tracer.ts:66-78).
+MCR_INLINE void sample_screen(const JobDev &j, uint32_t s, uint32_t seed, double &sx, double &sy, uint32_t &rng) {
   uint32_t li = s - j.sample_start;
   uint32_t x = (li % j.cols) * j.block, y = (li / j.cols) * j.block;
   int px = j.tx + (int)x, py = j.ty + (int)y;
-  uint32_t rng = rng_seed((uint32_t)px, (uint32_t)py, j.pass, V.st.seed);
+  rng = rng_seed((uint32_t)px, (uint32_t)py, j.pass, seed);
   double jx = 0.5, jy = 0.5;
   if (j.block > 1) {
     jx = js_min((double)j.block, (double)j.width - px) * 0.5;
@@ -103,7 +103,18 @@ MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, cons
     jx = rng_next(rng);
     jy = rng_next(rng);
   }
-  double sx = (px + jx) / j.width, sy = (py + jy) / j.height;
+  sx = (px + jx) / j.width;
+  sy = (py + jy) / j.height;
+}
+
+// Starts sample s of job j (its pixel, pass and block) in path slot `slot`. A sample carries its
+// own seed and output position, so which slot runs it, and when, never changes a result.
+MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, const JobDev &j,
+                                    const ViewDev &V) {
+  const mcr_camera &cam = V.cam;
+  double sx, sy;
+  uint32_t rng;
+  sample_screen(j, s, V.st.seed, sx, sy, rng);
   double nx = sx * 2 - 1, ny = 1 - sy * 2;
   double a[3], b[3];
   unproject(cam, nx, ny, -1, a);
@@ -131,6 +142,7 @@ MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, cons
   f[F_MG * P + slot] = 0;
   f[F_MB * P + slot] = 0;
   f[F_FOG * P + slot] = 0;
+  f[F_ENVS * P + slot] = 1;
   int32_t *iv = w.i;
   iv[I_RNG * P + slot] = (int32_t)rng;
   iv[I_FLAGS * P + slot] = PF_TONEMAP | PF_CULL;
@@ -138,6 +150,7 @@ MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, cons
   iv[I_PASSES * P + slot] = 0;
   iv[I_SAMPLE * P + slot] = (int32_t)s;
   iv[I_VIEW * P + slot] = (int32_t)j.view;
+  iv[I_ENV * P + slot] = -1;
 }
 
 // Starts sample s in `slot`: finds its job, then runs start_sample under the job's view.
@@ -284,6 +297,44 @@ MCR_INLINE bool light_at(const DevLight &l, double px, double py, double pz, Lig
   return false;
 }
 
+// background (tracer.ts:659-691) for a primary ray that met no surface: colour, screen texture,
+// environment or the clear colour. Screen and environment backgrounds are tone mapped unless their
+// texture holds sRGB-encoded data.
+MCR_INLINE void background(const ViewDev &V, const Control &ctl, uint32_t smp, double dx, double dy, double dz,
+                           double *c) {
+  const DevScene &sc = V.sc;
+  const mcr_settings &st = V.st;
+  if (sc.background_kind == MCR_BACKGROUND_COLOR) {
+    c[0] = sc.background_color[0];
+    c[1] = sc.background_color[1];
+    c[2] = sc.background_color[2];
+    c[3] = 1;
+  } else if (sc.background_kind == MCR_BACKGROUND_SCREEN || sc.background_kind == MCR_BACKGROUND_ENVIRONMENT) {
+    bool srgb;
+    if (sc.background_kind == MCR_BACKGROUND_SCREEN) {
+      const DevTexture &tx = sc.textures[sc.background_texture];
+      double sx, sy, t4[4];
+      uint32_t rng;
+      sample_screen(ctl.jobs[find_job(ctl.jobs, ctl.njobs, smp)], smp, st.seed, sx, sy, rng);
+      tex_sample(tx, sx, 1 - sy, t4);
+      for (int i = 0; i < 3; i++) c[i] = t4[i] * sc.background_intensity;
+      srgb = tx.srgb != 0;
+    } else {
+      const DevEnvironment &env = sc.environments[sc.background_environment];
+      env_sample(env, sc.textures, dx, dy, dz, c);
+      srgb = sc.textures[env.kind == MCR_ENV_CUBE ? env.faces[0] : env.texture].srgb != 0;
+    }
+    if (!srgb && st.tone_mapping != MCR_NO_TONE_MAPPING) tone_map(c, st.tone_mapping, st.tone_mapping_exposure);
+    c[3] = 1;
+  } else {
+    double a = st.clear_alpha;
+    c[0] = st.clear_color[0] * a;
+    c[1] = st.clear_color[1] * a;
+    c[2] = st.clear_color[2] * a;
+    c[3] = a;
+  }
+}
+
 // ---- shade: one path vertex of samplePixel (tracer.ts:113-469) ----------------------------------
 // Writes the vertex's unshadowed light terms and shadow rays into the light slots, the values
 // resolve needs to finish `lr += tr * (outR + er)`, and the continuation ray (the continuation
@@ -309,27 +360,25 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
     double tr = f[F_TR * P + slot], tg = f[F_TG * P + slot], tb = f[F_TB * P + slot];
     int k = iv[I_HTRI * P + slot];
     int mode;
-    for (uint32_t l = 0; l < nl; l++) w.status[(size_t)l * P + slot] = LS_NONE;
+    for (uint32_t l = 0; l < V.shadow_slots; l++) w.status[(size_t)l * P + slot] = LS_NONE;
     if (k < 0) {
       if (!(flags & PF_SURFACE)) {
-        double c[4];  // background (tracer.ts:659-691)
-        if (sc.background_kind == MCR_BACKGROUND_COLOR) {
-          c[0] = sc.background_color[0];
-          c[1] = sc.background_color[1];
-          c[2] = sc.background_color[2];
-          c[3] = 1;
-        } else {
-          double a = st.clear_alpha;
-          c[0] = st.clear_color[0] * a;
-          c[1] = st.clear_color[1] * a;
-          c[2] = st.clear_color[2] * a;
-          c[3] = a;
-        }
+        double c[4];
         uint32_t smp = (uint32_t)iv[I_SAMPLE * P + slot];
+        background(V, ctl, smp, dx, dy, dz, c);
         write_sample(ctl.jobs[find_job(ctl.jobs, ctl.njobs, smp)], ctl.out, smp, c);
         mode = MODE_DONE;
       } else {
-        mode = MODE_FINISH;  // no environment to add (tracer.ts:119-125)
+        int le = iv[I_ENV * P + slot];
+        if (le >= 0) {  // the last surface's environment (tracer.ts:119-124)
+          double env[3];
+          double scale = f[F_ENVS * P + slot];
+          env_sample(sc.environments[le], sc.textures, dx, dy, dz, env);
+          f[F_LR * P + slot] += tr * env[0] * scale;
+          f[F_LG * P + slot] += tg * env[1] * scale;
+          f[F_LB * P + slot] += tb * env[2] * scale;
+        }
+        mode = MODE_FINISH;
       }
     } else {
       double t = f[F_HT * P + slot];
@@ -355,14 +404,32 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
       gx = (gx / gl) * face;
       gy = (gy / gl) * face;
       gz = (gz / gl) * face;
+      // Texture coordinates (tracer.ts:151-153), read only by textured materials.
+      double tu = 0, tv = 0, texel[4];
+      if (mat.textured) {
+        const float *uvs = sc.uvs + (size_t)k * 6;
+        tu = uvs[0] * w0 + uvs[2] * w1 + uvs[4] * w2;
+        tv = uvs[1] * w0 + uvs[3] * w1 + uvs[5] * w2;
+      }
       double cr = mat.color[0], cg = mat.color[1], cb = mat.color[2];
       double alpha = mat.opacity;
+      if (mat.map >= 0) {
+        tex_sample(sc.textures[mat.map], tu, tv, texel);
+        cr *= texel[0];
+        cg *= texel[1];
+        cb *= texel[2];
+        alpha *= texel[3];
+      }
       if (mat.vertex_colors && sc.colors) {
         const float *col = sc.colors + (size_t)k * 12;
         cr *= col[0] * w0 + col[4] * w1 + col[8] * w2;
         cg *= col[1] * w0 + col[5] * w1 + col[9] * w2;
         cb *= col[2] * w0 + col[6] * w1 + col[10] * w2;
         alpha *= col[3] * w0 + col[7] * w1 + col[11] * w2;
+      }
+      if (mat.alpha_map >= 0) {
+        tex_sample(sc.textures[mat.alpha_map], tu, tv, texel);
+        alpha *= texel[1];
       }
       bool cutOut = mat.alpha_test > 0 && alpha < mat.alpha_test;
       if (cutOut || (mat.transparent && rng_next(rng) >= alpha)) {
@@ -397,6 +464,31 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
         nx /= nlen;
         ny /= nlen;
         nz /= nlen;
+        if (mat.normal_map >= 0) {  // tangent-space normal map (tracer.ts:191-209)
+          tex_sample(sc.textures[mat.normal_map], tu, tv, texel);
+          double mx = (texel[0] * 2 - 1) * mat.normal_scale[0];
+          double my = (texel[1] * 2 - 1) * mat.normal_scale[1];
+          double mz = texel[2] * 2 - 1;
+          const float *tg4 = sc.tangents + (size_t)k * 4;
+          double ax = tg4[0], ay = tg4[1], az = tg4[2];
+          double along = ax * nx + ay * ny + az * nz;
+          ax -= nx * along;
+          ay -= ny * along;
+          az -= nz * along;
+          double al = or_one(js_hypot3(ax, ay, az));
+          ax = (ax / al) * face;
+          ay = (ay / al) * face;
+          az = (az / al) * face;
+          double sign = tg4[3];
+          double bx = (ny * az - nz * ay) * sign, by = (nz * ax - nx * az) * sign, bz = (nx * ay - ny * ax) * sign;
+          double qx = ax * mx + bx * my + nx * mz;
+          double qy = ay * mx + by * my + ny * mz;
+          double qz = az * mx + bz * my + nz * mz;
+          nlen = or_one(js_hypot3(qx, qy, qz));
+          nx = qx / nlen;
+          ny = qy / nlen;
+          nz = qz / nlen;
+        }
         double vx = -dx, vy = -dy, vz = -dz;
         double dotNV = js_min(js_max(nx * vx + ny * vy + nz * vz, 0), 1);
         double aR = 0, aG = 0, aB = 0;  // out* before the light sum (basic: the colour)
@@ -408,6 +500,9 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
         double transR = 0, transG = 0, transB = 0;
         double roughness = 1;
         bool castOnly = st.shadows == MCR_SHADOWS_CAST;
+        int environment = -1;  // physical materials: envMap, else the scene's (tracer.ts:333-340)
+        double environmentScale = 1;
+        int combine = 0;
         if (mat.model == MCR_MODEL_BASIC) {
           aR = cr;
           aG = cg;
@@ -427,7 +522,11 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
               ib += L.ground_color[2] + (L.color[2] - L.ground_color[2]) * hw;
             }
           }
-          const double ao = 1;
+          double ao = 1;
+          if (mat.ao_map >= 0) {
+            tex_sample(sc.textures[mat.ao_map], tu, tv, texel);
+            ao = (texel[0] - 1) * mat.ao_map_intensity + 1;
+          }
           bool physicalModel = mat.model == MCR_MODEL_PHYSICAL;
           double metalness = 0, sr = 0.04, sg = 0.04, sb = 0.04, f90 = 1, br = 0, bg = 0, bbl = 0;
           double drr = 0, dgg = 0, dbb = 0, fa = 0, fb = 0, ems = 0, compR = 0, compG = 0, compB = 0;
@@ -435,6 +534,14 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
           if (physicalModel) {
             metalness = mat.metalness;
             roughness = mat.roughness;
+            if (mat.roughness_map >= 0) {
+              tex_sample(sc.textures[mat.roughness_map], tu, tv, texel);
+              roughness *= texel[1];
+            }
+            if (mat.metalness_map >= 0) {
+              tex_sample(sc.textures[mat.metalness_map], tu, tv, texel);
+              metalness *= texel[2];
+            }
             double shadeRoughness = js_min(js_max(roughness, 0.045), 1);
             if (mat.physical) {
               double fr = jm_pow((mat.ior - 1) / (mat.ior + 1), 2);
@@ -560,6 +667,39 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
               transG = transmission * (1 - fg) * cg;
               transB = transmission * (1 - fbl) * cb;
             }
+            if (mat.env_map >= 0) {
+              environment = mat.env_map;
+            } else if (sc.environment >= 0) {
+              environment = sc.environment;
+              environmentScale = mat.env_map_intensity;
+            }
+            // Environment light on the diffuse layer, one cosine-weighted direction traced with
+            // transmittance(..., castOnly false) in the slot after the lights (tracer.ts:341-353).
+            if (environment >= 0 && !st.global_illumination) {
+              double r1 = rng_next(rng), r2 = rng_next(rng);
+              double phi = 2 * MCR_PI * r1;
+              double radius = jm_sqrt(r2);
+              double ex, ey, ez;
+              to_world(nx, ny, nz, radius * jm_cos(phi), radius * jm_sin(phi), jm_sqrt(js_max(0, 1 - r2)), ex, ey, ez);
+              if (ex * gx + ey * gy + ez * gz > 0) {
+                double env[3];
+                env_sample(sc.environments[environment], sc.textures, ex, ey, ez, env);
+                const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
+                size_t ls_idx = (size_t)nl * P + slot;
+                w.contrib[0 * RS + ls_idx] = env[0] * environmentScale * diffR;
+                w.contrib[1 * RS + ls_idx] = env[1] * environmentScale * diffG;
+                w.contrib[2 * RS + ls_idx] = env[2] * environmentScale * diffB;
+                w.sray[0 * RS + ls_idx] = px + gx * eps;
+                w.sray[1 * RS + ls_idx] = py + gy * eps;
+                w.sray[2 * RS + ls_idx] = pz + gz * eps;
+                w.sray[3 * RS + ls_idx] = ex;
+                w.sray[4 * RS + ls_idx] = ey;
+                w.sray[5 * RS + ls_idx] = ez;
+                w.sray[6 * RS + ls_idx] = JM_INF;
+                w.status[ls_idx] = LS_RAY;
+                nshadow++;
+              }
+            }
             if (!st.global_illumination) diffR = diffG = diffB = 0;
           } else {
             iR = ir * MCR_INV_PI * cr * ao;
@@ -573,6 +713,28 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
           }
           has_ind = true;
         }
+        // envMap on lambert, phong, toon and basic materials (tracer.ts:392-406): the reflected
+        // sample and reflectivity, combined with out* in resolve after the light sum.
+        if (mat.model != MCR_MODEL_PHYSICAL && mat.env_map >= 0 && mat.combine >= MCR_COMBINE_MULTIPLY &&
+            mat.combine <= MCR_COMBINE_ADD) {
+          double dn = dx * nx + dy * ny + dz * nz;
+          double e[3];
+          env_sample(sc.environments[mat.env_map], sc.textures, dx - 2 * dn * nx, dy - 2 * dn * ny, dz - 2 * dn * nz,
+                     e);
+          f[F_XR * P + slot] = e[0];
+          f[F_XG * P + slot] = e[1];
+          f[F_XB * P + slot] = e[2];
+          f[F_XS * P + slot] = mat.reflectivity;
+          combine = mat.combine - MCR_COMBINE_MULTIPLY + 1;
+        }
+        flags = (flags & ~PF_COMBINE_MASK) | (combine << PF_COMBINE_SHIFT);
+        double er = mat.emissive[0], eg = mat.emissive[1], eb = mat.emissive[2];
+        if (mat.emissive_map >= 0 && (er > 0 || eg > 0 || eb > 0)) {
+          tex_sample(sc.textures[mat.emissive_map], tu, tv, texel);
+          er *= texel[0];
+          eg *= texel[1];
+          eb *= texel[2];
+        }
         f[F_QR * P + slot] = tr;
         f[F_QG * P + slot] = tg;
         f[F_QB * P + slot] = tb;
@@ -582,9 +744,9 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
         f[F_IR * P + slot] = iR;
         f[F_IG * P + slot] = iG;
         f[F_IB * P + slot] = iB;
-        f[F_ER * P + slot] = mat.emissive[0];
-        f[F_EG * P + slot] = mat.emissive[1];
-        f[F_EB * P + slot] = mat.emissive[2];
+        f[F_ER * P + slot] = er;
+        f[F_EG * P + slot] = eg;
+        f[F_EB * P + slot] = eb;
         flags = has_ind ? (flags | PF_INDIRECT) : (flags & ~PF_INDIRECT);
         // Continuation (tracer.ts:412-469).
         mode = MODE_ADD_FINISH;
@@ -663,6 +825,8 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
               if (sx * gx + sy * gy + sz * gz <= 0) stop = true;
             }
             if (!stop) {
+              iv[I_ENV * P + slot] = environment;  // lastEnvironment (tracer.ts:427-458)
+              f[F_ENVS * P + slot] = environmentScale;
               tr *= wr / p;
               tg *= wg / p;
               tb *= wb / p;
@@ -737,7 +901,7 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
     const Control &ctl = *w.ctl;
     with_view<MIX>(views, view, [&](const ViewDev &V) {
       nshadow = shade_slot(V, w, ctl, slot);
-      nl = V.light_slots;
+      nl = V.shadow_slots;
     });
   }
   // Enqueue the wave's shadow rays with one atomic, light by light inside the wave's block, so a
@@ -763,25 +927,56 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
 }
 
 // ---- shadow: transmittance + attenuateShadow for every queued shadow ray (tracer.ts:529-594) ----
+// attenuateShadow's texture lookups (tracer.ts:555-567): alpha and colour at the hit's uv.
+MCR_INLINE void shadow_texel(const DevScene &sc, const DevMaterial &m, int k, double u, double v, double &alpha,
+                             double &cr, double &cg, double &cb) {
+  const float *uvs = sc.uvs + (size_t)k * 6;
+  double w0 = 1 - u - v;
+  double tu = uvs[0] * w0 + uvs[2] * u + uvs[4] * v;
+  double tv = uvs[1] * w0 + uvs[3] * u + uvs[5] * v;
+  double tx[4];
+  cr = m.color[0];
+  cg = m.color[1];
+  cb = m.color[2];
+  if (m.map >= 0) {
+    tex_sample(sc.textures[m.map], tu, tv, tx);
+    alpha *= tx[3];
+    cr *= tx[0];
+    cg *= tx[1];
+    cb *= tx[2];
+  }
+  if (m.alpha_map >= 0) {
+    tex_sample(sc.textures[m.alpha_map], tu, tv, tx);
+    alpha *= tx[1];
+  }
+}
+
 // Collects the see-through hits of a shadow ray during a near-first traversal that stops at the
 // first opaque, shadow-casting hit (see occluded() in device.h).
+// TEX: materials may carry map or alphaMap; without them the hit's u, v are never kept.
+template <bool TEX>
 struct ShadowCollector {
   const DevScene *sc;
   bool castOnly;
   double *t;        // this thread's records, stride `stride`
   int2 *meta;       // (leaf first triangle, triangle index | front << 31)
+  double2 *uv;      // the hit's barycentrics, kept for textured materials (null without textures)
   size_t stride;
   int n;
   bool overflow;
-  __device__ bool operator()(int k, int leaf_first, double tk, bool front, const float4 &c) {
+  __device__ bool operator()(int k, int leaf_first, double tk, double u, double v, bool front, const float4 &c) {
     int flags = (int)((__float_as_uint(c.y) >> 8) & 0xffu);
     if (castOnly && (flags & MCR_TRI_CAST_SHADOW) == 0) return false;  // not in the way
     if (flags & MCR_TRI_OPAQUE) return true;
     const DevMaterial &m = sc->materials[__float_as_uint(c.z)];
-    if (m.alpha_test > 0 && m.opacity < m.alpha_test) return false;    // cut out
+    bool textured = TEX && (m.map >= 0 || m.alpha_map >= 0);
+    // Cut out (tracer.ts:567). With a texture the coverage is known only after the lookup, which
+    // the replay does: the hit is kept here and dropped there, outside the traversal loop.
+    if (!textured && m.alpha_test > 0 && m.opacity < m.alpha_test) return false;
     if (n < MCR_SHADOW_HIT_CAP) {
       t[(size_t)n * stride] = tk;
       meta[(size_t)n * stride] = make_int2(leaf_first, (int)((uint32_t)k | (front ? 0x80000000u : 0u)));
+      if (textured) uv[(size_t)n * stride] = make_double2(u, v);
       n++;
     } else {
       overflow = true;
@@ -797,9 +992,9 @@ __device__ inline bool visits_before(int2 a, int2 b) {
   return (a.y & 0x7fffffff) < (b.y & 0x7fffffff);
 }
 
-template <bool MIX, bool WIDE, class Stack>
+template <bool MIX, bool WIDE, bool TEX, class Stack>
 MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &w, Stack &st,
-                            double *seen, int2 *seen_meta, DevCounters *cnt) {
+                            double *seen, int2 *seen_meta, double2 *seen_uv, DevCounters *cnt) {
   uint32_t n = w.counts[C_SHADOW];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nthreads = gridDim.x * blockDim.x;
@@ -810,6 +1005,7 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
   bool overflow = false, bad = false;
   double *T = seen + gid;
   int2 *M = seen_meta + gid;
+  double2 *UV = TEX ? seen_uv + gid : nullptr;
   const size_t S = nthreads;
   for (uint32_t e = wave_fetch(&w.counts[C_SHADOW_FETCH]); __shfl(e, 0, 64) < n;
        e = wave_fetch(&w.counts[C_SHADOW_FETCH])) {
@@ -823,13 +1019,15 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     double c[3] = {1, 1, 1};
     with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
     const DevScene &sc = V.sc;
-    ShadowCollector col{&sc, V.st.shadows == MCR_SHADOWS_CAST, T, M, S, 0, false};
+    // The environment-diffuse ray (slot light_slots) is transmittance(..., false) (tracer.ts:346).
+    ShadowCollector<TEX> col{&sc, V.st.shadows == MCR_SHADOWS_CAST && (uint32_t)li != V.light_slots, T, M, UV, S, 0,
+                        false};
     bool blocked;
     if constexpr (WIDE) {
       WideRay wr;
       // Flags and material come from the triangle's FP64-path record.
-      auto on_hit = [&](int k, int leaf, float t, bool front) {
-        return col(k, leaf, (double)t, front, sc.tri[(size_t)k * 3 + 2]);
+      auto on_hit = [&](int k, int leaf, float t, float u, float v, bool front) {
+        return col(k, leaf, (double)t, (double)u, (double)v, front, sc.tri[(size_t)k * 3 + 2]);
       };
       blocked = wide_ray((float)ox, (float)oy, (float)oz, (float)dx, (float)dy, (float)dz, wr) &&
                 wide_occluded(V.wide, wr, 0.0f, (float)maxT, st, tc, bad, on_hit);
@@ -845,14 +1043,17 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
       for (int i = 1; i < col.n; i++) {
         double ti = T[i * S];
         int2 mi = M[i * S];
+        double2 ui = TEX ? UV[i * S] : make_double2(0, 0);
         int j = i - 1;
         while (j >= 0 && visits_before(mi, M[j * S])) {
           T[(j + 1) * S] = T[j * S];
           M[(j + 1) * S] = M[j * S];
+          if (TEX) UV[(j + 1) * S] = UV[j * S];
           j--;
         }
         T[(j + 1) * S] = ti;
         M[(j + 1) * S] = mi;
+        if (TEX) UV[(j + 1) * S] = ui;
       }
       double eps = sc.epsilon;
       for (int i = 0; i < col.n; i++) {  // attenuateShadow for each hit, in order
@@ -868,7 +1069,16 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
           continue;
         }
         const DevMaterial &m = sc.materials[__float_as_uint(sc.tri[(size_t)k * 3 + 2].z)];
-        double alpha = m.transparent ? m.opacity : 1.0;
+        double alpha = m.opacity, mcr = m.color[0], mcg = m.color[1], mcb = m.color[2];
+        if (TEX && (m.map >= 0 || m.alpha_map >= 0)) {
+          double2 hu = UV[i * S];
+          shadow_texel(sc, m, k, hu.x, hu.y, alpha, mcr, mcg, mcb);
+          if (m.alpha_test > 0 && alpha < m.alpha_test) {  // cut out: neither counted nor seen
+            M[i * S] = make_int2(-1, mi.y);
+            continue;
+          }
+        }
+        if (!m.transparent) alpha = 1;
         if (m.thickness > 0) {
           double sgn = front ? -t : t;
           depth[0] = depth[0] + m.attenuation[0] * sgn;
@@ -876,9 +1086,9 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
           depth[2] = depth[2] + m.attenuation[2] * sgn;
         }
         double open = 1 - alpha, through = m.transmission;
-        c[0] = c[0] * (open + alpha * through * m.color[0]);
-        c[1] = c[1] * (open + alpha * through * m.color[1]);
-        c[2] = c[2] * (open + alpha * through * m.color[2]);
+        c[0] = c[0] * (open + alpha * through * mcr);
+        c[1] = c[1] * (open + alpha * through * mcg);
+        c[2] = c[2] * (open + alpha * through * mcb);
         if (c[0] + c[1] + c[2] < 1e-4) {
           c[0] = c[1] = c[2] = 0;
           break;
@@ -900,10 +1110,11 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
   flush_counts(cnt, tc, 0, queries);
 }
 
-template <bool MIX>
+template <bool MIX, bool TEX>
 __global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ views, Wavefront w,
                                                 int32_t *spill, uint32_t spill_stride,
-                                                double *seen, int2 *seen_meta, DevCounters *cnt) {
+                                                double *seen, int2 *seen_meta, double2 *seen_uv,
+                                                DevCounters *cnt) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st;
@@ -911,19 +1122,20 @@ __global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ view
   st.stride = 256;
   st.spill = spill + gid;
   st.spill_stride = (int)spill_stride;
-  shadow_loop<MIX, false>(views, w, st, seen, seen_meta, cnt);
+  shadow_loop<MIX, false, TEX>(views, w, st, seen, seen_meta, seen_uv, cnt);
 }
 
 // Five waves per SIMD: the any-hit loop is latency bound, and at the register count the compiler
 // picks unasked only four fit (4.74 -> 4.44 ms per C1 pass). The closest-hit variant would spill.
-template <bool MIX>
+template <bool MIX, bool TEX>
 __global__ __launch_bounds__(256, 5) void k_shadow_wide(const ViewDev *__restrict__ views, Wavefront w,
                                                         int32_t *spill, uint32_t spill_stride,
-                                                        double *seen, int2 *seen_meta, DevCounters *cnt) {
+                                                        double *seen, int2 *seen_meta, double2 *seen_uv,
+                                                        DevCounters *cnt) {
   __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   WideStack st{lds_base + threadIdx.x, lds_bits + threadIdx.x, spill + gid, (int)spill_stride, 256, 0};
-  shadow_loop<MIX, true>(views, w, st, seen, seen_meta, cnt);
+  shadow_loop<MIX, true, TEX>(views, w, st, seen, seen_meta, seen_uv, cnt);
 }
 
 // ---- resolve: finish the vertex's radiance in the reference's summation order ---------------------
@@ -944,10 +1156,36 @@ MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control
       oG += w.contrib[1 * RS + idx] * w.shadow[1 * RS + idx];
       oB += w.contrib[2 * RS + idx] * w.shadow[2 * RS + idx];
     }
-    if (w.i[I_FLAGS * P + slot] & PF_INDIRECT) {
+    int flags = w.i[I_FLAGS * P + slot];
+    if (flags & PF_INDIRECT) {
       oR += f[F_IR * P + slot];
       oG += f[F_IG * P + slot];
       oB += f[F_IB * P + slot];
+    }
+    if (V.shadow_slots > nl) {  // environment light on the diffuse layer (tracer.ts:341-353)
+      size_t idx = (size_t)nl * P + slot;
+      if (w.status[idx] == LS_VISIBLE) {
+        oR += w.contrib[0 * RS + idx] * w.shadow[0 * RS + idx];
+        oG += w.contrib[1 * RS + idx] * w.shadow[1 * RS + idx];
+        oB += w.contrib[2 * RS + idx] * w.shadow[2 * RS + idx];
+      }
+    }
+    int combine = (flags & PF_COMBINE_MASK) >> PF_COMBINE_SHIFT;
+    if (combine) {  // tracer.ts:392-406
+      double eR = f[F_XR * P + slot], eG = f[F_XG * P + slot], eB = f[F_XB * P + slot], s = f[F_XS * P + slot];
+      if (combine == 1) {
+        oR *= 1 - s + s * eR;
+        oG *= 1 - s + s * eG;
+        oB *= 1 - s + s * eB;
+      } else if (combine == 2) {
+        oR += (eR - oR) * s;
+        oG += (eG - oG) * s;
+        oB += (eB - oB) * s;
+      } else {
+        oR += eR * s;
+        oG += eG * s;
+        oB += eB * s;
+      }
     }
     f[F_LR * P + slot] += f[F_QR * P + slot] * (oR + f[F_ER * P + slot]);
     f[F_LG * P + slot] += f[F_QG * P + slot] * (oG + f[F_EG * P + slot]);
@@ -1191,6 +1429,9 @@ __global__ __launch_bounds__(256) void k_encode_batch(const EncodeItem *items, u
 // ---- launch wrappers ----------------------------------------------------------------------------
 static inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
 
+hipError_t mcr_upload_srgb8(const float *table256) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_srgb8), table256, 256 * sizeof(float));
+}
 hipError_t mcr_upload_dfg(const float *lut512) {
   return hipMemcpyToSymbol(HIP_SYMBOL(g_dfg), lut512, 512 * sizeof(float));
 }
@@ -1221,11 +1462,31 @@ void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront
                   uint32_t max_live, DevCounters *cnt) {
   MCR_LAUNCH_MIX(k_shade, blocks_for(max_live ? max_live : 1), s, views, w, q, cnt);
 }
-void launch_shadow(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w, uint32_t grid,
-                   int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
-                   DevCounters *cnt) {
-  if (wide) MCR_LAUNCH_MIX(k_shadow_wide, grid, s, views, w, spill, spill_stride, seen, seen_meta, cnt);
-  else MCR_LAUNCH_MIX(k_shadow, grid, s, views, w, spill, spill_stride, seen, seen_meta, cnt);
+template <bool TEX>
+static void launch_shadow_tex(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w,
+                              uint32_t grid, int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
+                              double2 *seen_uv, DevCounters *cnt) {
+  if (wide) {
+    if (mix)
+      hipLaunchKernelGGL((k_shadow_wide<true, TEX>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride, seen,
+                         seen_meta, seen_uv, cnt);
+    else
+      hipLaunchKernelGGL((k_shadow_wide<false, TEX>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride,
+                         seen, seen_meta, seen_uv, cnt);
+  } else {
+    if (mix)
+      hipLaunchKernelGGL((k_shadow<true, TEX>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride, seen,
+                         seen_meta, seen_uv, cnt);
+    else
+      hipLaunchKernelGGL((k_shadow<false, TEX>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride, seen,
+                         seen_meta, seen_uv, cnt);
+  }
+}
+void launch_shadow(hipStream_t s, bool mix, bool wide, bool tex, const ViewDev *views, const Wavefront &w,
+                   uint32_t grid, int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta,
+                   double2 *seen_uv, DevCounters *cnt) {
+  if (tex) launch_shadow_tex<true>(s, mix, wide, views, w, grid, spill, spill_stride, seen, seen_meta, seen_uv, cnt);
+  else launch_shadow_tex<false>(s, mix, wide, views, w, grid, spill, spill_stride, seen, seen_meta, seen_uv, cnt);
 }
 void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                     uint32_t max_live) {

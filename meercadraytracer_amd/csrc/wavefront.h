@@ -25,14 +25,19 @@ enum PathField {
   F_ER, F_EG, F_EB,                     // emission
   F_FOG,                                // fog factor of the first surface
   F_HT, F_HU, F_HV,                     // hit t, u, v
+  F_ENVS,                               // lastEnvironmentScale (tracer.ts:427-458)
+  F_XR, F_XG, F_XB, F_XS,               // envMap sample and reflectivity of a non-physical vertex
   F_COUNT
 };
 
 // Integer fields. I_VIEW is the view (scene, camera, settings) of the slot's sample.
-enum PathIntField { I_RNG, I_FLAGS, I_BOUNCE, I_PASSES, I_SAMPLE, I_HTRI, I_VIEW, I_COUNT };
+// I_ENV is lastEnvironment (tracer.ts:427-458), -1 for none.
+enum PathIntField { I_RNG, I_FLAGS, I_BOUNCE, I_PASSES, I_SAMPLE, I_HTRI, I_VIEW, I_ENV, I_COUNT };
 
 enum PathFlags {
   PF_SURFACE = 1, PF_TONEMAP = 2, PF_CULL = 4, PF_FRONT = 8, PF_INDIRECT = 16,
+  PF_COMBINE_SHIFT = 5,  // 2 bits: 0 none, else the envMap combine operation + 1 (tracer.ts:393-406)
+  PF_COMBINE_MASK = 3 << 5,
 };
 
 // What resolve does with a slot after shade.
@@ -84,7 +89,9 @@ struct ViewDev {
   mcr_camera cam;
   mcr_settings st;
   uint32_t light_slots;      // min(light count, MCR_MAX_LIGHTS)
-  uint32_t pad[3];
+  uint32_t shadow_slots;     // light_slots, plus the environment-diffuse slot when the scene has environments
+  uint32_t shadow_tex;       // a material has map or alphaMap, which shadow rays sample
+  uint32_t pad;
 };
 
 struct Wavefront {

@@ -328,7 +328,7 @@ WHD bool wide_closest(const WideScene &ws, const WideRay &r, float tmin, float t
 }
 
 // Shadow-ray query over the wide BVH (tracer.ts:529-594 over bvh.ts:289-332): every triangle hit
-// in (tmin, tmax), no culling, nearer slots first; on_hit(k, leaf, t, front) returns true for an
+// in (tmin, tmax), no culling, nearer slots first; on_hit(k, leaf, t, u, v, front) returns true for an
 // opaque hit, which ends the query (the light is blocked whatever else lies on the segment).
 template <class Stack, class OnHit>
 WHD bool wide_occluded(const WideScene &ws, const WideRay &r, float tmin, float tmax, Stack &st,
@@ -365,7 +365,7 @@ WHD bool wide_occluded(const WideScene &ws, const WideRay &r, float tmin, float 
         bool front;
         if (!wide_tri_test(a, b4, c, r, 2, t, u, v, front)) continue;
         if (t > tmin && t < tmax &&
-            on_hit((int)__float_as_uint(a.w), (int)__float_as_uint(c.w), t, front))
+            on_hit((int)__float_as_uint(a.w), (int)__float_as_uint(c.w), t, u, v, front))
           return true;
       }
     }

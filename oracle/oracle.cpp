@@ -8,10 +8,16 @@
 //   buildBvh, intersectClosest/Any, Random, hash, makeTiles, writeTile: exact restatements using
 //     only correctly rounded IEEE operations; pinned by tests/test_oracle_bvh.py (the reference's
 //     bvh.test.ts invariants, brute force, culling, flat box, coincident and empty cases, plus a
-//     sequential-partition brute force and hand-derived KATs) and tests/test_oracle_misc.py.
-//   brdf.ts / color.ts: pinned by color.test.ts known answers and closed forms.
+//     sequential-partition brute force and hand-derived KATs) and tests/test_progressive.py.
+//   brdf.ts / color.ts: pinned by color.test.ts known answers and closed forms
+//     (tests/test_oracle_color.py).
 //   Tracer.renderTile/samplePixel: pinned by render.test.ts closed forms restated in
-//     tests/test_oracle_render.py. Transcendentals differ from V8's by rounding only.
+//     tests/test_render_closed_forms.py. Transcendentals differ from V8's by rounding only.
+//   textures.ts sampleTexture/sampleEnvironment and the texture branches of samplePixel: pinned
+//     by hand-derived texel, wrap, sRGB and environment-lookup answers and the reference's texture
+//     render tests (tests/test_render_textures.py).
+//   The 8-wide FP32 brute force (orc_wide_brute): pinned against exact FP64 intersection
+//     (tests/test_oracle_wide.py).
 #include "oracle.h"
 
 #include <math.h>
@@ -707,8 +713,8 @@ void cameraRay(const mcr_camera &camera, double x, double y, double *out) {
 const int MAX_PASS_THROUGH = 64;  // tracer.ts:30
 
 // ---- tracer.ts: Tracer ----------------------------------------------------------------------
-// This build traces scenes without textures or environments (checked by supported()); the
-// texture and environment branches of samplePixel are therefore never taken.
+// samplePixel with every branch of the reference: textures (sampleTexture), environments
+// (sampleEnvironment, lastEnvironment), backgrounds; supported() checks the indices first.
 class Tracer {
  public:
   Tracer(const mcr_geometry &g, const mcr_shading &s, const mcr_settings &st, mcr_counters *cnt)

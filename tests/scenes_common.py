@@ -98,3 +98,118 @@ BVH_CASES = {
     'clustered': lambda: (soup(4000, 5, spread=0.01, size=0.5), 4000),
     'flat': lambda: (np.ascontiguousarray(np.where(np.arange(9 * 3000) % 3 == 2, 0.0, soup(3000, 9)).astype(np.float32)), 3000),
 }
+
+
+def _data_texture(T, rng, w, h, kind='u8', srgb=False, wrap=None, repeat=None, lo=0.0, hi=1.0):
+    if kind == 'u8':
+        tex = T.DataTexture(rng.integers(0, 256, (h * w * 4,), dtype=np.uint8), w, h)
+    else:
+        tex = T.DataTexture(rng.uniform(lo, hi, (h * w * 4,)).astype(np.float32), w, h, T.FloatType)
+    if srgb:
+        tex.colorSpace = T.SRGBColorSpace
+    if wrap is not None:
+        tex.wrapS = tex.wrapT = wrap
+    if repeat is not None:
+        tex.repeat = list(repeat)
+        tex.updateMatrix()
+    return tex
+
+
+def textured_scene(background='environment', seed=0):
+    """A scene that takes every texture and environment branch of the tracer (tracer.ts:119-406):
+    a physical floor with colour (sRGB, repeated), normal, roughness, metalness and ao maps; a
+    transparent textured glass sphere (coloured, textured shadows); an alpha-mapped cut-out panel
+    (cut-outs in camera and shadow rays); an emissive-mapped lambert box; lambert, phong and basic
+    boxes with equirect and cube envMaps under the Multiply, Mix and Add combines; a rotated HDR
+    scene environment (the physical materials' reflections, misses after a bounce and the
+    environment-diffuse ray); directional, point and ambient light. background: 'environment'
+    (the scene environment), 'cube', 'screen' (a screen-space texture) or 'color'."""
+    import math
+    from meercadraytracer_amd import abi
+    from meercadraytracer_amd import three as T
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = T.Scene()
+    env = _data_texture(T, rng, 16, 8, 'f32', lo=0.2, hi=2.5)
+    env.mapping = T.EquirectangularReflectionMapping
+    s.environment = env
+    s.environmentIntensity = 0.7
+    s.environmentRotation = (0.0, 0.6, 0.0)
+    faces = [{'data': rng.integers(0, 256, (4 * 4 * 4,), dtype=np.uint8), 'width': 4, 'height': 4} for _ in range(6)]
+    cube = T.CubeTexture(faces)
+    cube.colorSpace = T.SRGBColorSpace
+    if background == 'environment':
+        s.background = env
+        s.backgroundIntensity = 1.3
+    elif background == 'cube':
+        s.background = cube
+    elif background == 'screen':
+        s.background = _data_texture(T, rng, 5, 3, 'u8', srgb=True)
+        s.backgroundIntensity = 0.8
+    else:
+        s.background = T.Color(0.1, 0.2, 0.3)
+
+    floor = T.Mesh(T.PlaneGeometry(10, 10, 3, 3), T.MeshStandardMaterial(
+        color=0xddccbb, roughness=0.7, metalness=0.6,
+        map=_data_texture(T, rng, 8, 8, 'u8', srgb=True, wrap=abi.RepeatWrapping, repeat=(3, 2)),
+        normalMap=_data_texture(T, rng, 4, 4, 'u8', wrap=abi.MirroredRepeatWrapping, repeat=(2, 2)),
+        normalScale=(0.7, -0.5),
+        roughnessMap=_data_texture(T, rng, 4, 4, 'u8'), metalnessMap=_data_texture(T, rng, 4, 4, 'u8'),
+        aoMap=_data_texture(T, rng, 4, 4, 'u8'), aoMapIntensity=0.8))
+    floor.rotation = [-math.pi / 2, 0.0, 0.0]
+    floor.receiveShadow = True
+    floor.castShadow = True
+
+    sphere_geo = T.IcosphereGeometry(1.0, 2)
+    p = sphere_geo.position / np.linalg.norm(sphere_geo.position, axis=1, keepdims=True)
+    sphere_geo.uv = np.stack([np.arctan2(p[:, 2], p[:, 0]) / (2 * math.pi) + 0.5,
+                              np.arcsin(np.clip(p[:, 1], -1, 1)) / math.pi + 0.5], 1).astype(np.float32)
+    glass = T.Mesh(sphere_geo, T.MeshPhysicalMaterial(
+        color=0xffffff, roughness=0.1, transmission=0.8, thickness=1.0, transparent=True, opacity=0.7,
+        map=_data_texture(T, rng, 4, 4, 'f32', wrap=abi.RepeatWrapping), attenuationColor=0xaaffcc,
+        attenuationDistance=2.0))
+    glass.position = [-1.5, 1.0, 0.5]
+    glass.castShadow = True
+    glass.receiveShadow = True
+
+    stripes = np.zeros((2, 4, 4), np.uint8)
+    stripes[:, :, 3] = 255
+    stripes[:, 1::2, 1] = 255
+    panel = T.Mesh(T.BoxGeometry(2.0, 2.0, 0.1), T.MeshStandardMaterial(
+        color=0x88aaff, roughness=0.4, alphaMap=T.DataTexture(stripes.reshape(-1), 4, 2), alphaTest=0.5,
+        side=abi.DoubleSide))
+    panel.position = [1.2, 1.2, -1.0]
+    panel.rotation = [0.0, 0.5, 0.0]
+    panel.castShadow = True
+
+    glow = T.Mesh(T.BoxGeometry(0.6, 0.6, 0.6), T.MeshLambertMaterial(
+        color=0x666666, emissive=0xffaa33, emissiveIntensity=1.5, emissiveMap=_data_texture(T, rng, 2, 2, 'u8')))
+    glow.position = [0.3, 0.3, 1.6]
+    glow.castShadow = True
+
+    eq_env = _data_texture(T, rng, 8, 4, 'f32', lo=0.0, hi=1.5)
+    eq_env.mapping = T.EquirectangularReflectionMapping
+    boxes = []
+    for i, (cls, combine, refl, envmap) in enumerate([(T.MeshLambertMaterial, abi.MultiplyOperation, 0.7, eq_env),
+                                                     (T.MeshPhongMaterial, abi.MixOperation, 0.6, cube),
+                                                     (T.MeshBasicMaterial, abi.AddOperation, 0.3, eq_env)]):
+        b = T.Mesh(T.BoxGeometry(0.7, 0.7, 0.7), cls(color=[0xcc8844, 0x44cc88, 0x8844cc][i], envMap=envmap,
+                                                   combine=combine, reflectivity=refl))
+        b.position = [-2.2 + 1.1 * i, 0.35, 2.4]
+        b.rotation = [0.3 * i, 0.7, 0.0]
+        b.castShadow = True
+        b.receiveShadow = True
+        boxes.append(b)
+
+    sun = T.DirectionalLight(0xfff4e0, 2.0)
+    sun.position = [3.0, 6.0, 2.0]
+    sun.castShadow = True
+    bulb = T.PointLight(0x88aaff, 6.0, 0.0, 2.0)
+    bulb.position = [-1.0, 2.5, 2.5]
+    bulb.castShadow = True
+    s.add(floor, glass, panel, glow, *boxes, sun, bulb, T.AmbientLight(0x202020, 1.0))
+    s.updateMatrixWorld()
+    camera = T.PerspectiveCamera(55, 1.5, 0.1, 100)
+    camera.position = [0.5, 3.5, 7.5]
+    camera.lookAt(0, 0.5, 0)
+    camera.updateMatrixWorld()
+    return s, camera
