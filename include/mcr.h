@@ -21,7 +21,7 @@ extern "C" {
 /* ---- status codes ------------------------------------------------------------------------ */
 #define MCR_OK 0
 #define MCR_E_INVALID (-1)     /* a null pointer, a size out of range, an unknown enum value */
-#define MCR_E_UNSUPPORTED (-2) /* scene data this build does not trace (textures, environments) */
+#define MCR_E_UNSUPPORTED (-2) /* scene data the path cannot trace (an index out of range, too many lights) */
 #define MCR_E_DEVICE (-3)      /* a HIP call failed */
 #define MCR_E_NO_SCENE (-4)    /* protocol.ts:77-79: the tile refers to a scene not received */
 #define MCR_E_OVERFLOW (-5)    /* a fixed device capacity was exceeded */

@@ -5,7 +5,7 @@ Public surface (mirroring the reference's src/index.ts for the trace path):
   Tracer (render_tile), camera rays                 src/tracer.ts (on the GPU through libmcr.so)
   ProgressiveRenderer, SceneSync, make_tiles        src/progressive.ts
   create_tile_handler, executors                    src/protocol.ts, src/executors.ts
-  compile_scene, compile_camera                     src/compile.ts (subset without textures)
+  compile_scene, compile_camera                     src/compile.ts, src/textures.ts (TextureTable)
 The GPU work runs in libmcr.so (include/mcr.h); see meercadraytracer_amd.native.
 """
 from . import abi, three  # noqa: F401

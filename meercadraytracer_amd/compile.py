@@ -1,7 +1,8 @@
 """Scene compilation: the three.js-shaped scene graph to the plain-data format.
 
-Restates src/compile.ts for the subset this build traces (meshes with one material each, no
-textures or environments), vectorised with numpy in the reference's FP64 order of operations:
+Restates src/compile.ts (meshes, materials with their maps, lights, backgrounds, environments,
+fog) and the TextureTable of src/textures.ts, vectorised with numpy in the reference's FP64 order
+of operations:
   collect_objects   compile.ts:64-80
   compile_material  compile.ts:94-147
   compile_light     compile.ts:149-177
@@ -13,6 +14,7 @@ textures or environments), vectorised with numpy in the reference's FP64 order o
 The BVH builder is a parameter: the default is the native GPU builder (mcr_build_bvh).
 """
 import math
+import weakref
 
 import numpy as np
 
@@ -110,6 +112,22 @@ def _read_pixels(image, type_):
     return np.ascontiguousarray(data.reshape(-1))
 
 
+_pixel_cache = {}  # (id(texture), face) -> (weak reference to the texture, version, pixels)
+
+
+def _cached_pixels(texture, face, read):
+    """textures.ts:107-113: one decoded pixel array per texture (or cube face) and version, so a
+    scene recompiled unchanged hands out the same arrays and keeps its texture ids."""
+    key = (id(texture), face)
+    version = getattr(texture, 'version', 0)
+    hit = _pixel_cache.get(key)
+    if hit is not None and hit[0]() is texture and hit[1] == version:
+        return hit[2]
+    pixels = read()
+    _pixel_cache[key] = (weakref.ref(texture, lambda _r, k=key: _pixel_cache.pop(k, None)), version, pixels)
+    return pixels
+
+
 def _euler_matrix_transposed(euler):  # textures.ts:204-217 (XYZ order)
     if euler is None or tuple(euler) == (0, 0, 0):
         return IDENTITY3
@@ -136,7 +154,7 @@ class TextureTable:  # textures.ts:134-202
             return self._index[id(texture)]
         if texture.matrixAutoUpdate is not False:
             texture.updateMatrix()
-        pixels = _read_pixels(texture.image, texture.type)
+        pixels = _cached_pixels(texture, None, lambda: _read_pixels(texture.image, texture.type))
         index = -1
         if pixels is not None:
             index = len(self.textures)
@@ -157,8 +175,8 @@ class TextureTable:  # textures.ts:134-202
         env = None
         if texture.isCubeTexture or texture.mapping in (301, 302):
             faces = []
-            for image in texture.image:
-                pixels = _read_pixels(image, texture.type)
+            for f, image in enumerate(texture.image):
+                pixels = _cached_pixels(texture, f, lambda image=image: _read_pixels(image, texture.type))
                 if pixels is None:
                     break
                 faces.append(len(self.textures))

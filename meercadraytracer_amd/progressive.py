@@ -14,18 +14,20 @@
 Executors take up to `concurrency` tiles at once (render_batch), so a GPU executor traces a whole
 batch of tiles in one wavefront run; results are written in work-list order.
 """
+import dataclasses
 import json
 import math
 import threading
 import time
 import warnings
+import weakref
 
 import numpy as np
 
 from . import native
 from .compile import add_mesh, compile_camera, compile_geometry, compile_shading, js_hypot
 from .executors import LocalExecutor
-from .types import Tile
+from .types import TextureRef, Tile
 
 
 def make_tiles(width, height, size):  # progressive.ts:160-170
@@ -51,6 +53,20 @@ def geometry_key(meshes):  # progressive.ts:53-73
     return '|'.join(parts)
 
 
+_texture_ids = {}  # id(pixel array) -> (weak reference, texture id)
+_next_texture_id = [0]
+
+
+def texture_id(data):  # progressive.ts:42-48: one id per pixel array, for the array's lifetime
+    entry = _texture_ids.get(id(data))
+    if entry is not None and entry[0]() is data:
+        return entry[1]
+    _next_texture_id[0] += 1
+    key = id(data)
+    _texture_ids[key] = (weakref.ref(data, lambda _r, k=key: _texture_ids.pop(k, None)), _next_texture_id[0])
+    return _next_texture_id[0]
+
+
 def _shading_key(shading):
     def enc(o):
         return o.__dict__ if hasattr(o, '__dict__') else str(o)
@@ -66,6 +82,8 @@ class SceneSync:  # progressive.ts:80-157
         self._geometry_key = ''
         self._shading_key = ''
         self._sent = {}
+        self._textures = []          # the current shading's CompiledTextures, with pixels
+        self._texture_id_list = []   # their ids, as the payload's refs name them
         self.build_bvh = build_bvh
 
     def update(self, scene, camera=None):
@@ -76,11 +94,18 @@ class SceneSync:  # progressive.ts:80-157
             self.geometry = compile_geometry(parts, self.build_bvh)
             self._geometry_key = key
             self.geometry_version += 1
-        skey = _shading_key(shading)
+        payload = self._to_payload(shading)
+        skey = _shading_key(payload)
         if skey != self._shading_key:
-            self.shading = shading
+            self.shading = payload
             self._shading_key = skey
+            self._textures = shading.textures
             self.shading_version += 1
+
+    def _to_payload(self, shading):  # progressive.ts:111-124: texture refs in place of the pixels
+        self._texture_id_list = [texture_id(t.data) for t in shading.textures]
+        refs = [TextureRef.of(t, i) for t, i in zip(shading.textures, self._texture_id_list)]
+        return dataclasses.replace(shading, textures=refs)
 
     def sync(self, executor):
         state = self._sent.setdefault(id(executor), {'geometry': 0, 'shading': 0})
@@ -89,10 +114,10 @@ class SceneSync:  # progressive.ts:80-157
             state['geometry'] = self.geometry_version
         if state['shading'] != self.shading_version and self.shading is not None:
             items = []
-            for t in self.shading.textures:
-                if t['id'] not in executor.textures:
-                    items.append({'id': t['id'], 'data': t.get('data')})
-                    executor.textures.add(t['id'])
+            for t, tid in zip(self._textures, self._texture_id_list):
+                if tid not in executor.textures:
+                    items.append({'id': tid, 'data': t.data})
+                    executor.textures.add(tid)
             if items:
                 executor.send({'type': 'textures', 'items': items})
             executor.send({'type': 'shading', 'version': self.shading_version, 'shading': self.shading})

@@ -9,8 +9,12 @@ Messages keep the reference's shapes (protocol.ts:29-49), as dicts:
 Responses: {'type': 'tile', 'job', 'data': float32 RGBA} or {'type': 'error', 'job', 'message'}.
 Geometry and shading are uploaded to the GPU once per version (mcr_scene_set_*), as the reference
 keeps the latest of each (protocol.ts:64-74); a tile naming another version gets the reference's
-error text (protocol.ts:77-79).
+error text (protocol.ts:77-79). The shading payload names its textures by id (TextureRef); their
+pixels arrive in textures messages and are joined to the refs when a tile first needs the shading
+(protocol.ts:81-87), so a missing texture is reported with the tile, in the reference's words.
 """
+import dataclasses
+
 from . import abi, native
 
 NO_SCENE = 'the tile refers to a scene this worker has not received'
@@ -26,7 +30,7 @@ class TileHandler:
         self.shading_version = None
         self.shading = None
         self.pixels = {}
-        self._shading_error = None
+        self._uploaded = None  # shading version on the device
 
     def _gpu(self):
         if self.scene is None:
@@ -46,11 +50,6 @@ class TileHandler:
         if kind == 'shading':
             self.shading = message['shading']
             self.shading_version = message['version']
-            try:
-                self._gpu().set_shading(self.shading, message['version'])
-                self._shading_error = None
-            except native.NativeError as e:  # e.g. textures: reported with the next tile
-                self._shading_error = e.message
             return None
         if kind == 'tile':
             return self.tile(message)
@@ -62,16 +61,26 @@ class TileHandler:
             if (self.geometry_version is None or self.geometry_version != m['geometryVersion'] or
                     self.shading_version is None or self.shading_version != m['shadingVersion']):
                 raise RuntimeError(NO_SCENE)
-            for ref in self.shading.textures:
-                if ref.get('id') not in self.pixels:
-                    raise RuntimeError(f"texture {ref.get('id')} was not sent")
-            if self._shading_error:
-                raise RuntimeError(self._shading_error)
+            self._upload_shading()
             data = self.scene.render_tile(m['camera'], m['settings'], m['tile'], m['width'], m['height'],
                                           m['pass'], m.get('block', 1))
             return {'type': 'tile', 'job': job, 'data': data}
         except (RuntimeError, native.NativeError) as e:
             return {'type': 'error', 'job': job, 'message': getattr(e, 'message', None) or str(e)}
+
+    def _upload_shading(self):
+        """Joins the payload's texture refs to their pixels and uploads the shading, once per
+        version (protocol.ts:81-87)."""
+        if self._uploaded == self.shading_version:
+            return
+        textures = []
+        for ref in self.shading.textures:
+            data = self.pixels.get(ref.id)
+            if data is None:
+                raise RuntimeError(f'texture {ref.id} was not sent')
+            textures.append(ref.with_data(data))
+        self._gpu().set_shading(dataclasses.replace(self.shading, textures=textures), self.shading_version)
+        self._uploaded = self.shading_version
 
     def tiles(self, messages):
         """Several tile messages of one camera and settings in one device run (the batched form
@@ -79,11 +88,16 @@ class TileHandler:
         if not messages:
             return []
         m0 = messages[0]
-        ok = (self.geometry_version is not None and self.shading_version is not None and not self._shading_error and
+        ok = (self.geometry_version is not None and self.shading_version is not None and
               all(m['geometryVersion'] == self.geometry_version and m['shadingVersion'] == self.shading_version and
                   m['camera'] is m0['camera'] and m['settings'] is m0['settings'] for m in messages))
         if not ok:
             return [self.tile(m) for m in messages]
+        try:
+            self._upload_shading()
+        except (RuntimeError, native.NativeError) as e:
+            msg = getattr(e, 'message', None) or str(e)
+            return [{'type': 'error', 'job': m['job'], 'message': msg} for m in messages]
         jobs, off = [], 0
         for m in messages:
             t = m['tile']
@@ -103,6 +117,7 @@ class TileHandler:
         if self.scene is not None:
             self.scene.release()
             self.scene = None
+        self._uploaded = None
 
 
 def create_tile_handler(ctx=None):

@@ -82,6 +82,27 @@ class CompiledTexture:  # types.ts:8-21
 
 
 @dataclass
+class TextureRef:  # protocol.ts:15-16: a CompiledTexture without its pixels, which travel once by id
+    id: int
+    width: int
+    height: int
+    srgb: bool = False
+    flipY: bool = False
+    wrapS: int = abi.ClampToEdgeWrapping
+    wrapT: int = abi.ClampToEdgeWrapping
+    matrix: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+    @classmethod
+    def of(cls, texture, id):
+        return cls(id, texture.width, texture.height, texture.srgb, texture.flipY, texture.wrapS, texture.wrapT,
+                   tuple(texture.matrix))
+
+    def with_data(self, data):  # protocol.ts:82-87
+        return CompiledTexture(self.width, self.height, data, self.srgb, self.flipY, self.wrapS, self.wrapT,
+                               self.matrix)
+
+
+@dataclass
 class CompiledEnvironment:  # types.ts:23-39
     kind: str                 # 'equirect' | 'cube'
     texture: int = -1

@@ -2,6 +2,7 @@
 oracle, an image that encodes with the oracle, and a fake WorkerLike (tests/progressive.test.ts:
 14-36): messages are copied both ways and answered by a handler."""
 import copy
+import dataclasses
 
 import numpy as np
 
@@ -10,20 +11,32 @@ from meercadraytracer_amd.executors import ExecutorError
 
 
 class OracleHandler:
+    """createTileHandler (protocol.ts:55-110) over the oracle: texture refs are joined to the
+    pixels of earlier textures messages when a tile is rendered."""
+
     def __init__(self):
         self.geometry = self.shading = None
         self.gv = self.sv = None
+        self.pixels = {}
 
     def __call__(self, m):
         if m['type'] == 'geometry':
             self.geometry, self.gv = m['geometry'], m['version']
+        elif m['type'] == 'textures':
+            for item in m['items']:
+                self.pixels[item['id']] = item['data']
         elif m['type'] == 'shading':
             self.shading, self.sv = m['shading'], m['version']
         elif m['type'] == 'tile':
             if self.gv != m['geometryVersion'] or self.sv != m['shadingVersion']:
                 return {'type': 'error', 'job': m['job'], 'message': 'the tile refers to a scene this worker has not received'}
+            missing = [r.id for r in self.shading.textures if r.id not in self.pixels]
+            if missing:
+                return {'type': 'error', 'job': m['job'], 'message': f'texture {missing[0]} was not sent'}
             from meercadraytracer_amd.types import CompiledScene
-            data, _ = oracle.render_tile(CompiledScene(self.geometry, self.shading), m['camera'], m['settings'],
+            shading = dataclasses.replace(self.shading, textures=[r.with_data(self.pixels[r.id])
+                                                                  for r in self.shading.textures])
+            data, _ = oracle.render_tile(CompiledScene(self.geometry, shading), m['camera'], m['settings'],
                                          m['tile'], m['width'], m['height'], m['pass'], m['block'])
             return {'type': 'tile', 'job': m['job'], 'data': data}
         return None

@@ -20,8 +20,12 @@ def world():
     b = scenes.compiled('plain_gpu', 2, build_bvh=oracle.build_bvh, width=40, height=40, quads=30, spheres=12,
                         subdivisions=2)
     c = scenes.compiled('plain_cpu', 4, build_bvh=oracle.build_bvh, width=64, height=48)
+    from meercadraytracer_amd.compile import compile_camera, compile_scene
+    from scenes_common import textured_scene
+    tex_scene, tex_cam = textured_scene('screen', seed=3)  # maps, environments, a screen background
+    d = (compile_scene(tex_scene, oracle.build_bvh), compile_camera(tex_cam), {})
     res = []
-    for sc, cam, o in (a, b, c):
+    for sc, cam, o in (a, b, c, d):
         s = native.Scene()
         s.set_geometry(sc.geometry, 1)
         s.set_shading(sc.shading, 1)
@@ -38,6 +42,8 @@ REQUESTS = [
     (2, 50, 30, 2, 64, 1, {'shadows': 'castShadow'}),
     (0, 33, 21, 5, 8, 3, {'globalIllumination': True, 'seed': 2}),
     (1, 24, 40, 1, 16, 4, {}),
+    (3, 48, 32, 3, 16, 4, {}),
+    (3, 30, 20, 2, 8, 1, {'globalIllumination': True, 'maxBounces': 3}),
 ]
 
 

@@ -15,9 +15,10 @@ from meercadraytracer_amd.raytrace import trace_settings
 from progressive_support import FakeWorker, OracleExecutor, OracleHandler, OracleImage
 
 
-def make_scene():  # progressive.test.ts:38-47 (without the texture, outside this build)
+def make_scene():  # progressive.test.ts:38-47
     s = T.Scene()
-    s.add(floor(T.MeshStandardMaterial(roughness=0.6)), overhead_light(2), T.AmbientLight(0xffffff, 1))
+    texture = T.DataTexture(np.array([255, 255, 255, 255, 128, 128, 128, 255], np.uint8), 2, 1)
+    s.add(floor(T.MeshStandardMaterial(map=texture, roughness=0.6)), overhead_light(2), T.AmbientLight(0xffffff, 1))
     box = T.Mesh(T.BoxGeometry(0.5, 0.5, 0.5), T.MeshPhongMaterial(color=0xff8800))
     box.position = [0.0, 0.0, 0.5]
     s.add(box)
@@ -71,14 +72,14 @@ def test_progressive_through_workers_sends_scene_once():  # progressive.test.ts:
     assert r.render(scene, camera, options(), lambda f: image.update(img=f['image'])) is True
     assert np.array_equal(image['img'], oracle_raytrace(scene, camera, 20, 12, 3))
     count = lambda kind: sum(1 for w in workers for m in w.received if m['type'] == kind)
-    assert count('geometry') == 2 and count('shading') == 2
+    assert count('geometry') == 2 and count('textures') == 2 and count('shading') == 2
     camera.position = [0.3, 0.0, 10.0]
     camera.updateMatrixWorld()
     r.render(scene, camera, options(), lambda f: None)
     assert count('geometry') == 2 and count('shading') == 2       # a camera move sends tiles only
     scene.children[0].material.color = T.Color(0x00ff00)
     r.render(scene, camera, options(), lambda f: None)
-    assert count('geometry') == 2 and count('shading') == 4       # a colour change resends shading
+    assert count('geometry') == 2 and count('textures') == 2 and count('shading') == 4  # a colour change resends shading
     scene.children[3].position = [0.2, 0.0, 0.5]
     r.render(scene, camera, options(), lambda f: None)
     assert count('geometry') == 4                                  # moving a mesh resends geometry
@@ -172,4 +173,15 @@ def test_gpu_workers_and_protocol_errors():
            'settings': trace_settings({}), 'tile': make_tiles(20, 12, 8)[0], 'width': 20, 'height': 12, 'pass': 0,
            'block': 1}
     assert h(msg) == {'type': 'error', 'job': 9, 'message': NO_SCENE}
+    # protocol.ts:84: a shading whose texture pixels never arrived fails its tiles, in those words
+    from meercadraytracer_amd.progressive import SceneSync
+    sync = SceneSync(oracle.build_bvh)
+    sync.update(scene, camera)
+    h(dict(type='geometry', version=1, geometry=sync.geometry))
+    h(dict(type='shading', version=1, shading=sync.shading))
+    tid = sync.shading.textures[0].id
+    assert h(msg) == {'type': 'error', 'job': 9, 'message': f'texture {tid} was not sent'}
+    h(dict(type='textures', items=[{'id': tid, 'data': sync._textures[0].data}]))
+    assert h(msg)['type'] == 'tile'
+    h.release()
     r.dispose()
