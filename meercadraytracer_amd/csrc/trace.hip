@@ -19,6 +19,12 @@ __constant__ float g_dfg[512];
 __constant__ float g_srgb8[256];
 
 // ---- helpers ----------------------------------------------------------------------------------
+// The light-slot arrays (unshadowed terms, shadow rays, shadow colours) are written by one kernel
+// and read once by the next: streamed with non-temporal accesses, so they pass the caches by and
+// leave them to the BVH and triangles the traversals re-read.
+__device__ inline void nt_st(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ inline double nt_ld(const double *p) { return __builtin_nontemporal_load(p); }
+
 // Exclusive prefix of v over the active lanes of the wave, plus the wave total.
 __device__ inline uint32_t wave_exclusive(uint32_t v, uint32_t &total) {
   uint32_t lane = threadIdx.x & 63;
@@ -50,6 +56,23 @@ __device__ inline int find_job(const JobDev *jobs, uint32_t njobs, uint32_t s) {
     else hi = mid - 1;
   }
   return lo;
+}
+
+// find_job with a guess: `hint` is returned when the table says it is the job holding sample s
+// (the last job whose first sample is at or before s), else the table is searched. Both checks
+// load at once, so a right guess costs one memory round trip instead of a search.
+__device__ inline int job_of(const JobDev *jobs, uint32_t njobs, uint32_t s, int hint) {
+  if (njobs <= 1) return 0;
+  uint32_t h = min((uint32_t)max(hint, 0), njobs - 1), h1 = min(h + 1, njobs - 1);
+  uint32_t a = jobs[h].sample_start, b = jobs[h1].sample_start;
+  if (hint >= 0 && a <= s && (h1 == h || b > s)) return (int)h;
+  return find_job(jobs, njobs, s);
+}
+
+// The job hint for samples handed out to a wave in one consecutive range: one search for the
+// range's first sample (the first active lane's), wave-uniform, so it runs on the scalar unit.
+__device__ inline int wave_job_hint(const JobDev *jobs, uint32_t njobs, uint32_t s) {
+  return find_job(jobs, njobs, __builtin_amdgcn_readfirstlane(s));
 }
 
 // tracer.ts:715-727
@@ -109,7 +132,7 @@ MCR_INLINE void sample_screen(const JobDev &j, uint32_t s, uint32_t seed, double
 
 // Starts sample s of job j (its pixel, pass and block) in path slot `slot`. A sample carries its
 // own seed and output position, so which slot runs it, and when, never changes a result.
-MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, const JobDev &j,
+MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, const JobDev &j, int job,
                                     const ViewDev &V) {
   const mcr_camera &cam = V.cam;
   double sx, sy;
@@ -151,14 +174,17 @@ MCR_INLINE void start_sample(const Wavefront &w, uint32_t slot, uint32_t s, cons
   iv[I_SAMPLE * P + slot] = (int32_t)s;
   iv[I_VIEW * P + slot] = (int32_t)j.view;
   iv[I_ENV * P + slot] = -1;
+  iv[I_JOB * P + slot] = job;
 }
 
-// Starts sample s in `slot`: finds its job, then runs start_sample under the job's view.
+// Starts sample s in `slot`: finds its job (starting from `hint`), then runs start_sample under
+// the job's view.
 template <bool MIX>
 MCR_INLINE void start_sample_any(const ViewDev *__restrict__ views, const Wavefront &w,
-                                        const Control &ctl, uint32_t slot, uint32_t s) {
-  const JobDev &j = ctl.jobs[find_job(ctl.jobs, ctl.njobs, s)];
-  with_view<MIX>(views, j.view, [&](const ViewDev &V) { start_sample(w, slot, s, j, V); });
+                                        const Control &ctl, uint32_t slot, uint32_t s, int hint) {
+  int job = job_of(ctl.jobs, ctl.njobs, s, hint);
+  const JobDev &j = ctl.jobs[job];
+  with_view<MIX>(views, j.view, [&](const ViewDev &V) { start_sample(w, slot, s, j, job, V); });
 }
 
 // Next work index for every lane of the wave: one atomic per wave, 64 consecutive indices.
@@ -300,8 +326,8 @@ MCR_INLINE bool light_at(const DevLight &l, double px, double py, double pz, Lig
 // background (tracer.ts:659-691) for a primary ray that met no surface: colour, screen texture,
 // environment or the clear colour. Screen and environment backgrounds are tone mapped unless their
 // texture holds sRGB-encoded data.
-MCR_INLINE void background(const ViewDev &V, const Control &ctl, uint32_t smp, double dx, double dy, double dz,
-                           double *c) {
+MCR_INLINE void background(const ViewDev &V, const Control &ctl, uint32_t smp, int job, double dx, double dy,
+                           double dz, double *c) {
   const DevScene &sc = V.sc;
   const mcr_settings &st = V.st;
   if (sc.background_kind == MCR_BACKGROUND_COLOR) {
@@ -315,7 +341,7 @@ MCR_INLINE void background(const ViewDev &V, const Control &ctl, uint32_t smp, d
       const DevTexture &tx = sc.textures[sc.background_texture];
       double sx, sy, t4[4];
       uint32_t rng;
-      sample_screen(ctl.jobs[find_job(ctl.jobs, ctl.njobs, smp)], smp, st.seed, sx, sy, rng);
+      sample_screen(ctl.jobs[job], smp, st.seed, sx, sy, rng);
       tex_sample(tx, sx, 1 - sy, t4);
       for (int i = 0; i < 3; i++) c[i] = t4[i] * sc.background_intensity;
       srgb = tx.srgb != 0;
@@ -365,8 +391,9 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
       if (!(flags & PF_SURFACE)) {
         double c[4];
         uint32_t smp = (uint32_t)iv[I_SAMPLE * P + slot];
-        background(V, ctl, smp, dx, dy, dz, c);
-        write_sample(ctl.jobs[find_job(ctl.jobs, ctl.njobs, smp)], ctl.out, smp, c);
+        int job = job_of(ctl.jobs, ctl.njobs, smp, iv[I_JOB * P + slot]);
+        background(V, ctl, smp, job, dx, dy, dz, c);
+        write_sample(ctl.jobs[job], ctl.out, smp, c);
         mode = MODE_DONE;
       } else {
         int le = iv[I_ENV * P + slot];
@@ -623,24 +650,24 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
             if (r + g + b <= 0) continue;
             size_t ls_idx = (size_t)li * P + slot;
             const size_t RS = (size_t)MCR_MAX_LIGHTS * P;  // one row of light slots
-            w.contrib[0 * RS + ls_idx] = r;
-            w.contrib[1 * RS + ls_idx] = g;
-            w.contrib[2 * RS + ls_idx] = b;
+            nt_st(&w.contrib[0 * RS + ls_idx], r);
+            nt_st(&w.contrib[1 * RS + ls_idx], g);
+            nt_st(&w.contrib[2 * RS + ls_idx], b);
             // visibleLight (tracer.ts:515-526); 'castShadow' in light holds for these three types.
             if (st.shadows == MCR_SHADOWS_NONE || (castOnly && (!L.cast_shadow || !receive))) {
-              w.shadow[0 * RS + ls_idx] = 1;
-              w.shadow[1 * RS + ls_idx] = 1;
-              w.shadow[2 * RS + ls_idx] = 1;
+              nt_st(&w.shadow[0 * RS + ls_idx], 1);
+              nt_st(&w.shadow[1 * RS + ls_idx], 1);
+              nt_st(&w.shadow[2 * RS + ls_idx], 1);
               w.status[ls_idx] = LS_VISIBLE;
             } else {
               double side = gx * ls.lx + gy * ls.ly + gz * ls.lz > 0 ? eps : -eps;
-              w.sray[0 * RS + ls_idx] = px + gx * side;
-              w.sray[1 * RS + ls_idx] = py + gy * side;
-              w.sray[2 * RS + ls_idx] = pz + gz * side;
-              w.sray[3 * RS + ls_idx] = ls.lx;
-              w.sray[4 * RS + ls_idx] = ls.ly;
-              w.sray[5 * RS + ls_idx] = ls.lz;
-              w.sray[6 * RS + ls_idx] = ls.dist == JM_INF ? JM_INF : ls.dist - 2 * eps;
+              nt_st(&w.sray[0 * RS + ls_idx], px + gx * side);
+              nt_st(&w.sray[1 * RS + ls_idx], py + gy * side);
+              nt_st(&w.sray[2 * RS + ls_idx], pz + gz * side);
+              nt_st(&w.sray[3 * RS + ls_idx], ls.lx);
+              nt_st(&w.sray[4 * RS + ls_idx], ls.ly);
+              nt_st(&w.sray[5 * RS + ls_idx], ls.lz);
+              nt_st(&w.sray[6 * RS + ls_idx], ls.dist == JM_INF ? JM_INF : ls.dist - 2 * eps);
               w.status[ls_idx] = LS_RAY;
               nshadow++;
             }
@@ -686,16 +713,16 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
                 env_sample(sc.environments[environment], sc.textures, ex, ey, ez, env);
                 const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
                 size_t ls_idx = (size_t)nl * P + slot;
-                w.contrib[0 * RS + ls_idx] = env[0] * environmentScale * diffR;
-                w.contrib[1 * RS + ls_idx] = env[1] * environmentScale * diffG;
-                w.contrib[2 * RS + ls_idx] = env[2] * environmentScale * diffB;
-                w.sray[0 * RS + ls_idx] = px + gx * eps;
-                w.sray[1 * RS + ls_idx] = py + gy * eps;
-                w.sray[2 * RS + ls_idx] = pz + gz * eps;
-                w.sray[3 * RS + ls_idx] = ex;
-                w.sray[4 * RS + ls_idx] = ey;
-                w.sray[5 * RS + ls_idx] = ez;
-                w.sray[6 * RS + ls_idx] = JM_INF;
+                nt_st(&w.contrib[0 * RS + ls_idx], env[0] * environmentScale * diffR);
+                nt_st(&w.contrib[1 * RS + ls_idx], env[1] * environmentScale * diffG);
+                nt_st(&w.contrib[2 * RS + ls_idx], env[2] * environmentScale * diffB);
+                nt_st(&w.sray[0 * RS + ls_idx], px + gx * eps);
+                nt_st(&w.sray[1 * RS + ls_idx], py + gy * eps);
+                nt_st(&w.sray[2 * RS + ls_idx], pz + gz * eps);
+                nt_st(&w.sray[3 * RS + ls_idx], ex);
+                nt_st(&w.sray[4 * RS + ls_idx], ey);
+                nt_st(&w.sray[5 * RS + ls_idx], ez);
+                nt_st(&w.sray[6 * RS + ls_idx], JM_INF);
                 w.status[ls_idx] = LS_RAY;
                 nshadow++;
               }
@@ -1013,9 +1040,9 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     int32_t ref = w.sref[e];
     int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
     size_t idx = (size_t)li * P + slot;
-    double ox = w.sray[0 * RS + idx], oy = w.sray[1 * RS + idx], oz = w.sray[2 * RS + idx];
-    double dx = w.sray[3 * RS + idx], dy = w.sray[4 * RS + idx], dz = w.sray[5 * RS + idx];
-    double maxT = w.sray[6 * RS + idx];
+    double ox = nt_ld(&w.sray[0 * RS + idx]), oy = nt_ld(&w.sray[1 * RS + idx]), oz = nt_ld(&w.sray[2 * RS + idx]);
+    double dx = nt_ld(&w.sray[3 * RS + idx]), dy = nt_ld(&w.sray[4 * RS + idx]), dz = nt_ld(&w.sray[5 * RS + idx]);
+    double maxT = nt_ld(&w.sray[6 * RS + idx]);
     double c[3] = {1, 1, 1};
     with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
     const DevScene &sc = V.sc;
@@ -1100,9 +1127,9 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     });
     queries++;
     bool visible = c[0] + c[1] + c[2] > 0;
-    w.shadow[0 * RS + idx] = c[0];
-    w.shadow[1 * RS + idx] = c[1];
-    w.shadow[2 * RS + idx] = c[2];
+    nt_st(&w.shadow[0 * RS + idx], c[0]);
+    nt_st(&w.shadow[1 * RS + idx], c[1]);
+    nt_st(&w.shadow[2 * RS + idx], c[2]);
     w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
   }
   if (overflow) atomicOr(&w.counts[C_OVERFLOW], OVERFLOW_SEEN);
@@ -1152,9 +1179,9 @@ MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control
     for (uint32_t li = 0; li < nl; li++) {
       size_t idx = (size_t)li * P + slot;
       if (w.status[idx] != LS_VISIBLE) continue;
-      oR += w.contrib[0 * RS + idx] * w.shadow[0 * RS + idx];
-      oG += w.contrib[1 * RS + idx] * w.shadow[1 * RS + idx];
-      oB += w.contrib[2 * RS + idx] * w.shadow[2 * RS + idx];
+      oR += nt_ld(&w.contrib[0 * RS + idx]) * nt_ld(&w.shadow[0 * RS + idx]);
+      oG += nt_ld(&w.contrib[1 * RS + idx]) * nt_ld(&w.shadow[1 * RS + idx]);
+      oB += nt_ld(&w.contrib[2 * RS + idx]) * nt_ld(&w.shadow[2 * RS + idx]);
     }
     int flags = w.i[I_FLAGS * P + slot];
     if (flags & PF_INDIRECT) {
@@ -1165,9 +1192,9 @@ MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control
     if (V.shadow_slots > nl) {  // environment light on the diffuse layer (tracer.ts:341-353)
       size_t idx = (size_t)nl * P + slot;
       if (w.status[idx] == LS_VISIBLE) {
-        oR += w.contrib[0 * RS + idx] * w.shadow[0 * RS + idx];
-        oG += w.contrib[1 * RS + idx] * w.shadow[1 * RS + idx];
-        oB += w.contrib[2 * RS + idx] * w.shadow[2 * RS + idx];
+        oR += nt_ld(&w.contrib[0 * RS + idx]) * nt_ld(&w.shadow[0 * RS + idx]);
+        oG += nt_ld(&w.contrib[1 * RS + idx]) * nt_ld(&w.shadow[1 * RS + idx]);
+        oB += nt_ld(&w.contrib[2 * RS + idx]) * nt_ld(&w.shadow[2 * RS + idx]);
       }
     }
     int combine = (flags & PF_COMBINE_MASK) >> PF_COMBINE_SHIFT;
@@ -1201,7 +1228,7 @@ MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control
     if (fogf > 0 && sc.fog_kind != MCR_FOG_NONE)
       for (int k = 0; k < 3; k++) c[k] = c[k] + (sc.fog_color[k] - c[k]) * fogf;
     uint32_t smp = (uint32_t)w.i[I_SAMPLE * P + slot];
-    write_sample(ctl.jobs[find_job(ctl.jobs, ctl.njobs, smp)], ctl.out, smp, c);
+    write_sample(ctl.jobs[job_of(ctl.jobs, ctl.njobs, smp, w.i[I_JOB * P + slot])], ctl.out, smp, c);
   }
 }
 
@@ -1221,7 +1248,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
   bool requeue = mode == MODE_EXTEND || mode == MODE_ADD_EXTEND;
   bool finished = live && !requeue;
   if (ctl.job_done) {
-    int job = finished ? find_job(ctl.jobs, ctl.njobs, (uint32_t)w.i[I_SAMPLE * P + slot]) : 0;
+    int job = finished ? job_of(ctl.jobs, ctl.njobs, (uint32_t)w.i[I_SAMPLE * P + slot], w.i[I_JOB * P + slot]) : 0;
     count_done(ctl.job_done, job, finished);
   }
   // A finished slot starts the call's next sample at once, so the slot keeps its place in the
@@ -1231,7 +1258,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
   bool idle = false;
   if (finished) {
     if (next < ctl.total) {
-      start_sample_any<MIX>(views, w, ctl, (uint32_t)slot, next);
+      start_sample_any<MIX>(views, w, ctl, (uint32_t)slot, next, wave_job_hint(ctl.jobs, ctl.njobs, next));
       requeue = true;
     } else {
       idle = true;
@@ -1259,7 +1286,8 @@ __global__ __launch_bounds__(256) void k_refill(const ViewDev *__restrict__ view
   if (i >= count) return;
   int slot = w.free_list[free_top - 1 - i];
   w.queue[q][queued + i] = slot;
-  start_sample_any<MIX>(views, w, *w.ctl, (uint32_t)slot, next + i);
+  start_sample_any<MIX>(views, w, *w.ctl, (uint32_t)slot, next + i,
+                        wave_job_hint(w.ctl->jobs, w.ctl->njobs, next + i));
 }
 
 // ---- queue re-sort: live slots of queue q in ascending slot order ---------------------------------

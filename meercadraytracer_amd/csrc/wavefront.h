@@ -31,8 +31,10 @@ enum PathField {
 };
 
 // Integer fields. I_VIEW is the view (scene, camera, settings) of the slot's sample.
-// I_ENV is lastEnvironment (tracer.ts:427-458), -1 for none.
-enum PathIntField { I_RNG, I_FLAGS, I_BOUNCE, I_PASSES, I_SAMPLE, I_HTRI, I_VIEW, I_ENV, I_COUNT };
+// I_ENV is lastEnvironment (tracer.ts:427-458), -1 for none. I_JOB is the index the sample's job
+// had in the job table when the sample started: a hint, checked against the table before use,
+// since a served run may rewrite the table while samples are in flight.
+enum PathIntField { I_RNG, I_FLAGS, I_BOUNCE, I_PASSES, I_SAMPLE, I_HTRI, I_VIEW, I_ENV, I_JOB, I_COUNT };
 
 enum PathFlags {
   PF_SURFACE = 1, PF_TONEMAP = 2, PF_CULL = 4, PF_FRONT = 8, PF_INDIRECT = 16,
