@@ -197,7 +197,8 @@ def main():
     ap.add_argument('--served-seconds', type=float, default=12.0, help='arrival window per rate')
     ap.add_argument('--served-slots', type=int, default=0, help='server path slots (0 = default)')
     ap.add_argument('--no-precision', action='store_true', help='skip the precision-form probe measurement')
-    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0', 'precision'], default=None,
+    ap.add_argument('--no-textures', action='store_true', help='skip the textured-workload measurement')
+    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0', 'precision', 'textures'], default=None,
                     help='run one workload and print its JSON object only')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB (for scaling HBM counters)')
@@ -231,6 +232,9 @@ def main():
         return
     if args.only == 'c0':
         print(json.dumps(c0(ctx)), flush=True)
+        return
+    if args.only == 'textures':
+        print(json.dumps(workloads.textured_form(ctx, W, H, log=log)), flush=True)
         return
     t0 = time.time()
     scene, camera, opts = scenes.plain_gpu_scene(0, W, H)
@@ -335,6 +339,9 @@ def main():
             print(json.dumps(result['precision']), flush=True)
             return
     gpu.release()
+    if not args.no_textures:
+        log('textured workload ...')
+        result['textures'] = workloads.textured_form(ctx, W, H, plain_mrays=value, log=log)
     if not args.no_batched:
         log('batched form ...')
         result['batched'] = workloads.batched_form(ctx, log=log)

@@ -12,6 +12,11 @@
                           the same material distributions with 20% glass (MeshPhysicalMaterial,
                           transmission 1, ior 1.5, thickness = diameter); a sun and 4 point lights;
                           perspective camera, fov 50. Rendered at 1920x1080, 64 spp, GI on, 8 bounces.
+  textured_gpu_scene(seed) plain_gpu_scene(seed) with textures and an environment: the terrain
+                          gets a repeated 512x512 sRGB colour map, a 256x256 normal map and a
+                          roughness map from fBm noise; non-glass spheres get a 64x32 sRGB map
+                          over spherical uvs; a 256x128 HDR sky is the scene environment and the
+                          background. Rendered like plain_gpu.
 Every value is drawn from numpy's PCG64 with the given seed, so a seed names the scene.
 """
 import math
@@ -119,6 +124,65 @@ def plain_gpu_scene(seed=0, width=1920, height=1080, quads=724, spheres=64, subd
     return scene, camera, options
 
 
+def _noise_rgba(rng, size, lo, hi, srgb):
+    """A seeded RGBA8 texture of fBm noise: each channel a separate field mapped to [lo, hi]."""
+    chans = [fbm_heights(rng, size, size, amplitude=1.0) for _ in range(3)]
+    data = np.empty((size, size, 4), np.uint8)
+    for c, f in enumerate(chans):
+        f = (f - f.min()) / max(f.max() - f.min(), 1e-12)
+        data[..., c] = np.clip(np.round((lo + (hi - lo) * f) * 255), 0, 255).astype(np.uint8)
+    data[..., 3] = 255
+    tex = T.DataTexture(data.reshape(-1), size, size)
+    if srgb:
+        tex.colorSpace = T.SRGBColorSpace
+    return tex
+
+
+def _sky(rng, width=256, height=128):
+    """An HDR equirect sky: a vertical gradient, a warm sun disc and seeded cloud noise."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    U, V = np.meshgrid(u, v)
+    clouds = fbm_heights(rng, height, width, amplitude=1.0)
+    clouds = (clouds - clouds.min()) / max(clouds.max() - clouds.min(), 1e-12)
+    sky = np.stack([0.35 + 0.5 * V, 0.5 + 0.4 * V, 0.9 + 0.3 * V], -1) * (0.6 + 0.4 * clouds[..., None])
+    sun = np.exp(-(((U - 0.3) / 0.02) ** 2 + ((V - 0.75) / 0.03) ** 2)) * 40.0
+    sky = sky + sun[..., None] * np.array([1.0, 0.9, 0.7])
+    data = np.concatenate([sky, np.ones((height, width, 1))], -1).astype(np.float32)
+    tex = T.DataTexture(data.reshape(-1), width, height, T.FloatType)
+    tex.mapping = T.EquirectangularReflectionMapping
+    return tex
+
+
+def textured_gpu_scene(seed=0, width=1920, height=1080, quads=724, spheres=64, subdivisions=4):
+    scene, camera, options = plain_gpu_scene(seed, width, height, quads, spheres, subdivisions)
+    rng = np.random.Generator(np.random.PCG64(seed + 1_000_003))
+    terrain = scene.children[0]
+    colour = _noise_rgba(rng, 512, 0.25, 0.95, True)
+    colour.wrapS = colour.wrapT = abi.RepeatWrapping
+    colour.repeat = [16.0, 16.0]
+    normal = _noise_rgba(rng, 256, 0.3, 0.7, False)
+    normal.wrapS = normal.wrapT = abi.RepeatWrapping
+    normal.repeat = [32.0, 32.0]
+    rough = _noise_rgba(rng, 256, 0.4, 1.0, False)
+    rough.wrapS = rough.wrapT = abi.MirroredRepeatWrapping
+    rough.repeat = [8.0, 8.0]
+    terrain.material.map, terrain.material.normalMap, terrain.material.roughnessMap = colour, normal, rough
+    stripes = _noise_rgba(rng, 64, 0.2, 1.0, True)
+    stripes.wrapS = abi.RepeatWrapping
+    for m in scene.children[1:]:
+        if isinstance(m, T.Mesh) and getattr(m.material, 'transmission', 0) == 0:
+            g = m.geometry
+            p = g.position / np.linalg.norm(g.position, axis=1, keepdims=True)
+            g.uv = np.stack([np.arctan2(p[:, 2], p[:, 0]) / (2 * math.pi) + 0.5,
+                             np.arcsin(np.clip(p[:, 1], -1, 1)) / math.pi + 0.5], 1).astype(np.float32)
+            m.material.map = stripes
+    sky = _sky(rng)
+    scene.environment = sky
+    scene.background = sky
+    return scene, camera, options
+
+
 def sized_gpu_scene(seed, triangles):
     """The plain_gpu generator scaled to about `triangles` triangles, in the same proportions as
     the 1.38M-triangle scene: about 76% heightfield and 24% icospheres (subdivision 4, or 3 below
@@ -142,6 +206,6 @@ def orbit_camera(width, height, angle=0.0, radius=15.0):
 
 def compiled(kind, seed=0, build_bvh=None, **kw):
     """(CompiledScene, CompiledCamera, options) of a named workload."""
-    maker = {'plain_cpu': plain_cpu_scene, 'plain_gpu': plain_gpu_scene}[kind]
+    maker = {'plain_cpu': plain_cpu_scene, 'plain_gpu': plain_gpu_scene, 'textured_gpu': textured_gpu_scene}[kind]
     scene, camera, options = maker(seed, **kw)
     return compile_scene(scene, build_bvh), compile_camera(camera), options

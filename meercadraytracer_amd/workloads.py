@@ -10,6 +10,8 @@
                  log-uniform in [4, 256]; 30% of arrivals are camera moves that cancel an
                  in-flight render of the same scene and submit the moved view. Reports
                  time-to-first-preview and time-to-target-samples percentiles and Mrays/s.
+  textured_form  the 1.38M-triangle workload with maps and an HDR environment: Mrays/s, and the
+                 upload costs of a scene version (geometry, shading with texels, shading only).
 Log-uniform draws are exp(U[ln a, ln b]) rounded to an integer.
 """
 import math
@@ -417,3 +419,54 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
         if own:
             for gpu, _, _ in scenes:
                 gpu.release()
+
+
+def textured_form(ctx, width=1920, height=1080, steps=16, warmup=1, seed=0, plain_mrays=None, log=print):
+    """The plain_gpu workload with textures and an environment (scenes.textured_gpu_scene): Mrays/s
+    over `steps` one-sample passes of the resident scene, and the residency costs of a scene
+    version (textures.ts TextureTable upload with the shading; progressive.ts:80-157): the geometry
+    upload, the first shading upload with its texels, and a shading-only update (one material
+    colour changed, texels unchanged)."""
+    from .raytrace import render_passes
+    from .scenes import textured_gpu_scene
+    scene, camera, opts = textured_gpu_scene(seed, width, height)
+    sc = compile_scene(scene, ctx.build_bvh)
+    cam = compile_camera(camera)
+    st = trace_settings(opts)
+    texels = sum(int(t.data.nbytes) for t in sc.shading.textures)
+    gpu = native.Scene(ctx)
+    t0 = time.perf_counter()
+    gpu.set_geometry(sc.geometry, 1)
+    t1 = time.perf_counter()
+    gpu.set_shading(sc.shading, 1)
+    t2 = time.perf_counter()
+    sc.shading.materials[0].color = (0.5, 0.5, 0.5)
+    t3 = time.perf_counter()
+    gpu.set_shading(sc.shading, 2)
+    t4 = time.perf_counter()
+    npx = width * height
+    out = native.DeviceBuffer(ctx, max(steps, warmup, 1) * npx * 16)
+    if warmup:
+        render_passes(gpu, cam, st, width, height, range(warmup), out.ptr, abi.Counters())
+    cnt = abi.Counters()
+    ts = time.perf_counter()
+    render_passes(gpu, cam, st, width, height, range(warmup, warmup + steps), out.ptr, cnt)
+    elapsed = time.perf_counter() - ts
+    c = cnt.as_dict()
+    rays = c['closest_queries'] + c['any_queries']
+    out.free()
+    gpu.release()
+    mrays = rays / elapsed / 1e6
+    log(f'textured: {steps} passes in {elapsed:.3f} s, {mrays:.1f} Mrays/s')
+    res = {'workload': 'textured_gpu: plain_gpu with a 512x512 sRGB colour map, 256x256 normal and roughness '
+                       'maps on the terrain, 64x32 sRGB maps on the non-glass spheres, a 256x128 HDR sky as '
+                       'environment and background; 1920x1080, GI on, 8 bounces, ACES',
+           'textures': len(sc.shading.textures), 'texel_bytes': texels, 'steps': steps,
+           'ms_per_step': round(elapsed / steps * 1e3, 2), 'mrays_s': round(mrays, 2),
+           'rays_per_step': int(rays / steps),
+           'residency_ms': {'geometry_upload': round((t1 - t0) * 1e3, 2),
+                            'shading_upload_with_texels': round((t2 - t1) * 1e3, 2),
+                            'shading_only_update': round((t4 - t3) * 1e3, 2)}}
+    if plain_mrays:
+        res['vs_plain'] = round(mrays / plain_mrays, 3)
+    return res

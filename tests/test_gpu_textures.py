@@ -122,3 +122,21 @@ def test_textured_render_through_wide_bvh():
     rel = abs(g.mean() - f.mean()) / f.mean()
     print(f'textured wide: {outside:.4%} outside 3 sigma, mean rel {rel:.2e}', flush=True)
     assert outside <= 0.01 and rel <= 0.01
+
+
+def test_textured_workload_bit_exact():
+    """The textured benchmark workload's generator (repeated sRGB colour, normal and roughness maps
+    on a heightfield, mapped spheres, an HDR sky as environment and background, GI on, 8 bounces)
+    at a small size, every float identical to the oracle."""
+    sc, cam, o = scenes.compiled('textured_gpu', 0, build_bvh=oracle.build_bvh, width=96, height=54, quads=60,
+                                 spheres=16, subdivisions=2)
+    st = trace_settings(o)
+    s = _resident(sc)
+    try:
+        for tile, p in [(Tile(0, 0, 96, 54), 0), (Tile(20, 10, 40, 30), 2)]:
+            print(f'textured_gpu tile {tile} pass {p}', flush=True)
+            g = s.render_tile(cam, st, tile, 96, 54, p, 1)
+            ref, _ = oracle.render_tile(sc, cam, st, tile, 96, 54, p, 1)
+            _same(g, ref, f'textured_gpu pass {p}')
+    finally:
+        s.release()
