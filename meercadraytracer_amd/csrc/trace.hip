@@ -25,27 +25,52 @@ __constant__ float g_srgb8[256];
 __device__ inline void nt_st(double *p, double v) { __builtin_nontemporal_store(v, p); }
 __device__ inline double nt_ld(const double *p) { return __builtin_nontemporal_load(p); }
 
-// Exclusive prefix of v over the active lanes of the wave, plus the wave total.
-__device__ inline uint32_t wave_exclusive(uint32_t v, uint32_t &total) {
-  uint32_t lane = threadIdx.x & 63;
-  uint32_t x = v;
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t y = __shfl_up(x, off, 64);
-    if (lane >= (uint32_t)off) x += y;
+
+// Block-wide queue appends. A same-address atomic per wave serialises thousands of waves on the
+// counter's cache line (the resolve pass spent most of its time there); one atomic per block of
+// 256 threads (four waves) per counter cuts that four times. Every thread of the block must call
+// these; `lds` is shared memory of 4 words per counter.
+//
+// block_reserve: the base of this wave's `total` entries (wave-uniform) on each of K counters.
+template <int K>
+__device__ inline void block_reserve(uint32_t *const (&counter)[K], const uint32_t (&total)[K], uint32_t *lds,
+                                     uint32_t (&base)[K]) {
+  uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < K; k++) lds[4 * k + wv] = total[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t got[K];
+    for (int k = 0; k < K; k++) {  // all K atomics in flight before any result is used
+      uint32_t t = lds[4 * k] + lds[4 * k + 1] + lds[4 * k + 2] + lds[4 * k + 3];
+      got[k] = t ? atomicAdd(counter[k], t) : 0u;
+    }
+    for (int k = 0; k < K; k++) {
+      uint32_t a = lds[4 * k], b = lds[4 * k + 1], c = lds[4 * k + 2];
+      lds[4 * k] = got[k];
+      lds[4 * k + 1] = got[k] + a;
+      lds[4 * k + 2] = got[k] + a + b;
+      lds[4 * k + 3] = got[k] + a + b + c;
+    }
   }
-  total = __shfl(x, 63, 64);
-  return x - v;
+  __syncthreads();
+  for (int k = 0; k < K; k++) base[k] = lds[4 * k + wv];
+  __syncthreads();  // the words are reused by the next call
 }
 
-// Appends `want` (0/1/...) entries for this lane to a queue whose length is *counter, with one
-// atomic per wave. Every lane of the wave must call it.
-__device__ inline uint32_t wave_append(uint32_t *counter, uint32_t want) {
-  uint32_t total;
-  uint32_t pre = wave_exclusive(want, total);
-  uint32_t base = 0;
-  if ((threadIdx.x & 63) == 63 && total) base = atomicAdd(counter, total);
-  base = __shfl(base, 63, 64);
-  return base + pre;
+// block_append: one entry for each thread with `want`, on each of K counters; returns positions.
+template <int K>
+__device__ inline void block_append(uint32_t *const (&counter)[K], const bool (&want)[K], uint32_t *lds,
+                                    uint32_t (&pos)[K]) {
+  uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
+  uint64_t m[K];
+  uint32_t total[K];
+  for (int k = 0; k < K; k++) {
+    m[k] = __ballot(want[k]);
+    total[k] = (uint32_t)__popcll(m[k]);
+  }
+  block_reserve<K>(counter, total, lds, pos);
+  for (int k = 0; k < K; k++) pos[k] += (uint32_t)__popcll(m[k] & below);
 }
 
 __device__ inline int find_job(const JobDev *jobs, uint32_t njobs, uint32_t s) {
@@ -906,7 +931,7 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
     w.counts[C_SHADOW_FETCH] = 0;
   }
   if (blockIdx.x * blockDim.x >= n) return;
-  // Every lane reaches the wave_append below, so the bound check guards the work only.
+  // Every thread of the block reaches the queue append below, so the bound check guards the work only.
   bool live = gid < n;
   size_t P = w.capacity;
   int slot = live ? w.queue[q][gid] : 0;
@@ -931,8 +956,8 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
       nl = V.shadow_slots;
     });
   }
-  // Enqueue the wave's shadow rays with one atomic, light by light inside the wave's block, so a
-  // traversal wave mostly holds rays towards one light from neighbouring paths.
+  // Enqueue the wave's shadow rays (one atomic per block), light by light inside the wave's range,
+  // so a traversal wave mostly holds rays towards one light from neighbouring paths.
   uint32_t mask = 0;
   if (live && nshadow)
     for (uint32_t li = 0; li < nl; li++)
@@ -943,9 +968,11 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
     bits[li] = __ballot((mask >> li) & 1u);
     total += (uint32_t)__popcll(bits[li]);
   }
-  uint32_t base = 0;
-  if ((threadIdx.x & 63) == 0 && total) base = atomicAdd(&w.counts[C_SHADOW], total);
-  base = __shfl(base, 0, 64);
+  __shared__ uint32_t lds[4];
+  uint32_t *ctr[1] = {&w.counts[C_SHADOW]};
+  uint32_t tot[1] = {total}, got[1];
+  block_reserve<1>(ctr, tot, lds, got);
+  uint32_t base = got[0];
   uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
   for (int li = 0; li < MCR_MAX_LIGHTS; li++) {
     if ((mask >> li) & 1u) w.sref[base + (uint32_t)__popcll(bits[li] & below)] = slot * MCR_MAX_LIGHTS + li;
@@ -1254,7 +1281,15 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
   // A finished slot starts the call's next sample at once, so the slot keeps its place in the
   // queue and consecutive lanes keep reading consecutive slots. With no sample left it goes to
   // the free list, from which a refill restarts it when the job list grows.
-  uint32_t next = wave_append(&w.counts[C_NEXT], finished ? 1u : 0u);
+  __shared__ uint32_t lds[8];
+  uint32_t next;
+  {
+    uint32_t *c[1] = {&w.counts[C_NEXT]};
+    bool want[1] = {finished};
+    uint32_t pos[1];
+    block_append<1>(c, want, lds, pos);
+    next = pos[0];
+  }
   bool idle = false;
   if (finished) {
     if (next < ctl.total) {
@@ -1264,10 +1299,12 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
       idle = true;
     }
   }
-  uint32_t fpos = wave_append(&w.counts[C_FREE], idle ? 1u : 0u);
-  if (idle) w.free_list[fpos] = slot;
-  uint32_t pos = wave_append(&w.counts[q ^ 1], requeue ? 1u : 0u);
-  if (requeue) w.queue[q ^ 1][pos] = slot;
+  uint32_t *c[2] = {&w.counts[C_FREE], &w.counts[q ^ 1]};
+  bool want[2] = {idle, requeue};
+  uint32_t pos[2];
+  block_append<2>(c, want, lds, pos);
+  if (idle) w.free_list[pos[0]] = slot;
+  if (requeue) w.queue[q ^ 1][pos[1]] = slot;
 }
 
 // Restarts `count` free slots (the top of the free list) with samples next .. next+count-1 and
@@ -1305,9 +1342,13 @@ __global__ __launch_bounds__(256) void k_sort_mark(Wavefront w, int q) {
 __global__ __launch_bounds__(256) void k_sort_gather(Wavefront w, int q) {
   uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = slot < w.capacity && w.mode[slot] == MODE_SORT_MARK;
-  uint32_t pos = wave_append(&w.counts[C_SORT], live ? 1u : 0u);
+  __shared__ uint32_t lds[4];
+  uint32_t *c[1] = {&w.counts[C_SORT]};
+  bool want[1] = {live};
+  uint32_t pos[1];
+  block_append<1>(c, want, lds, pos);
   if (live) {
-    w.queue[q][pos] = (int32_t)slot;
+    w.queue[q][pos[0]] = (int32_t)slot;
     w.mode[slot] = (uint8_t)MODE_DONE;
   }
 }
