@@ -737,7 +737,7 @@ int mcr_intersect_closest(mcr_ctx *c, mcr_scene *s, const double *rays, uint32_t
     HIP_TRY(hipMemcpyAsync(r, rays, (size_t)n * 48, hipMemcpyHostToDevice, sm));
     drays = r;
   }
-  HIP_TRY(hipMemsetAsync(c->eng.counters, 0, sizeof(DevCounters), sm));
+  HIP_TRY(counters_clear(c->eng.counters, sm));
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[0], sm));
   launch_probe_closest(sm, s->d, drays, n, t_min, t_max, cull, dtri, dt, du, dv, dfront,
                        std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->eng.spill, TRAVERSE_THREADS,
@@ -760,7 +760,7 @@ int mcr_intersect_closest(mcr_ctx *c, mcr_scene *s, const double *rays, uint32_t
   if (tmp) hipFree(tmp);
   if (counters) {
     DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, c->eng.counters, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(counters_read(c->eng.counters, hc));
     counters->closest_queries += hc.closest_queries;
     counters->node_visits += hc.node_visits;
     counters->leaf_visits += hc.leaf_visits;
@@ -787,7 +787,7 @@ int mcr_intersect_any_count(mcr_ctx *c, mcr_scene *s, const double *rays, uint32
     HIP_TRY(hipMemcpyAsync(r, rays, (size_t)n * 48, hipMemcpyHostToDevice, sm));
     drays = r;
   }
-  HIP_TRY(hipMemsetAsync(c->eng.counters, 0, sizeof(DevCounters), sm));
+  HIP_TRY(counters_clear(c->eng.counters, sm));
   launch_probe_any_count(sm, s->d, drays, n, t_min, t_max, dcount,
                          std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->eng.spill, TRAVERSE_THREADS,
                          c->eng.counters);
@@ -797,7 +797,7 @@ int mcr_intersect_any_count(mcr_ctx *c, mcr_scene *s, const double *rays, uint32
   if (tmp) hipFree(tmp);
   if (counters) {
     DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, c->eng.counters, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(counters_read(c->eng.counters, hc));
     counters->any_queries += hc.any_queries;
     counters->node_visits += hc.node_visits;
     counters->leaf_visits += hc.leaf_visits;

@@ -110,6 +110,11 @@ int engine_check(Engine &e);
 int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<JobDev> &jobs,
                uint32_t total, float *out, mcr_counters *counters);
 void engine_add_counters(const DevCounters &d, uint64_t paths, mcr_counters *out);
+// The MCR_COUNTER_COPIES device copies of the counters: bytes, clear, and their sum on the host.
+const size_t COUNTER_BYTES = (size_t)MCR_COUNTER_COPIES * MCR_COUNTER_STRIDE;
+hipError_t counters_clear(DevCounters *d, hipStream_t s);
+hipError_t counters_read(const DevCounters *d, DevCounters &sum);
+DevCounters counters_sum(const void *copies);
 
 struct mcr_ctx {
   int device = 0;

@@ -105,6 +105,18 @@ struct DevCounters {  // accumulated per wave, flushed with one atomic per field
   unsigned long long shade_waves, mixed_view_waves;  // shade waves, and those holding several views
 };
 
+// The counters live in MCR_COUNTER_COPIES copies, 128 bytes apart, and block b adds to copy
+// b % MCR_COUNTER_COPIES: thousands of waves flushing at a kernel's end then spread over as many
+// cache lines instead of queueing on one. Readers sum the copies.
+#define MCR_COUNTER_COPIES 64
+#define MCR_COUNTER_STRIDE 128
+static_assert(sizeof(DevCounters) <= MCR_COUNTER_STRIDE, "one counter copy per stride");
+
+__device__ inline DevCounters *counter_copy(DevCounters *base) {
+  return reinterpret_cast<DevCounters *>(reinterpret_cast<char *>(base) +
+                                         (size_t)(blockIdx.x % MCR_COUNTER_COPIES) * MCR_COUNTER_STRIDE);
+}
+
 __device__ inline bool ref_is_leaf(int32_t ref) { return (uint32_t)ref & MCR_LEAF_BIT; }
 
 MCR_INLINE void leaf_range(const DevScene &s, int32_t ref, int &first, int &count) {
@@ -441,6 +453,7 @@ __device__ inline void flush_counts(DevCounters *c, const TravCount &t, unsigned
                                     unsigned long long any) {
   unsigned long long n = wave_sum(t.nodes), l = wave_sum(t.leaves), k = wave_sum(t.tris);
   unsigned long long q = wave_sum(closest), a = wave_sum(any);
+  c = counter_copy(c);
   if ((threadIdx.x & 63) == 0) {
     if (n) atomicAdd(&c->node_visits, n);
     if (l) atomicAdd(&c->leaf_visits, l);

@@ -90,7 +90,7 @@ int engine_init(Engine &e, int device, hipStream_t stream) {
   HIP_TRY(hipMalloc(&e.seen, sizeof(double) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
   HIP_TRY(hipMalloc(&e.seen_meta, sizeof(int2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
   HIP_TRY(hipMalloc(&e.seen_uv, sizeof(double2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
-  HIP_TRY(hipMalloc(&e.counters, sizeof(DevCounters)));
+  HIP_TRY(hipMalloc(&e.counters, COUNTER_BYTES));
   HIP_TRY(hipMalloc(&e.ctl, sizeof(Control)));
   HIP_TRY(hipHostMalloc(&e.ctl_host, sizeof(Control), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc(&e.counts_host, sizeof(uint32_t) * C_WORDS, hipHostMallocDefault));
@@ -271,6 +271,27 @@ int engine_sync(Engine &e) {
   return engine_check(e);
 }
 
+DevCounters counters_sum(const void *copies) {
+  DevCounters s = {};
+  const size_t words = sizeof(DevCounters) / sizeof(unsigned long long);
+  unsigned long long *o = reinterpret_cast<unsigned long long *>(&s);
+  for (int c = 0; c < MCR_COUNTER_COPIES; c++) {
+    const unsigned long long *in =
+        reinterpret_cast<const unsigned long long *>(static_cast<const char *>(copies) + (size_t)c * MCR_COUNTER_STRIDE);
+    for (size_t k = 0; k < words; k++) o[k] += in[k];
+  }
+  return s;
+}
+
+hipError_t counters_clear(DevCounters *d, hipStream_t s) { return hipMemsetAsync(d, 0, COUNTER_BYTES, s); }
+
+hipError_t counters_read(const DevCounters *d, DevCounters &sum) {
+  std::vector<char> buf(COUNTER_BYTES);
+  hipError_t err = hipMemcpy(buf.data(), d, COUNTER_BYTES, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) sum = counters_sum(buf.data());
+  return err;
+}
+
 void engine_add_counters(const DevCounters &hc, uint64_t paths, mcr_counters *counters) {
   counters->closest_queries += hc.closest_queries;
   counters->any_queries += hc.any_queries;
@@ -302,7 +323,7 @@ int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<J
   MCR_TRY(engine_set_control(e, out, njobs, total, false));
   MCR_TRY(engine_upload_views(e, views.data(), nviews));
   MCR_TRY(engine_upload_jobs(e, jobs.data(), 0, njobs));
-  HIP_TRY(hipMemsetAsync(e.counters, 0, sizeof(DevCounters), sm));
+  HIP_TRY(counters_clear(e.counters, sm));
   if (e.timing) HIP_TRY(hipEventRecord(e.ev[6], sm));
   // Path regeneration: resolve starts the call's next sample in every slot whose sample finished,
   // so the wavefront stays full across passes and tiles and only the very end of the call runs
@@ -338,7 +359,7 @@ int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<J
   }
   if (counters) {
     DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, e.counters, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(counters_read(e.counters, hc));
     engine_add_counters(hc, total, counters);
   }
   return MCR_OK;

@@ -161,7 +161,7 @@ struct mcr_server {
   JobDev *jobs_host = nullptr;     // pinned staging of the job list
   uint32_t *done_host = nullptr;   // pinned
   uint32_t host_cap = 0;
-  DevCounters *counters_host = nullptr;  // pinned
+  void *counters_host = nullptr;  // pinned: the device counter copies
   // stats
   uint64_t iterations = 0, paths = 0;
   double device_ms = 0;
@@ -548,12 +548,12 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
   if (!rc) {
     hipError_t err = hipSuccess;
     if (n) err = hipMemcpyAsync(s->done_host, e.job_done, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
-    if (err == hipSuccess) err = hipMemcpyAsync(s->counters_host, e.counters, sizeof(DevCounters), hipMemcpyDeviceToHost, s->stream);
+    if (err == hipSuccess) err = hipMemcpyAsync(s->counters_host, e.counters, COUNTER_BYTES, hipMemcpyDeviceToHost, s->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(s->stream);
     if (err != hipSuccess) rc = mcr_fail(MCR_E_DEVICE, std::string("job counts: ") + hipGetErrorString(err));
   }
   if (!rc) rc = engine_check(e);
-  if (!rc) s->counters = *s->counters_host;
+  if (!rc) s->counters = counters_sum(s->counters_host);
   lk.lock();
   s->device_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_dev).count();
   if (rc) return rc;
@@ -627,14 +627,14 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   s->arena_floats = std::max<uint64_t>((uint64_t)s->cfg.window * 4 * 16, 1ull << 26);
   HIP_TRY(hipMalloc(&s->arena_dev, s->arena_floats * sizeof(float)));
   s->arena.reset(s->arena_floats);
-  HIP_TRY(hipHostMalloc(&s->counters_host, sizeof(DevCounters), hipHostMallocDefault));
-  memset(s->counters_host, 0, sizeof(DevCounters));
+  HIP_TRY(hipHostMalloc(&s->counters_host, COUNTER_BYTES, hipHostMallocDefault));
+  memset(s->counters_host, 0, COUNTER_BYTES);
   s->enc_cap = 1u << 16;
   HIP_TRY(hipHostMalloc(&s->enc_host, s->enc_cap * sizeof(EncodeItem), hipHostMallocDefault));
   HIP_TRY(hipMalloc(&s->enc_dev, s->enc_cap * sizeof(EncodeItem)));
   MCR_TRY(engine_reserve(s->eng, s->cfg.slots, s->cfg.max_requests, 1024, true));
   MCR_TRY(engine_reset(s->eng));
-  HIP_TRY(hipMemsetAsync(s->eng.counters, 0, sizeof(DevCounters), s->stream));
+  HIP_TRY(counters_clear(s->eng.counters, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   memset(s->eng.counts_host, 0, sizeof(uint32_t) * C_WORDS);
   s->eng.counts_host[C_FREE] = s->eng.capacity;

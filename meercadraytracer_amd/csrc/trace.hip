@@ -944,8 +944,9 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
       uint32_t v0 = __shfl(view, leader, 64);
       bool mixed = __ballot(live && view != v0) != 0;
       if ((int)(threadIdx.x & 63) == leader) {
-        atomicAdd(&cnt->shade_waves, 1ull);
-        if (mixed) atomicAdd(&cnt->mixed_view_waves, 1ull);
+        DevCounters *cc = counter_copy(cnt);
+        atomicAdd(&cc->shade_waves, 1ull);
+        if (mixed) atomicAdd(&cc->mixed_view_waves, 1ull);
       }
     }
   }

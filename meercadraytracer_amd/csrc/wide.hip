@@ -403,7 +403,7 @@ int mcr_intersect_closest_wide(mcr_ctx *c, mcr_scene *s, const float *rays, uint
     HIP_TRY(hipMemcpyAsync(r, rays, (size_t)n * 24, hipMemcpyHostToDevice, sm));
     drays = r;
   }
-  HIP_TRY(hipMemsetAsync(c->eng.counters, 0, sizeof(DevCounters), sm));
+  HIP_TRY(counters_clear(c->eng.counters, sm));
   HIP_TRY(hipMemsetAsync(derr, 0, 4, sm));
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[0], sm));
   uint32_t grid = std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256);
@@ -430,7 +430,7 @@ int mcr_intersect_closest_wide(mcr_ctx *c, mcr_scene *s, const float *rays, uint
   }
   if (counters) {
     DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, c->eng.counters, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(counters_read(c->eng.counters, hc));
     counters->closest_queries += hc.closest_queries;
     counters->node_visits += hc.node_visits;
     counters->leaf_visits += hc.leaf_visits;
