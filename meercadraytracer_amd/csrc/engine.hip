@@ -34,7 +34,7 @@ int alloc_wavefront(Engine &e, uint32_t capacity) {
   size_t o_status = take(L);
   size_t o_q0 = take(sizeof(int32_t) * P);
   size_t o_q1 = take(sizeof(int32_t) * P);
-  size_t o_counts = take(sizeof(uint32_t) * C_WORDS);
+  size_t o_counts = take(sizeof(uint32_t) * COUNT_WORDS_ALLOC);
   size_t o_sref = take(sizeof(int32_t) * L);
   size_t o_free = take(sizeof(int32_t) * P);
   HIP_TRY(hipMalloc(&e.w_block, bytes));
@@ -93,9 +93,9 @@ int engine_init(Engine &e, int device, hipStream_t stream) {
   HIP_TRY(hipMalloc(&e.counters, COUNTER_BYTES));
   HIP_TRY(hipMalloc(&e.ctl, sizeof(Control)));
   HIP_TRY(hipHostMalloc(&e.ctl_host, sizeof(Control), hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc(&e.counts_host, sizeof(uint32_t) * C_WORDS, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(&e.counts_host, sizeof(uint32_t) * COUNT_WORDS_ALLOC, hipHostMallocDefault));
   memset(e.ctl_host, 0, sizeof(Control));
-  memset(e.counts_host, 0, sizeof(uint32_t) * C_WORDS);
+  memset(e.counts_host, 0, sizeof(uint32_t) * COUNT_WORDS_ALLOC);
   for (auto &ev : e.ev) HIP_TRY(hipEventCreate(&ev));
   return MCR_OK;
 }
@@ -250,13 +250,13 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
 }
 
 int engine_fetch_counts(Engine &e) {
-  HIP_TRY(hipMemcpyAsync(e.counts_host, e.w.counts, sizeof(uint32_t) * C_WORDS, hipMemcpyDeviceToHost,
+  HIP_TRY(hipMemcpyAsync(e.counts_host, e.w.counts, sizeof(uint32_t) * COUNT_WORDS_ALLOC, hipMemcpyDeviceToHost,
                          e.stream));
   return MCR_OK;
 }
 
 int engine_check(Engine &e) {
-  uint32_t o = e.counts_host[C_OVERFLOW];
+  uint32_t o = e.counts_host[cw(C_OVERFLOW)];
   if (o & OVERFLOW_WIDE)
     return mcr_fail(MCR_E_INTERNAL, "a wide-BVH traversal met a corrupt node or overflowed its stack");
   if (o)
@@ -340,13 +340,13 @@ int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<J
     if (graph) {
       MCR_TRY(engine_iteration_pair(e, max_lights));
       MCR_TRY(engine_sync(e));
-      live = e.counts_host[C_QUEUE0];
+      live = e.counts_host[cw(C_QUEUE0)];
     } else {
       MCR_TRY(engine_iteration(e, q, live, max_lights, &t_closest, &t_shade, &t_any));
       n_closest++;
       if (max_lights) n_any++;
       MCR_TRY(engine_sync(e));
-      live = e.counts_host[q ^ 1];
+      live = e.counts_host[cw(q ^ 1)];
       q ^= 1;
     }
   }

@@ -483,10 +483,10 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
   MCR_TRY(housekeeping(s));
   // With no path live every started sample has finished and no job in the table has started,
   // so sample numbering starts again from 0.
-  uint32_t next = e.counts_host[s->q] == 0 ? 0 : std::min(e.counts_host[C_NEXT], s->total);
-  plan_window(s, next, e.counts_host[C_FREE]);
+  uint32_t next = e.counts_host[cw(s->q)] == 0 ? 0 : std::min(e.counts_host[cw(C_NEXT)], s->total);
+  plan_window(s, next, e.counts_host[cw(C_FREE)]);
   uint32_t n = (uint32_t)s->table.size();
-  if (n == 0 && e.counts_host[s->q] == 0) return deliver(s, progress);
+  if (n == 0 && e.counts_host[cw(s->q)] == 0) return deliver(s, progress);
   progress = true;
   MCR_TRY(engine_reserve(e, e.capacity_limit, (uint32_t)s->views.size(), n, true));
   MCR_TRY(ensure_host(s, std::max<uint32_t>(n, 1)));
@@ -509,7 +509,7 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
     HIP_TRY(hipMemcpyAsync(e.job_done, s->done_host, n * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
   }
   MCR_TRY(engine_set_control(e, s->arena_dev, n, s->total, true));
-  uint32_t free_slots = e.counts_host[C_FREE], live = e.counts_host[s->q];
+  uint32_t free_slots = e.counts_host[cw(C_FREE)], live = e.counts_host[cw(s->q)];
   uint32_t k = std::min<uint32_t>(free_slots, s->total - next);
   MCR_TRY(engine_refill(e, s->q, k, free_slots, next, live));
   live += k;
@@ -567,7 +567,7 @@ void run(mcr_server *s) {
   while (!s->stop) {
     Engine &e = s->eng;
     bool busy = !s->running.empty() || !s->admit_queue.empty() || !s->table.empty() ||
-                e.counts_host[s->q] != 0;
+                e.counts_host[cw(s->q)] != 0;
     bool chores = false;
     for (auto &kv : s->reqs) chores |= kv.second->cancel || kv.second->release;
     if (!busy && !chores) {
@@ -636,8 +636,8 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   MCR_TRY(engine_reset(s->eng));
   HIP_TRY(counters_clear(s->eng.counters, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
-  memset(s->eng.counts_host, 0, sizeof(uint32_t) * C_WORDS);
-  s->eng.counts_host[C_FREE] = s->eng.capacity;
+  memset(s->eng.counts_host, 0, sizeof(uint32_t) * COUNT_WORDS_ALLOC);
+  s->eng.counts_host[cw(C_FREE)] = s->eng.capacity;
   uint64_t fixed = (uint64_t)s->arena_floats * 4;
   s->bytes_used = fixed;
   if (s->bytes_used > s->cfg.hbm_budget) return mcr_fail(MCR_E_OVERFLOW, "the HBM budget does not cover the server's own buffers");

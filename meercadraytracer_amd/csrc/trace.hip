@@ -212,13 +212,33 @@ MCR_INLINE void start_sample_any(const ViewDev *__restrict__ views, const Wavefr
   with_view<MIX>(views, j.view, [&](const ViewDev &V) { start_sample(w, slot, s, j, job, V); });
 }
 
-// Next work index for every lane of the wave: one atomic per wave, 64 consecutive indices.
-__device__ inline uint32_t wave_fetch(uint32_t *counter) {
-  uint32_t base = 0;
-  if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64u);
-  base = __shfl(base, 0, 64);
-  return base + (threadIdx.x & 63);
-}
+// Work distribution of the traversal kernels over a queue of n items in 64-item chunks. One
+// counter would serialise every wave's atomic on one cache line; MCR_FETCH_COUNTERS counters
+// interleave instead: counter k hands out chunks k, k + K, k + 2K, ... (K counters), each wave
+// starts on counter (its wave index mod K) and moves on to the next counter when its own runs out.
+// The counters advance together, so the chunks in flight still sweep the queue front to back
+// (neighbouring paths stay in flight together) while each counter sees a K-th of the atomics.
+// Every lane of the wave must call next(); it returns the chunk's first item, or n when none is left.
+struct WaveFetch {
+  uint32_t *counters;
+  uint32_t n, k, tried;  // wave-uniform
+  __device__ WaveFetch(uint32_t *c, uint32_t items)
+      : counters(c), n(items), k((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % MCR_FETCH_COUNTERS),
+        tried(0) {}
+  __device__ uint32_t next() {
+    uint32_t chunks = (n + 63) / 64;
+    while (tried < MCR_FETCH_COUNTERS) {
+      uint32_t j = 0;
+      if ((threadIdx.x & 63) == 0) j = atomicAdd(&counters[cw(k)], 1u);
+      j = __builtin_amdgcn_readfirstlane(__shfl(j, 0, 64));
+      uint64_t chunk = (uint64_t)j * MCR_FETCH_COUNTERS + k;
+      if (chunk < chunks) return (uint32_t)chunk * 64u;
+      k = k + 1 == MCR_FETCH_COUNTERS ? 0 : k + 1;
+      tried++;
+    }
+    return n;
+  }
+};
 
 // ---- extend: closest hit for every live slot (bvh.ts:202-282) ------------------------------------
 // WIDE: the precision form's FP32 query over the scene's 8-wide BVH (wide.h) instead of the FP64
@@ -226,7 +246,7 @@ __device__ inline uint32_t wave_fetch(uint32_t *counter) {
 template <bool MIX, bool WIDE, class Stack>
 MCR_INLINE void extend_loop(const ViewDev *__restrict__ views, const Wavefront &w, int q, Stack &st,
                             DevCounters *cnt) {
-  uint32_t n = w.counts[q];
+  uint32_t n = w.counts[cw(q)];
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
   bool bad = false;
@@ -234,8 +254,9 @@ MCR_INLINE void extend_loop(const ViewDev *__restrict__ views, const Wavefront &
   const double *f = w.f;
   // Waves take 64 rays at a time from a shared counter, so no wave idles while others still
   // hold long traversals (a static split leaves the slowest waves running alone).
-  for (uint32_t i = wave_fetch(&w.counts[C_EXTEND_FETCH]); __shfl(i, 0, 64) < n;
-       i = wave_fetch(&w.counts[C_EXTEND_FETCH])) {
+  WaveFetch fetch(&w.counts[cw(C_EXTEND_FETCH)], n);
+  for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
+    uint32_t i = b + (threadIdx.x & 63);
     if (i >= n) continue;
     int slot = w.queue[q][i];
     double ox = f[F_OX * P + slot], oy = f[F_OY * P + slot], oz = f[F_OZ * P + slot];
@@ -268,17 +289,17 @@ MCR_INLINE void extend_loop(const ViewDev *__restrict__ views, const Wavefront &
     w.f[F_HV * P + slot] = h.v;
     w.i[I_FLAGS * P + slot] = h.front ? (flags | PF_FRONT) : (flags & ~PF_FRONT);
   }
-  if (bad) atomicOr(&w.counts[C_OVERFLOW], OVERFLOW_WIDE);
+  if (bad) atomicOr(&w.counts[cw(C_OVERFLOW)], OVERFLOW_WIDE);
   flush_counts(cnt, tc, queries, 0);
 }
 
 template <bool MIX>
-__global__ __launch_bounds__(256) void k_extend(const ViewDev *__restrict__ views, Wavefront w, int q,
+__global__ __launch_bounds__(256, MIX ? 4 : 5) void k_extend(const ViewDev *__restrict__ views, Wavefront w, int q,
                                                 int32_t *spill, uint32_t spill_stride,
                                                 DevCounters *cnt) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid == 0) w.counts[C_SHADOW] = 0;  // shade of this iteration appends to it
+  if (gid == 0) w.counts[cw(C_SHADOW)] = 0;  // shade of this iteration appends to it
   LaneStack st;
   st.lds = lds_stack + threadIdx.x;
   st.stride = 256;
@@ -293,7 +314,7 @@ __global__ __launch_bounds__(256) void k_extend_wide(const ViewDev *__restrict__
                                                         DevCounters *cnt) {
   __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid == 0) w.counts[C_SHADOW] = 0;
+  if (gid == 0) w.counts[cw(C_SHADOW)] = 0;
   WideStack st{lds_base + threadIdx.x, lds_bits + threadIdx.x, spill + gid, (int)spill_stride, 256, 0};
   extend_loop<MIX, true>(views, w, q, st, cnt);
 }
@@ -924,11 +945,13 @@ template <bool MIX>
 __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views, Wavefront w, int q,
                                                DevCounters *cnt) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = w.counts[q];
+  uint32_t n = w.counts[cw(q)];
   if (gid == 0) {
-    w.counts[q ^ 1] = 0;  // resolve of this iteration queues into it
-    w.counts[C_EXTEND_FETCH] = 0;
-    w.counts[C_SHADOW_FETCH] = 0;
+    w.counts[cw(q ^ 1)] = 0;  // resolve of this iteration queues into it
+    for (int k = 0; k < MCR_FETCH_COUNTERS; k++) {
+      w.counts[cw(C_EXTEND_FETCH + k)] = 0;
+      w.counts[cw(C_SHADOW_FETCH + k)] = 0;
+    }
   }
   if (blockIdx.x * blockDim.x >= n) return;
   // Every thread of the block reaches the queue append below, so the bound check guards the work only.
@@ -970,7 +993,7 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
     total += (uint32_t)__popcll(bits[li]);
   }
   __shared__ uint32_t lds[4];
-  uint32_t *ctr[1] = {&w.counts[C_SHADOW]};
+  uint32_t *ctr[1] = {&w.counts[cw(C_SHADOW)]};
   uint32_t tot[1] = {total}, got[1];
   block_reserve<1>(ctr, tot, lds, got);
   uint32_t base = got[0];
@@ -1050,7 +1073,7 @@ __device__ inline bool visits_before(int2 a, int2 b) {
 template <bool MIX, bool WIDE, bool TEX, class Stack>
 MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &w, Stack &st,
                             double *seen, int2 *seen_meta, double2 *seen_uv, DevCounters *cnt) {
-  uint32_t n = w.counts[C_SHADOW];
+  uint32_t n = w.counts[cw(C_SHADOW)];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nthreads = gridDim.x * blockDim.x;
   TravCount tc = {0, 0, 0};
@@ -1062,8 +1085,9 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
   int2 *M = seen_meta + gid;
   double2 *UV = TEX ? seen_uv + gid : nullptr;
   const size_t S = nthreads;
-  for (uint32_t e = wave_fetch(&w.counts[C_SHADOW_FETCH]); __shfl(e, 0, 64) < n;
-       e = wave_fetch(&w.counts[C_SHADOW_FETCH])) {
+  WaveFetch fetch(&w.counts[cw(C_SHADOW_FETCH)], n);
+  for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
+    uint32_t e = b + (threadIdx.x & 63);
     if (e >= n) continue;
     int32_t ref = w.sref[e];
     int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
@@ -1160,8 +1184,8 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     nt_st(&w.shadow[2 * RS + idx], c[2]);
     w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
   }
-  if (overflow) atomicOr(&w.counts[C_OVERFLOW], OVERFLOW_SEEN);
-  if (bad) atomicOr(&w.counts[C_OVERFLOW], OVERFLOW_WIDE);
+  if (overflow) atomicOr(&w.counts[cw(C_OVERFLOW)], OVERFLOW_SEEN);
+  if (bad) atomicOr(&w.counts[cw(C_OVERFLOW)], OVERFLOW_WIDE);
   flush_counts(cnt, tc, 0, queries);
 }
 
@@ -1263,7 +1287,7 @@ MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control
 template <bool MIX>
 __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ views, Wavefront w, int q) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = w.counts[q];
+  uint32_t n = w.counts[cw(q)];
   if (blockIdx.x * blockDim.x >= n) return;
   bool live = gid < n;
   size_t P = w.capacity;
@@ -1285,7 +1309,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
   __shared__ uint32_t lds[8];
   uint32_t next;
   {
-    uint32_t *c[1] = {&w.counts[C_NEXT]};
+    uint32_t *c[1] = {&w.counts[cw(C_NEXT)]};
     bool want[1] = {finished};
     uint32_t pos[1];
     block_append<1>(c, want, lds, pos);
@@ -1300,7 +1324,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
       idle = true;
     }
   }
-  uint32_t *c[2] = {&w.counts[C_FREE], &w.counts[q ^ 1]};
+  uint32_t *c[2] = {&w.counts[cw(C_FREE)], &w.counts[cw(q ^ 1)]};
   bool want[2] = {idle, requeue};
   uint32_t pos[2];
   block_append<2>(c, want, lds, pos);
@@ -1317,9 +1341,9 @@ __global__ __launch_bounds__(256) void k_refill(const ViewDev *__restrict__ view
                                                 uint32_t queued) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
-    w.counts[C_FREE] = free_top - count;
-    w.counts[C_NEXT] = next + count;
-    w.counts[q] = queued + count;
+    w.counts[cw(C_FREE)] = free_top - count;
+    w.counts[cw(C_NEXT)] = next + count;
+    w.counts[cw(q)] = queued + count;
   }
   if (i >= count) return;
   int slot = w.free_list[free_top - 1 - i];
@@ -1336,15 +1360,15 @@ __global__ __launch_bounds__(256) void k_refill(const ViewDev *__restrict__ view
 #define MODE_SORT_MARK 0xffu
 __global__ __launch_bounds__(256) void k_sort_mark(Wavefront w, int q) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) w.counts[C_SORT] = 0;
-  if (i < w.counts[q]) w.mode[w.queue[q][i]] = (uint8_t)MODE_SORT_MARK;
+  if (i == 0) w.counts[cw(C_SORT)] = 0;
+  if (i < w.counts[cw(q)]) w.mode[w.queue[q][i]] = (uint8_t)MODE_SORT_MARK;
 }
 
 __global__ __launch_bounds__(256) void k_sort_gather(Wavefront w, int q) {
   uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = slot < w.capacity && w.mode[slot] == MODE_SORT_MARK;
   __shared__ uint32_t lds[4];
-  uint32_t *c[1] = {&w.counts[C_SORT]};
+  uint32_t *c[1] = {&w.counts[cw(C_SORT)]};
   bool want[1] = {live};
   uint32_t pos[1];
   block_append<1>(c, want, lds, pos);
@@ -1358,7 +1382,7 @@ __global__ __launch_bounds__(256) void k_sort_gather(Wavefront w, int q) {
 __global__ __launch_bounds__(256) void k_reset_slots(Wavefront w) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0)
-    for (int k = 0; k < C_WORDS; k++) w.counts[k] = k == C_FREE ? w.capacity : 0;
+    for (int k = 0; k < C_WORDS; k++) w.counts[cw(k)] = k == C_FREE ? w.capacity : 0;
   if (i < w.capacity) w.free_list[i] = (int32_t)(w.capacity - 1 - i);
 }
 

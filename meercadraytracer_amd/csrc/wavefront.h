@@ -11,6 +11,7 @@
 #include "wide.h"
 
 #define MCR_MAX_LIGHTS 16          // punctual lights traced per scene
+#define MCR_FETCH_COUNTERS 8       // interleaved work counters of the traversal kernels
 #define MCR_SHADOW_HIT_CAP 64      // see-through surfaces remembered per shadow ray (tracer.ts:48)
 
 // Double fields of a path slot.
@@ -63,13 +64,20 @@ enum CountWord {
                     // wide traversal met a corrupt node or a full stack
   C_FREE = 4,       // slots on the free list
   C_NEXT = 5,       // next sample to start (may run past Control::total; the host clamps it)
-  C_EXTEND_FETCH = 6,
-  C_SHADOW_FETCH = 7,
-  C_SORT = 8,       // appends of the queue re-sort (k_sort_gather)
-  C_WORDS = 9,
+  C_SORT = 6,       // appends of the queue re-sort (k_sort_gather)
+  C_EXTEND_FETCH = 7,                            // MCR_FETCH_COUNTERS words (WaveFetch)
+  C_SHADOW_FETCH = C_EXTEND_FETCH + MCR_FETCH_COUNTERS,
+  C_WORDS = C_SHADOW_FETCH + MCR_FETCH_COUNTERS,
 };
 
 enum OverflowBits { OVERFLOW_SEEN = 1, OVERFLOW_WIDE = 2 };
+
+// Each counter word sits on a 128-byte cache line of its own: atomics on one line serialise, and
+// the traversal kernels' work counters must not queue behind each other or behind the queue
+// lengths. Wavefront::counts holds C_WORDS such lines; word x is counts[cw(x)].
+#define MCR_COUNT_STRIDE 32
+__host__ __device__ constexpr inline uint32_t cw(uint32_t word) { return word * MCR_COUNT_STRIDE; }
+const uint32_t COUNT_WORDS_ALLOC = C_WORDS * MCR_COUNT_STRIDE;
 
 struct JobDev;
 
