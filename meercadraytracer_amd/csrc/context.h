@@ -38,7 +38,7 @@ hipError_t dalloc(T **p, size_t count) {
 }
 
 // Grid of persistent traversal blocks: enough waves to fill 256 CUs several times over.
-const uint32_t TRAVERSE_GRID = 256 * 8;
+const uint32_t TRAVERSE_GRID = 256 * 5;
 const uint32_t TRAVERSE_THREADS = TRAVERSE_GRID * 256;
 const uint32_t DEFAULT_CAPACITY = 1u << 21;  // path slots
 
