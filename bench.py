@@ -141,7 +141,7 @@ def c0(ctx):
     threads = os.cpu_count() or 1
     secs, c = oracle.render_items_mt(sc, cam, trace_settings(o), jobs, threads)
     cpu_rays = c['closest_queries'] + c['any_queries']
-    return {'workload': 'plain_cpu (C0): 10,242 triangles, 256x256, 16 spp, 4 bounces, GI off, ACES, sRGB',
+    return {'workload': 'plain_cpu: 10,242 triangles, 256x256, 16 spp, 4 bounces, GI off, ACES, sRGB',
             'gpu': {'seconds': round(dt, 4), 'rays': int(gpu_rays), 'mrays_s': round(gpu_rays / dt / 1e6, 2),
                     'note': 'raytrace_compiled: scene upload, 16 passes, encode and readback'},
             'cpu': {'seconds': round(secs, 3), 'rays': int(cpu_rays), 'mrays_s': round(cpu_rays / secs / 1e6, 2),

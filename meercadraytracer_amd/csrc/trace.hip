@@ -1205,7 +1205,7 @@ __global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ view
 }
 
 // Five waves per SIMD: the any-hit loop is latency bound, and at the register count the compiler
-// picks unasked only four fit (4.74 -> 4.44 ms per C1 pass). The closest-hit variant would spill.
+// picks unasked only four fit (4.74 -> 4.44 ms per pass of the 1.38M-triangle 1080p frame). The closest-hit variant would spill.
 template <bool MIX, bool TEX>
 __global__ __launch_bounds__(256, 5) void k_shadow_wide(const ViewDev *__restrict__ views, Wavefront w,
                                                         int32_t *spill, uint32_t spill_stride,

@@ -32,7 +32,8 @@ def c0():
 
 @pytest.fixture(scope='module')
 def glass():
-    """A small C1-like scene: heightfield, icospheres with 20% glass, sun and point lights."""
+    """A small version of the 1.38M-triangle scene: heightfield, icospheres with 20% glass, sun and
+    point lights."""
     s = _scene('plain_gpu', 2, width=96, height=96, quads=40, spheres=16, subdivisions=2)
     yield s
     s[3].release()
@@ -131,7 +132,7 @@ def test_c1_crop_4096_spp():
     gpu.release()
     mf, mg = f[..., :3].astype(np.float64).mean(), g[..., :3].astype(np.float64).mean()
     rel = abs(mg - mf) / mf
-    print(f'C1 crop at 4096 spp: fp64 {mf:.6f} wide {mg:.6f} rel {rel:.2e}; '
+    print(f'1.38M-triangle crop at 4096 spp: fp64 {mf:.6f} wide {mg:.6f} rel {rel:.2e}; '
           f'{_three_sigma(g, f):.4%} of pixel channels outside 3 sigma', flush=True)
     assert rel <= 0.005
     assert _three_sigma(g, f) <= 0.01
