@@ -140,3 +140,40 @@ def test_textured_workload_bit_exact():
             _same(g, ref, f'textured_gpu pass {p}')
     finally:
         s.release()
+
+
+def test_bad_texture_data_is_refused():
+    """Indices out of range are refused when the shading is set, with a message naming the part;
+    a scene with environments keeps one light slot for the environment-diffuse ray (at most 15
+    punctual lights)."""
+    import dataclasses
+    from meercadraytracer_amd.types import CompiledLight
+    sc, cam = _compiled('environment')
+    s = _resident(sc)
+    try:
+        bad_mat = dataclasses.replace(sc.shading.materials[0], map=len(sc.shading.textures))
+        cases = [
+            (dataclasses.replace(sc.shading, materials=[bad_mat] + sc.shading.materials[1:]), 'material 0'),
+            (dataclasses.replace(sc.shading, environments=[dataclasses.replace(sc.shading.environments[0],
+                                                                                 texture=99)]
+                                 + sc.shading.environments[1:]), 'environment 0'),
+            (dataclasses.replace(sc.shading, environment=len(sc.shading.environments)), 'scene environment'),
+            (dataclasses.replace(sc.shading, lights=[CompiledLight('point', position=(0.0, 3.0, float(i)))
+                                                     for i in range(16)]), 'more than 15 lights'),
+        ]
+        for shading, words in cases:
+            with pytest.raises(native.NativeError) as e:
+                s.set_shading(shading, 7)
+            assert words in str(e.value), (words, str(e.value))
+        # 15 punctual lights with environments still trace
+        ok = dataclasses.replace(sc.shading, lights=[CompiledLight('point', color=(0.2, 0.2, 0.2),
+                                                                   position=(float(i % 5) - 2, 3.0, float(i // 5)))
+                                                     for i in range(15)])
+        s.set_shading(ok, 8)
+        st = trace_settings({})
+        g = s.render_tile(cam, st, Tile(0, 0, 24, 16), W, H, 0, 1)
+        from meercadraytracer_amd.types import CompiledScene
+        ref, _ = oracle.render_tile(CompiledScene(sc.geometry, ok), cam, st, Tile(0, 0, 24, 16), W, H, 0, 1)
+        _same(g, ref, '15 lights with environments')
+    finally:
+        s.release()
