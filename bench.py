@@ -249,11 +249,23 @@ def main():
     sc = compile_scene(scene, timed_build)
     compile_host_s = time.time() - tc
     # compileGeometry on the device (flatten, build, BVH-order gather, tangents) on the same scene.
-    from meercadraytracer_amd.compile import compile_scene_device
+    from meercadraytracer_amd.compile import compile_scene_device, compile_scene_resident
     compile_scene_device(scene, ctx)  # allocations and first-touch
     t1 = time.time()
-    compile_scene_device(scene, ctx)
+    dev_sc = compile_scene_device(scene, ctx)
     compile_device_s = time.time() - t1
+    # ... then made resident from the downloaded arrays, against compiling straight into a scene.
+    tmp = native.Scene(ctx)
+    t1 = time.time()
+    tmp.set_geometry(dev_sc.geometry, 1)
+    tmp.set_shading(dev_sc.shading, 1)
+    upload_s = time.time() - t1
+    compile_scene_resident(scene, tmp, 2)  # first-touch
+    t1 = time.time()
+    compile_scene_resident(scene, tmp, 3)
+    resident_s = time.time() - t1
+    tmp.release()
+    del dev_sc
     cam = compile_camera(camera)
     st = trace_settings(opts)
     log(f'compiled {sc.geometry.triangleCount} triangles, {sc.geometry.bvh.nodeCount} nodes in '
@@ -317,7 +329,9 @@ def main():
                                  'device_total': round(timing['total_ms'] / args.steps, 2)},
                    'bvh_build_ms_incl_copies': round(build_times[0] * 1e3, 1),
                    'compile_geometry_ms': {'host_mirror_with_gpu_build': round(compile_host_s * 1e3, 1),
-                                           'device': round(compile_device_s * 1e3, 1)}},
+                                           'device': round(compile_device_s * 1e3, 1),
+                                           'device_then_upload': round((compile_device_s + upload_s) * 1e3, 1),
+                                           'resident': round(resident_s * 1e3, 1)}},
         'roofline': roofline,
     }
     if not args.no_traffic:

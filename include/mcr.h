@@ -419,6 +419,20 @@ int mcr_compile_count(const mcr_mesh *meshes, uint32_t mesh_count, uint32_t *tri
  * the epsilon. Replaces the main-thread compile (compile.ts:361 calls buildBvh inside it). */
 int mcr_compile_geometry(mcr_ctx *ctx, const mcr_mesh *meshes, uint32_t mesh_count, mcr_geometry_out *out);
 
+typedef struct mcr_compile_info {
+  uint32_t triangle_count, node_count;
+  uint32_t host_layout;  /* 1: leaves over 31 triangles (buildBvh's depth limit) made the scene fall
+                            back to the layout mcr_scene_set_geometry builds from host arrays */
+  double epsilon;
+} mcr_compile_info;
+/* compileGeometry straight into a resident scene (compile.ts:327-432 followed by the worker's
+ * stored geometry, protocol.ts:64-67): the same flatten, build and BVH-order arrays as
+ * mcr_compile_geometry, laid out as the scene's device records without leaving the device, then
+ * stored under `version` as mcr_scene_set_geometry would. Renders are identical to those of a scene
+ * given the mcr_compile_geometry arrays. info may be NULL. */
+int mcr_scene_compile_geometry(mcr_scene *scene, const mcr_mesh *meshes, uint32_t mesh_count, uint32_t version,
+                               mcr_compile_info *info);
+
 /* ---- device buffers, for hosts without a GPU allocator of their own ----------------------- */
 int mcr_device_alloc(mcr_ctx *ctx, uint64_t bytes, void **out);
 int mcr_device_free(mcr_ctx *ctx, void *ptr);

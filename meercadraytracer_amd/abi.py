@@ -173,6 +173,11 @@ class GeometryOut(C.Structure):
 INF_COUNT = 0xffffffff
 
 
+class CompileInfo(C.Structure):  # mcr_compile_info
+    _fields_ = [('triangle_count', C.c_uint32), ('node_count', C.c_uint32), ('host_layout', C.c_uint32),
+                ('epsilon', C.c_double)]
+
+
 class WideInfo(C.Structure):
     _fields_ = [('node_count', C.c_uint32), ('triangle_count', C.c_uint32), ('depth', C.c_uint32),
                 ('pad0', C.c_uint32), ('bytes', C.c_uint64), ('build_ms', C.c_double)]

@@ -507,6 +507,17 @@ def compile_scene_device(scene, ctx=None):
     return CompiledScene(geometry, shading)
 
 
+def compile_scene_resident(scene, gpu_scene, version):
+    """compileGeometry on the device straight into a resident scene (mcr_scene_compile_geometry),
+    then the shading; returns the ShadingPayload and the compile info. Renders match a scene given
+    compile_scene_device's arrays."""
+    shading, meshes, index = compile_shading(scene)
+    descs = [mesh_descriptor(m, index.get(id(m.material), 0), m.material) for m in meshes]
+    info = gpu_scene.compile_meshes([d for d in descs if d is not None], version)
+    gpu_scene.set_shading(shading, version)
+    return shading, info
+
+
 def compile_camera(camera, aspect=None):  # compile.ts:536-557
     if camera.parent is None:
         camera.updateMatrixWorld()

@@ -28,10 +28,7 @@ int mcr_fail(int code, const std::string &msg) {
   return code;
 }
 
-namespace {
-
-int fail(int code, const std::string &msg) { return mcr_fail(code, msg); }
-
+// Releases a scene's geometry arrays (and its wide BVH).
 void free_geometry(mcr_scene *s) {
   wide_free(s);
   hipFree(s->tri);
@@ -51,6 +48,11 @@ void free_geometry(mcr_scene *s) {
   s->bytes_geometry = 0;
   s->has_geometry = false;
 }
+
+namespace {
+
+int fail(int code, const std::string &msg) { return mcr_fail(code, msg); }
+
 
 int ensure_out_tmp(mcr_ctx *c, size_t floats) {
   if (c->out_tmp_cap >= floats) return MCR_OK;

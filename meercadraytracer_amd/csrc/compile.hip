@@ -274,38 +274,26 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t *order, uint32_t 
   if (threadIdx.x < 6) block_bounds[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-}  // namespace
+// flatten + build + gather on the device (compileGeometry up to its stored arrays). The results
+// stay in one device allocation (`dev`, freed with the Pipeline) at the o_* offsets; the caller
+// holds the context lock and either downloads them or builds a resident scene from them.
+struct Pipeline {
+  char *dev = nullptr;
+  uint32_t n = 0, node_count = 0;
+  bool any_colors = false;
+  size_t o_otri = 0, o_onrm = 0, o_ouv = 0, o_ocol = 0, o_otan = 0, o_omat = 0, o_oflg = 0, o_osid = 0;
+  size_t o_bounds = 0, o_nodes = 0;
+  double epsilon = 1e-5;
+  Pipeline() = default;
+  Pipeline(const Pipeline &) = delete;
+  Pipeline &operator=(const Pipeline &) = delete;
+  ~Pipeline() { hipFree(dev); }
+};
 
-extern "C" {
-
-int mcr_compile_count(const mcr_mesh *meshes, uint32_t mesh_count, uint32_t *triangle_count_out,
-                      int *any_colors_out) {
-  if ((!meshes && mesh_count) || !triangle_count_out) return fail(MCR_E_INVALID, "null argument");
-  uint64_t tris;
-  bool colors;
-  MCR_TRY(count_meshes(meshes, mesh_count, tris, colors));
-  *triangle_count_out = (uint32_t)tris;
-  if (any_colors_out) *any_colors_out = colors ? 1 : 0;
-  return MCR_OK;
-}
-
-int mcr_compile_geometry(mcr_ctx *c, const mcr_mesh *meshes, uint32_t mesh_count, mcr_geometry_out *out) {
-  if (!c || (!meshes && mesh_count) || !out) return fail(MCR_E_INVALID, "null argument");
-  uint64_t tris;
-  bool any_colors;
-  MCR_TRY(count_meshes(meshes, mesh_count, tris, any_colors));
-  uint32_t n = (uint32_t)tris;
-  if (out->triangle_count != n)
-    return fail(MCR_E_INVALID, "out->triangle_count must be the count mcr_compile_count gives (" +
-                                   std::to_string(n) + ")");
-  if (n == 0) {
-    out->node_count = 0;
-    out->epsilon = 1e-5;  // Math.max(1 * 1e-5, 0 * 2e-6, 1e-7): the diagonal of an empty scene is 1
-    return MCR_OK;
-  }
-  if (!out->triangles || !out->normals || !out->uvs || !out->tangents || !out->materials || !out->flags ||
-      !out->sides || !out->bvh_bounds || !out->bvh_nodes || (any_colors && !out->colors))
-    return fail(MCR_E_INVALID, "output arrays missing");
+int run_pipeline(mcr_ctx *c, const mcr_mesh *meshes, uint32_t mesh_count, uint32_t n, bool any_colors,
+                 Pipeline &pl) {
+  pl.n = n;
+  pl.any_colors = any_colors;
   // Segments and per-mesh constants; vertex arrays packed into one upload.
   std::vector<SegDev> segs;
   std::vector<MeshDev> mdev(mesh_count);
@@ -366,7 +354,6 @@ int mcr_compile_geometry(mcr_ctx *c, const mcr_mesh *meshes, uint32_t mesh_count
   size_t o_bounds = take(nodes_cap * 24), o_nodes = take(nodes_cap * 12), o_order = take((size_t)n * 4);
   uint32_t blocks = (n + 255) / 256;
   size_t o_bb = take((size_t)blocks * 48), o_maxc = take(8);
-  std::lock_guard<std::mutex> lock(c->mu);
   HIP_TRY(hipSetDevice(c->device));
   hipStream_t sm = c->stream;
   char *dev = nullptr;
@@ -419,23 +406,12 @@ int mcr_compile_geometry(mcr_ctx *c, const mcr_mesh *meshes, uint32_t mesh_count
                        (float *)(dev + o_otan), (uint32_t *)(dev + o_omat), (uint8_t *)(dev + o_oflg),
                        (uint8_t *)(dev + o_osid), (double *)(dev + o_bb));
     HIP_TRY(hipGetLastError());
-    auto down = [&](void *dst, size_t off, size_t b) { return hipMemcpyAsync(dst, dev + off, b, hipMemcpyDeviceToHost, sm); };
-    HIP_TRY(down(out->triangles, o_otri, (size_t)n * 36));
-    HIP_TRY(down(out->normals, o_onrm, (size_t)n * 36));
-    HIP_TRY(down(out->uvs, o_ouv, (size_t)n * 24));
-    HIP_TRY(down(out->tangents, o_otan, (size_t)n * 16));
-    if (any_colors) HIP_TRY(down(out->colors, o_ocol, (size_t)n * 48));
-    HIP_TRY(down(out->materials, o_omat, (size_t)n * 4));
-    HIP_TRY(down(out->flags, o_oflg, n));
-    HIP_TRY(down(out->sides, o_osid, n));
-    HIP_TRY(down(out->bvh_bounds, o_bounds, (size_t)node_count * 24));
-    HIP_TRY(down(out->bvh_nodes, o_nodes, (size_t)node_count * 12));
-    HIP_TRY(down(bb.data(), o_bb, bb.size() * 8));
-    HIP_TRY(down(&maxc_bits, o_maxc, 8));
+    HIP_TRY(hipMemcpyAsync(bb.data(), dev + o_bb, bb.size() * 8, hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipMemcpyAsync(&maxc_bits, dev + o_maxc, 8, hipMemcpyDeviceToHost, sm));
     HIP_TRY(hipStreamSynchronize(sm));
     return MCR_OK;
   }();
-  hipFree(dev);
+  pl.dev = dev;
   if (rc) return rc;
   double lo[3] = {JM_INF, JM_INF, JM_INF}, hi[3] = {-JM_INF, -JM_INF, -JM_INF};
   for (uint32_t b = 0; b < blocks; b++)
@@ -446,8 +422,224 @@ int mcr_compile_geometry(mcr_ctx *c, const mcr_mesh *meshes, uint32_t mesh_count
   double max_coord;
   memcpy(&max_coord, &maxc_bits, 8);
   double diagonal = js_hypot3(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
-  out->node_count = node_count;
-  out->epsilon = std::max(std::max(diagonal * 1e-5, max_coord * 2e-6), 1e-7);
+  pl.node_count = node_count;
+  pl.epsilon = std::max(std::max(diagonal * 1e-5, max_coord * 2e-6), 1e-7);
+  pl.o_otri = o_otri;
+  pl.o_onrm = o_onrm;
+  pl.o_ouv = o_ouv;
+  pl.o_ocol = o_ocol;
+  pl.o_otan = o_otan;
+  pl.o_omat = o_omat;
+  pl.o_oflg = o_oflg;
+  pl.o_osid = o_osid;
+  pl.o_bounds = o_bounds;
+  pl.o_nodes = o_nodes;
+  return MCR_OK;
+}
+
+// Triangle records of the resident layout (48 B: v0, e1, e2, side | flags << 8, material).
+__global__ __launch_bounds__(256) void k_tri_records(uint32_t n, const float *tri, const uint32_t *mat,
+                                                     const uint8_t *flags, const uint8_t *sides, float4 *out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float *t = tri + (size_t)k * 9;
+  uint32_t word = (uint32_t)sides[k] | ((uint32_t)flags[k] << 8);
+  out[(size_t)k * 3 + 0] = make_float4(t[0], t[1], t[2], t[3]);
+  out[(size_t)k * 3 + 1] = make_float4(t[4], t[5], t[6], t[7]);
+  out[(size_t)k * 3 + 2] = make_float4(t[8], __uint_as_float(word), __uint_as_float(mat[k]), 0.0f);
+}
+
+// Child ref of node c in the resident layout: the inner record of an inner node is its rank r in
+// buildBvh's allocation order, read off its children's indices (1 + 2r, 2 + 2r; bvh.ts:165-171);
+// a leaf packs into the ref unless it holds over 31 triangles (then *long_leaf is set and the
+// caller lays the scene out on the host instead).
+__device__ inline int32_t resident_ref(const int32_t *nodes, uint32_t c, uint32_t *long_leaf) {
+  int32_t first = nodes[(size_t)c * 3], count = nodes[(size_t)c * 3 + 1];
+  if (count == 0) return (first - 1) / 2;
+  if (count <= 31 && (uint32_t)first < MCR_MAX_TRIANGLES)
+    return (int32_t)(MCR_LEAF_BIT | ((uint32_t)count << MCR_LEAF_FIRST_BITS) | (uint32_t)first);
+  atomicOr(long_leaf, 1u);
+  return -1;
+}
+
+// Inner records of the resident layout, one thread per BVH node: both children's boxes, both refs
+// and the split axis, at the node's rank (device.h layout).
+__global__ __launch_bounds__(256) void k_inner_records(uint32_t node_count, const int32_t *nodes,
+                                                       const float *bounds, float4 *inner, uint32_t *long_leaf) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= node_count || nodes[(size_t)i * 3 + 1] != 0) return;
+  int32_t left = nodes[(size_t)i * 3];
+  const float *bl = bounds + (size_t)left * 6, *br = bounds + (size_t)(left + 1) * 6;
+  size_t r = (size_t)((left - 1) / 2) * 4;
+  inner[r + 0] = make_float4(bl[0], bl[1], bl[2], bl[3]);
+  inner[r + 1] = make_float4(bl[4], bl[5], br[0], br[1]);
+  inner[r + 2] = make_float4(br[2], br[3], br[4], br[5]);
+  int4 m = make_int4(resident_ref(nodes, (uint32_t)left, long_leaf), resident_ref(nodes, (uint32_t)left + 1, long_leaf),
+                     nodes[(size_t)i * 3 + 2], 0);
+  *reinterpret_cast<int4 *>(&inner[r + 3]) = m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcr_compile_count(const mcr_mesh *meshes, uint32_t mesh_count, uint32_t *triangle_count_out,
+                      int *any_colors_out) {
+  if ((!meshes && mesh_count) || !triangle_count_out) return fail(MCR_E_INVALID, "null argument");
+  uint64_t tris;
+  bool colors;
+  MCR_TRY(count_meshes(meshes, mesh_count, tris, colors));
+  *triangle_count_out = (uint32_t)tris;
+  if (any_colors_out) *any_colors_out = colors ? 1 : 0;
+  return MCR_OK;
+}
+
+int mcr_compile_geometry(mcr_ctx *c, const mcr_mesh *meshes, uint32_t mesh_count, mcr_geometry_out *out) {
+  if (!c || (!meshes && mesh_count) || !out) return fail(MCR_E_INVALID, "null argument");
+  uint64_t tris;
+  bool any_colors;
+  MCR_TRY(count_meshes(meshes, mesh_count, tris, any_colors));
+  uint32_t n = (uint32_t)tris;
+  if (out->triangle_count != n)
+    return fail(MCR_E_INVALID, "out->triangle_count must be the count mcr_compile_count gives (" +
+                                   std::to_string(n) + ")");
+  if (n == 0) {
+    out->node_count = 0;
+    out->epsilon = 1e-5;  // Math.max(1 * 1e-5, 0 * 2e-6, 1e-7): the diagonal of an empty scene is 1
+    return MCR_OK;
+  }
+  if (!out->triangles || !out->normals || !out->uvs || !out->tangents || !out->materials || !out->flags ||
+      !out->sides || !out->bvh_bounds || !out->bvh_nodes || (any_colors && !out->colors))
+    return fail(MCR_E_INVALID, "output arrays missing");
+  std::lock_guard<std::mutex> lock(c->mu);
+  Pipeline pl;
+  MCR_TRY(run_pipeline(c, meshes, mesh_count, n, any_colors, pl));
+  hipStream_t sm = c->stream;
+  auto down = [&](void *dst, size_t off, size_t b) { return hipMemcpyAsync(dst, pl.dev + off, b, hipMemcpyDeviceToHost, sm); };
+  HIP_TRY(down(out->triangles, pl.o_otri, (size_t)n * 36));
+  HIP_TRY(down(out->normals, pl.o_onrm, (size_t)n * 36));
+  HIP_TRY(down(out->uvs, pl.o_ouv, (size_t)n * 24));
+  HIP_TRY(down(out->tangents, pl.o_otan, (size_t)n * 16));
+  if (any_colors) HIP_TRY(down(out->colors, pl.o_ocol, (size_t)n * 48));
+  HIP_TRY(down(out->materials, pl.o_omat, (size_t)n * 4));
+  HIP_TRY(down(out->flags, pl.o_oflg, n));
+  HIP_TRY(down(out->sides, pl.o_osid, n));
+  HIP_TRY(down(out->bvh_bounds, pl.o_bounds, (size_t)pl.node_count * 24));
+  HIP_TRY(down(out->bvh_nodes, pl.o_nodes, (size_t)pl.node_count * 12));
+  HIP_TRY(hipStreamSynchronize(sm));
+  out->node_count = pl.node_count;
+  out->epsilon = pl.epsilon;
+  return MCR_OK;
+}
+
+int mcr_scene_compile_geometry(mcr_scene *s, const mcr_mesh *meshes, uint32_t mesh_count, uint32_t version,
+                               mcr_compile_info *info) {
+  if (!s || (!meshes && mesh_count)) return fail(MCR_E_INVALID, "null argument");
+  uint64_t tris;
+  bool any_colors;
+  MCR_TRY(count_meshes(meshes, mesh_count, tris, any_colors));
+  uint32_t n = (uint32_t)tris;
+  mcr_ctx *c = s->ctx;
+  if (n == 0) {  // an empty scene, as mcr_scene_set_geometry lays it out
+    mcr_geometry g = {};
+    g.epsilon = 1e-5;
+    MCR_TRY(mcr_scene_set_geometry(s, &g, version));
+    if (info) *info = mcr_compile_info{0, 0, 0, g.epsilon};
+    return MCR_OK;
+  }
+  std::unique_lock<std::mutex> lock(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t sm = c->stream;
+  HIP_TRY(hipStreamSynchronize(sm));
+  Pipeline pl;
+  MCR_TRY(run_pipeline(c, meshes, mesh_count, n, any_colors, pl));
+  uint32_t nodes = pl.node_count, ninner = (nodes - 1) / 2;
+  free_geometry(s);
+  uint32_t *long_leaf = nullptr;
+  int rc = [&]() -> int {
+    HIP_TRY(dalloc(&s->tri, (size_t)n * 3));
+    HIP_TRY(dalloc(&s->inner, (size_t)std::max<uint32_t>(ninner, 1) * 4));
+    HIP_TRY(dalloc(&s->normals, (size_t)n * 9));
+    HIP_TRY(dalloc(&s->uvs, (size_t)n * 6));
+    HIP_TRY(dalloc(&s->tangents, (size_t)n * 4));
+    if (any_colors) HIP_TRY(dalloc(&s->colors, (size_t)n * 12));
+    HIP_TRY(dalloc(&long_leaf, 1));
+    HIP_TRY(hipMemsetAsync(long_leaf, 0, 4, sm));
+    uint32_t tb = (n + 255) / 256, nb = (nodes + 255) / 256;
+    hipLaunchKernelGGL(k_tri_records, dim3(tb), dim3(256), 0, sm, n, (const float *)(pl.dev + pl.o_otri),
+                       (const uint32_t *)(pl.dev + pl.o_omat), (const uint8_t *)(pl.dev + pl.o_oflg),
+                       (const uint8_t *)(pl.dev + pl.o_osid), (float4 *)s->tri);
+    hipLaunchKernelGGL(k_inner_records, dim3(nb), dim3(256), 0, sm, nodes, (const int32_t *)(pl.dev + pl.o_nodes),
+                       (const float *)(pl.dev + pl.o_bounds), (float4 *)s->inner, long_leaf);
+    HIP_TRY(hipGetLastError());
+    auto copy = [&](void *dst, size_t off, size_t b) {
+      return hipMemcpyAsync(dst, pl.dev + off, b, hipMemcpyDeviceToDevice, sm);
+    };
+    HIP_TRY(copy(s->normals, pl.o_onrm, (size_t)n * 36));
+    HIP_TRY(copy(s->uvs, pl.o_ouv, (size_t)n * 24));
+    HIP_TRY(copy(s->tangents, pl.o_otan, (size_t)n * 16));
+    if (any_colors) HIP_TRY(copy(s->colors, pl.o_ocol, (size_t)n * 48));
+    return MCR_OK;
+  }();
+  uint32_t long_host = 0;
+  int32_t root[3] = {0, 0, 0};
+  float root_box[6] = {0, 0, 0, 0, 0, 0};
+  if (!rc) {
+    rc = [&]() -> int {
+      HIP_TRY(hipMemcpyAsync(&long_host, long_leaf, 4, hipMemcpyDeviceToHost, sm));
+      HIP_TRY(hipMemcpyAsync(root, pl.dev + pl.o_nodes, 12, hipMemcpyDeviceToHost, sm));
+      HIP_TRY(hipMemcpyAsync(root_box, pl.dev + pl.o_bounds, 24, hipMemcpyDeviceToHost, sm));
+      HIP_TRY(hipStreamSynchronize(sm));
+      return MCR_OK;
+    }();
+  }
+  hipFree(long_leaf);
+  if (rc || long_host || (root[1] > 31)) {
+    free_geometry(s);
+    if (rc) return rc;
+    // Leaves over 31 triangles (only at buildBvh's depth limit) need the host layout's long-leaf
+    // table: lay the scene out from the downloaded arrays instead.
+    std::vector<float> tri((size_t)n * 9), nrm((size_t)n * 9), uv((size_t)n * 6), tan((size_t)n * 4),
+        col(any_colors ? (size_t)n * 12 : 0), bnd((size_t)nodes * 6);
+    std::vector<uint32_t> mat(n);
+    std::vector<uint8_t> flg(n), sid(n);
+    std::vector<int32_t> nod((size_t)nodes * 3);
+    auto down = [&](void *dst, size_t off, size_t b) { return hipMemcpyAsync(dst, pl.dev + off, b, hipMemcpyDeviceToHost, sm); };
+    HIP_TRY(down(tri.data(), pl.o_otri, tri.size() * 4));
+    HIP_TRY(down(nrm.data(), pl.o_onrm, nrm.size() * 4));
+    HIP_TRY(down(uv.data(), pl.o_ouv, uv.size() * 4));
+    HIP_TRY(down(tan.data(), pl.o_otan, tan.size() * 4));
+    if (any_colors) HIP_TRY(down(col.data(), pl.o_ocol, col.size() * 4));
+    HIP_TRY(down(mat.data(), pl.o_omat, (size_t)n * 4));
+    HIP_TRY(down(flg.data(), pl.o_oflg, n));
+    HIP_TRY(down(sid.data(), pl.o_osid, n));
+    HIP_TRY(down(bnd.data(), pl.o_bounds, bnd.size() * 4));
+    HIP_TRY(down(nod.data(), pl.o_nodes, nod.size() * 4));
+    HIP_TRY(hipStreamSynchronize(sm));
+    lock.unlock();
+    mcr_geometry g = {n, nodes, tri.data(), nrm.data(), uv.data(), tan.data(), any_colors ? col.data() : nullptr,
+                      mat.data(), flg.data(), sid.data(), bnd.data(), nod.data(), pl.epsilon};
+    MCR_TRY(mcr_scene_set_geometry(s, &g, version));
+    if (info) *info = mcr_compile_info{n, nodes, 1, pl.epsilon};
+    return MCR_OK;
+  }
+  DevScene &d = s->d;
+  d.triangle_count = n;
+  d.inner_count = ninner;
+  d.epsilon = pl.epsilon;
+  d.root_ref = root[1] == 0 ? 0 : (int32_t)(MCR_LEAF_BIT | ((uint32_t)root[1] << MCR_LEAF_FIRST_BITS) | (uint32_t)root[0]);
+  for (int k = 0; k < 6; k++) d.root_box[k] = root_box[k];
+  d.tri = s->tri;
+  d.inner = s->inner;
+  d.longleaf = nullptr;
+  d.normals = s->normals;
+  d.colors = s->colors;
+  d.uvs = s->uvs;
+  d.tangents = s->tangents;
+  s->bytes_geometry = (size_t)n * 48 + (size_t)ninner * 64 + (size_t)n * (9 + 6 + 4) * 4 + (any_colors ? (size_t)n * 48 : 0);
+  s->geometry_version = version;
+  s->has_geometry = true;
+  if (info) *info = mcr_compile_info{n, nodes, 0, pl.epsilon};
   return MCR_OK;
 }
 

@@ -156,6 +156,7 @@ struct mcr_scene {
 
 // Frees the scene's wide BVH (wide.hip).
 void wide_free(mcr_scene *s);
+void free_geometry(mcr_scene *s);
 
 // The view of (scene, camera, settings) as the kernels read it.
 ViewDev make_view(const mcr_scene *s, const mcr_camera &cam, const mcr_settings &st);

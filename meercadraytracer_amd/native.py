@@ -76,6 +76,7 @@ def lib():
                                                      P, P, C.c_int, P]),
             'mcr_compile_count': (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
             'mcr_compile_geometry': (C.c_int, [P, P, C.c_uint32, C.POINTER(abi.GeometryOut)]),
+            'mcr_scene_compile_geometry': (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.POINTER(abi.CompileInfo)]),
             'mcr_intersect_any_count': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, P, C.c_int, P]),
             'mcr_device_alloc': (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
             'mcr_device_free': (C.c_int, [P, P]),
@@ -213,6 +214,18 @@ class Scene:
         check(lib().mcr_scene_set_geometry(self.handle, C.byref(g), version))
         self.geometry_version = version
 
+    def compile_meshes(self, meshes, version):
+        """compileGeometry straight into this scene (mcr_scene_compile_geometry): mesh descriptors as
+        compile_meshes takes them; the arrays never leave the device. Returns triangle and node
+        counts, the epsilon and whether the scene fell back to the host layout."""
+        arr, keep = _mesh_array(meshes)
+        info = abi.CompileInfo()
+        check(lib().mcr_scene_compile_geometry(self.handle, C.cast(arr, C.c_void_p), len(meshes), version,
+                                               C.byref(info)))
+        self.geometry_version = version
+        return {'triangle_count': info.triangle_count, 'node_count': info.node_count,
+                'host_layout': bool(info.host_layout), 'epsilon': info.epsilon}
+
     def set_shading(self, shading, version):
         s = shading.to_c()
         check(lib().mcr_scene_set_shading(self.handle, C.byref(s), version))
@@ -307,13 +320,9 @@ class Scene:
             pass
 
 
-def compile_meshes(ctx, meshes):
-    """compileGeometry on the device (mcr_compile_geometry). meshes: list of dicts with 'world'
-    (16 floats, column-major), 'positions' (v, 3) float32 and optional 'normals', 'uvs', 'colors'
-    ((v, 3|4) float32), 'index' (uint32), 'draw' ((start, count) or None) and 'groups': list of
-    (start, count, material, flags, side, flat_shading, vertex_colors), count None = Infinity.
-    Returns the arrays of a CompiledGeometry as a dict."""
-    ctx = ctx or default_context()
+def _mesh_array(meshes):
+    """The mcr_mesh array of mesh descriptors (see compile_meshes), with the numpy arrays it points
+    into (keep them alive while the array is used)."""
     keep = []
     arr = (abi.Mesh * max(1, len(meshes)))()
     for i, m in enumerate(meshes):
@@ -345,6 +354,17 @@ def compile_meshes(ctx, meshes):
                                       int(bool(flat)), int(bool(colors)))
         keep.append(groups)
         d.groups, d.group_count = C.cast(groups, C.c_void_p), len(m['groups'])
+    return arr, keep
+
+
+def compile_meshes(ctx, meshes):
+    """compileGeometry on the device (mcr_compile_geometry). meshes: list of dicts with 'world'
+    (16 floats, column-major), 'positions' (v, 3) float32 and optional 'normals', 'uvs', 'colors'
+    ((v, 3|4) float32), 'index' (uint32), 'draw' ((start, count) or None) and 'groups': list of
+    (start, count, material, flags, side, flat_shading, vertex_colors), count None = Infinity.
+    Returns the arrays of a CompiledGeometry as a dict."""
+    ctx = ctx or default_context()
+    arr, keep = _mesh_array(meshes)
     n = C.c_uint32()
     any_colors = C.c_int()
     check(lib().mcr_compile_count(C.cast(arr, C.c_void_p), len(meshes), C.byref(n), C.byref(any_colors)))

@@ -46,6 +46,7 @@ STRUCTS = {
     'mcr_request_status': abi.RequestStatus, 'mcr_server_stats': abi.ServerStats,
     'mcr_wide_info': abi.WideInfo, 'mcr_mesh_group': abi.MeshGroup, 'mcr_mesh': abi.Mesh,
     'mcr_geometry_out': abi.GeometryOut, 'mcr_texture': abi.Texture, 'mcr_environment': abi.Environment,
+    'mcr_compile_info': abi.CompileInfo,
 }
 
 

@@ -111,3 +111,102 @@ def test_empty_and_errors():
     with pytest.raises(native.NativeError) as e:
         native.compile_meshes(ctx, [bad])
     assert 'out of range' in str(e.value)
+
+
+@pytest.mark.parametrize('kind,seed,kw', [('plain_cpu', 0, {}), ('plain_gpu', 2, dict(quads=30, spheres=12, subdivisions=2)),
+                                          ('textured_gpu', 1, dict(quads=40, spheres=10, subdivisions=2))])
+def test_resident_compile_renders_identically(kind, seed, kw):
+    """mcr_scene_compile_geometry: the compiled arrays laid out as the scene's device records
+    without leaving the device. Every float of rendered tiles equals the render of a scene given
+    the downloaded arrays (mcr_compile_geometry + mcr_scene_set_geometry) and the oracle's; the
+    wide BVH builds from the resident layout and renders the same as from the host layout."""
+    import oracle
+    from meercadraytracer_amd.compile import compile_camera, compile_scene_resident
+    from meercadraytracer_amd.raytrace import trace_settings
+    from meercadraytracer_amd.types import Tile
+    ctx = native.default_context()
+    maker = {'plain_cpu': scenes.plain_cpu_scene, 'plain_gpu': scenes.plain_gpu_scene,
+             'textured_gpu': scenes.textured_gpu_scene}[kind]
+    scene, camera, o = maker(seed, 64, 48, **kw) if kind != 'plain_cpu' else maker(seed, 64, 48)
+    cam = compile_camera(camera)
+    st = trace_settings(dict(o, maxBounces=4))
+    host = compile_scene_device(scene, ctx)
+    a = native.Scene(ctx)
+    a.set_geometry(host.geometry, 1)
+    a.set_shading(host.shading, 1)
+    b = native.Scene(ctx)
+    shading, info = compile_scene_resident(scene, b, 1)
+    assert info['triangle_count'] == host.geometry.triangleCount and not info['host_layout']
+    assert info['node_count'] == host.geometry.bvh.nodeCount and info['epsilon'] == host.geometry.epsilon
+    try:
+        for tile, p in [(Tile(0, 0, 64, 48), 0), (Tile(8, 4, 40, 30), 3)]:
+            print(f'{kind} resident tile {tile} pass {p}', flush=True)
+            ga = a.render_tile(cam, st, tile, 64, 48, p, 1)
+            gb = b.render_tile(cam, st, tile, 64, 48, p, 1)
+            ref, _ = oracle.render_tile(host, cam, st, tile, 64, 48, p, 1)
+            assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32)), kind
+            assert np.array_equal(gb.view(np.uint32), ref.view(np.uint32)), kind
+        ia, ib = a.build_wide(), b.build_wide()
+        assert ia['node_count'] == ib['node_count'] and ia['triangle_count'] == ib['triangle_count']
+        ctx.set_intersection(abi.MCR_ISECT_WIDE_FP32)
+        try:
+            wa = a.render_tile(cam, st, Tile(0, 0, 64, 48), 64, 48, 1, 1)
+            wb = b.render_tile(cam, st, Tile(0, 0, 64, 48), 64, 48, 1, 1)
+        finally:
+            ctx.set_intersection(abi.MCR_ISECT_FP64)
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+    finally:
+        a.release()
+        b.release()
+
+
+def test_resident_compile_empty_and_errors():
+    ctx = native.default_context()
+    s = native.Scene(ctx)
+    try:
+        info = s.compile_meshes([], 3)
+        assert info['triangle_count'] == 0 and info['epsilon'] == 1e-5
+        pos = np.zeros((3, 3), np.float32)
+        bad = {'world': np.eye(4).reshape(-1), 'positions': pos, 'index': np.array([0, 1, 7], np.uint32),
+               'groups': [(0, None, 0, 0, 0, False, False)]}
+        with pytest.raises(native.NativeError) as e:
+            s.compile_meshes([bad], 4)
+        assert 'out of range' in str(e.value)
+    finally:
+        s.release()
+
+
+def test_resident_compile_coincident_pile():
+    """500 coincident triangles over a plane: buildBvh halves runs whose centroids coincide
+    (bvh.ts:158-162), so the resident layout packs every leaf; the scene renders exactly as the
+    oracle renders the downloaded arrays."""
+    import oracle
+    from meercadraytracer_amd.compile import compile_camera, compile_scene_resident
+    from meercadraytracer_amd.raytrace import trace_settings
+    from meercadraytracer_amd.types import Tile
+    ctx = native.default_context()
+    tri = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]] * 500, np.float32)
+    root = three.Scene()
+    pile = three.Mesh(three.BufferGeometry(tri), three.MeshStandardMaterial(color=0x88aacc))
+    plane = three.Mesh(three.PlaneGeometry(6, 6), three.MeshLambertMaterial(color=0xcc8844))
+    plane.position = [0.0, 0.0, -0.5]
+    sun = three.DirectionalLight(0xffffff, 2)
+    sun.position = [1.0, 2.0, 5.0]
+    root.add(pile, plane, sun)
+    root.updateMatrixWorld()
+    camera = three.PerspectiveCamera(50, 1.0, 0.1, 100)
+    camera.position = [0.0, 0.0, 6.0]
+    camera.lookAt(0, 0, 0)
+    camera.updateMatrixWorld()
+    cam = compile_camera(camera)
+    st = trace_settings({})
+    host = compile_scene_device(root, ctx)
+    b = native.Scene(ctx)
+    try:
+        shading, info = compile_scene_resident(root, b, 1)
+        assert not info['host_layout'] and info['triangle_count'] == 502
+        g = b.render_tile(cam, st, Tile(0, 0, 32, 32), 32, 32, 0, 1)
+        ref, _ = oracle.render_tile(host, cam, st, Tile(0, 0, 32, 32), 32, 32, 0, 1)
+        assert np.array_equal(g.view(np.uint32), ref.view(np.uint32))
+    finally:
+        b.release()
