@@ -741,9 +741,10 @@ int mcr_intersect_closest(mcr_ctx *c, mcr_scene *s, const double *rays, uint32_t
   }
   HIP_TRY(counters_clear(c->eng.counters, sm));
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[0], sm));
+  HIP_TRY(hipMemsetAsync(c->eng.fetch, 0, FETCH_BYTES, sm));
   launch_probe_closest(sm, s->d, drays, n, t_min, t_max, cull, dtri, dt, du, dv, dfront,
                        std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->eng.spill, TRAVERSE_THREADS,
-                       c->eng.counters);
+                       c->eng.counters, c->eng.fetch);
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[1], sm));
   HIP_TRY(hipGetLastError());
   if (!device_ptrs) {
@@ -790,9 +791,10 @@ int mcr_intersect_any_count(mcr_ctx *c, mcr_scene *s, const double *rays, uint32
     drays = r;
   }
   HIP_TRY(counters_clear(c->eng.counters, sm));
+  HIP_TRY(hipMemsetAsync(c->eng.fetch, 0, FETCH_BYTES, sm));
   launch_probe_any_count(sm, s->d, drays, n, t_min, t_max, dcount,
                          std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256), c->eng.spill, TRAVERSE_THREADS,
-                         c->eng.counters);
+                         c->eng.counters, c->eng.fetch);
   HIP_TRY(hipGetLastError());
   if (!device_ptrs) HIP_TRY(hipMemcpyAsync(count_out, dcount, (size_t)n * 4, hipMemcpyDeviceToHost, sm));
   HIP_TRY(hipStreamSynchronize(sm));

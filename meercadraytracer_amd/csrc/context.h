@@ -56,6 +56,7 @@ struct Engine {
   int2 *seen_meta = nullptr;
   double2 *seen_uv = nullptr;      // their barycentrics, for textured see-through materials
   DevCounters *counters = nullptr;
+  uint32_t *fetch = nullptr;       // work counters of the ray-query kernels (WaveFetch)
   ViewDev *views = nullptr;
   uint32_t views_cap = 0;
   JobDev *jobs = nullptr;
@@ -112,6 +113,7 @@ int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<J
 void engine_add_counters(const DevCounters &d, uint64_t paths, mcr_counters *out);
 // The MCR_COUNTER_COPIES device copies of the counters: bytes, clear, and their sum on the host.
 const size_t COUNTER_BYTES = (size_t)MCR_COUNTER_COPIES * MCR_COUNTER_STRIDE;
+const size_t FETCH_BYTES = sizeof(uint32_t) * MCR_FETCH_COUNTERS * MCR_COUNT_STRIDE;
 hipError_t counters_clear(DevCounters *d, hipStream_t s);
 hipError_t counters_read(const DevCounters *d, DevCounters &sum);
 DevCounters counters_sum(const void *copies);

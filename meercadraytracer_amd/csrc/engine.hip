@@ -91,6 +91,7 @@ int engine_init(Engine &e, int device, hipStream_t stream) {
   HIP_TRY(hipMalloc(&e.seen_meta, sizeof(int2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
   HIP_TRY(hipMalloc(&e.seen_uv, sizeof(double2) * (size_t)MCR_SHADOW_HIT_CAP * TRAVERSE_THREADS));
   HIP_TRY(hipMalloc(&e.counters, COUNTER_BYTES));
+  HIP_TRY(hipMalloc(&e.fetch, FETCH_BYTES));
   HIP_TRY(hipMalloc(&e.ctl, sizeof(Control)));
   HIP_TRY(hipHostMalloc(&e.ctl_host, sizeof(Control), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc(&e.counts_host, sizeof(uint32_t) * COUNT_WORDS_ALLOC, hipHostMallocDefault));
@@ -109,6 +110,7 @@ void engine_free(Engine &e) {
   hipFree(e.seen_meta);
   hipFree(e.seen_uv);
   hipFree(e.counters);
+  hipFree(e.fetch);
   hipFree(e.views);
   hipFree(e.jobs);
   hipFree(e.job_done);

@@ -29,10 +29,10 @@ void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefro
 void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                           double tmin, double tmax, int cull, int32_t *tri, double *t, double *u,
                           double *v, uint8_t *front, uint32_t grid, int32_t *spill,
-                          uint32_t spill_stride, DevCounters *cnt);
+                          uint32_t spill_stride, DevCounters *cnt, uint32_t *fetch);
 void launch_probe_any_count(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                             double tmin, double tmax, uint32_t *count, uint32_t grid,
-                            int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
+                            int32_t *spill, uint32_t spill_stride, DevCounters *cnt, uint32_t *fetch);
 void launch_encode_batch(hipStream_t s, const EncodeItem *items, uint32_t n, uint32_t blocks);
 void launch_encode(hipStream_t s, uint8_t *image, uint32_t width, const mcr_tile &tile,
                    const float *data, float *sum, double *sum64, uint32_t count, int srgb);

@@ -212,33 +212,6 @@ MCR_INLINE void start_sample_any(const ViewDev *__restrict__ views, const Wavefr
   with_view<MIX>(views, j.view, [&](const ViewDev &V) { start_sample(w, slot, s, j, job, V); });
 }
 
-// Work distribution of the traversal kernels over a queue of n items in 64-item chunks. One
-// counter would serialise every wave's atomic on one cache line; MCR_FETCH_COUNTERS counters
-// interleave instead: counter k hands out chunks k, k + K, k + 2K, ... (K counters), each wave
-// starts on counter (its wave index mod K) and moves on to the next counter when its own runs out.
-// The counters advance together, so the chunks in flight still sweep the queue front to back
-// (neighbouring paths stay in flight together) while each counter sees a K-th of the atomics.
-// Every lane of the wave must call next(); it returns the chunk's first item, or n when none is left.
-struct WaveFetch {
-  uint32_t *counters;
-  uint32_t n, k, tried;  // wave-uniform
-  __device__ WaveFetch(uint32_t *c, uint32_t items)
-      : counters(c), n(items), k((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % MCR_FETCH_COUNTERS),
-        tried(0) {}
-  __device__ uint32_t next() {
-    uint32_t chunks = (n + 63) / 64;
-    while (tried < MCR_FETCH_COUNTERS) {
-      uint32_t j = 0;
-      if ((threadIdx.x & 63) == 0) j = atomicAdd(&counters[cw(k)], 1u);
-      j = __builtin_amdgcn_readfirstlane(__shfl(j, 0, 64));
-      uint64_t chunk = (uint64_t)j * MCR_FETCH_COUNTERS + k;
-      if (chunk < chunks) return (uint32_t)chunk * 64u;
-      k = k + 1 == MCR_FETCH_COUNTERS ? 0 : k + 1;
-      tried++;
-    }
-    return n;
-  }
-};
 
 // ---- extend: closest hit for every live slot (bvh.ts:202-282) ------------------------------------
 // WIDE: the precision form's FP32 query over the scene's 8-wide BVH (wide.h) instead of the FP64
@@ -1391,10 +1364,10 @@ __global__ __launch_bounds__(256) void k_probe_closest(DevScene sc, const double
                                                        double tmin, double tmax, int cull,
                                                        int32_t *tri, double *t, double *u, double *v,
                                                        uint8_t *front, int32_t *spill,
-                                                       uint32_t spill_stride, DevCounters *cnt) {
+                                                       uint32_t spill_stride, DevCounters *cnt,
+                                                       uint32_t *fetch_counters) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t nthreads = gridDim.x * blockDim.x;
   LaneStack st;
   st.lds = lds_stack + threadIdx.x;
   st.stride = 256;
@@ -1402,7 +1375,10 @@ __global__ __launch_bounds__(256) void k_probe_closest(DevScene sc, const double
   st.spill_stride = (int)spill_stride;
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
-  for (uint32_t i = gid; i < n; i += nthreads) {
+  WaveFetch fetch(fetch_counters, n);
+  for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
+    uint32_t i = b + (threadIdx.x & 63);
+    if (i >= n) continue;
     const double *r = rays + (size_t)i * 6;
     TriHit h;
     intersect_closest(sc, r[0], r[1], r[2], r[3], r[4], r[5], tmin, tmax, cull != 0, h, st, tc);
@@ -1427,10 +1403,9 @@ struct CountVisitor {
 __global__ __launch_bounds__(256) void k_probe_any_count(DevScene sc, const double *rays, uint32_t n,
                                                          double tmin, double tmax, uint32_t *count,
                                                          int32_t *spill, uint32_t spill_stride,
-                                                         DevCounters *cnt) {
+                                                         DevCounters *cnt, uint32_t *fetch_counters) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t nthreads = gridDim.x * blockDim.x;
   LaneStack st;
   st.lds = lds_stack + threadIdx.x;
   st.stride = 256;
@@ -1438,7 +1413,10 @@ __global__ __launch_bounds__(256) void k_probe_any_count(DevScene sc, const doub
   st.spill_stride = (int)spill_stride;
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
-  for (uint32_t i = gid; i < n; i += nthreads) {
+  WaveFetch fetch(fetch_counters, n);
+  for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
+    uint32_t i = b + (threadIdx.x & 63);
+    if (i >= n) continue;
     const double *r = rays + (size_t)i * 6;
     CountVisitor cv{0};
     intersect_any(sc, r[0], r[1], r[2], r[3], r[4], r[5], tmin, tmax, st, tc, cv);
@@ -1589,15 +1567,15 @@ void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefro
 void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                           double tmin, double tmax, int cull, int32_t *tri, double *t, double *u,
                           double *v, uint8_t *front, uint32_t grid, int32_t *spill,
-                          uint32_t spill_stride, DevCounters *cnt) {
+                          uint32_t spill_stride, DevCounters *cnt, uint32_t *fetch) {
   hipLaunchKernelGGL(k_probe_closest, dim3(grid), dim3(256), 0, s, sc, rays, n, tmin, tmax, cull,
-                     tri, t, u, v, front, spill, spill_stride, cnt);
+                     tri, t, u, v, front, spill, spill_stride, cnt, fetch);
 }
 void launch_probe_any_count(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                             double tmin, double tmax, uint32_t *count, uint32_t grid,
-                            int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
+                            int32_t *spill, uint32_t spill_stride, DevCounters *cnt, uint32_t *fetch) {
   hipLaunchKernelGGL(k_probe_any_count, dim3(grid), dim3(256), 0, s, sc, rays, n, tmin, tmax,
-                     count, spill, spill_stride, cnt);
+                     count, spill, spill_stride, cnt, fetch);
 }
 void launch_encode_batch(hipStream_t s, const EncodeItem *items, uint32_t n, uint32_t blocks) {
   if (n && blocks) hipLaunchKernelGGL(k_encode_batch, dim3(blocks), dim3(256), 0, s, items, n);

@@ -79,6 +79,34 @@ enum OverflowBits { OVERFLOW_SEEN = 1, OVERFLOW_WIDE = 2 };
 __host__ __device__ constexpr inline uint32_t cw(uint32_t word) { return word * MCR_COUNT_STRIDE; }
 const uint32_t COUNT_WORDS_ALLOC = C_WORDS * MCR_COUNT_STRIDE;
 
+// Work distribution of the traversal kernels over a queue of n items in 64-item chunks. One
+// counter would serialise every wave's atomic on one cache line; MCR_FETCH_COUNTERS counters
+// interleave instead: counter k hands out chunks k, k + K, k + 2K, ... (K counters), each wave
+// starts on counter (its wave index mod K) and moves on to the next counter when its own runs out.
+// The counters advance together, so the chunks in flight still sweep the queue front to back
+// (neighbouring paths stay in flight together) while each counter sees a K-th of the atomics.
+// Every lane of the wave must call next(); it returns the chunk's first item, or n when none is left.
+struct WaveFetch {
+  uint32_t *counters;
+  uint32_t n, k, tried;  // wave-uniform
+  __device__ WaveFetch(uint32_t *c, uint32_t items)
+      : counters(c), n(items), k((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % MCR_FETCH_COUNTERS),
+        tried(0) {}
+  __device__ uint32_t next() {
+    uint32_t chunks = (n + 63) / 64;
+    while (tried < MCR_FETCH_COUNTERS) {
+      uint32_t j = 0;
+      if ((threadIdx.x & 63) == 0) j = atomicAdd(&counters[cw(k)], 1u);
+      j = __builtin_amdgcn_readfirstlane(__shfl(j, 0, 64));
+      uint64_t chunk = (uint64_t)j * MCR_FETCH_COUNTERS + k;
+      if (chunk < chunks) return (uint32_t)chunk * 64u;
+      k = k + 1 == MCR_FETCH_COUNTERS ? 0 : k + 1;
+      tried++;
+    }
+    return n;
+  }
+};
+
 struct JobDev;
 
 // What the kernels of one call read besides the path state. It lives in device memory so a

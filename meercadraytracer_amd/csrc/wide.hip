@@ -222,10 +222,10 @@ __global__ __launch_bounds__(256) void k_wide_level(WideBuild a, uint32_t lo, ui
 __global__ __launch_bounds__(256) void k_wide_closest(WideScene ws, const float *rays, uint32_t n, float tmin,
                                                       float tmax, int cull, int32_t *tri, float *t, float *u,
                                                       float *v, uint8_t *front, int32_t *spill,
-                                                      uint32_t spill_stride, DevCounters *cnt, uint32_t *err) {
+                                                      uint32_t spill_stride, DevCounters *cnt, uint32_t *err,
+                                                      uint32_t *fetch_counters) {
   __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t nthreads = gridDim.x * blockDim.x;
   WideStack st;
   st.lb = lds_base + threadIdx.x;
   st.lw = lds_bits + threadIdx.x;
@@ -236,7 +236,10 @@ __global__ __launch_bounds__(256) void k_wide_closest(WideScene ws, const float 
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
   bool bad = false;
-  for (uint32_t i = gid; i < n; i += nthreads) {
+  WaveFetch fetch(fetch_counters, n);
+  for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
+    uint32_t i = b + (threadIdx.x & 63);
+    if (i >= n) continue;
     const float *r = rays + (size_t)i * 6;
     WideRay wr;
     WideHit h;
@@ -407,8 +410,9 @@ int mcr_intersect_closest_wide(mcr_ctx *c, mcr_scene *s, const float *rays, uint
   HIP_TRY(hipMemsetAsync(derr, 0, 4, sm));
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[0], sm));
   uint32_t grid = std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256);
+  HIP_TRY(hipMemsetAsync(c->eng.fetch, 0, FETCH_BYTES, sm));
   hipLaunchKernelGGL(k_wide_closest, dim3(grid), dim3(256), 0, sm, s->wide, drays, n, t_min, t_max, cull, dtri,
-                     dt, du, dv, dfront, c->eng.spill, TRAVERSE_THREADS, c->eng.counters, derr);
+                     dt, du, dv, dfront, c->eng.spill, TRAVERSE_THREADS, c->eng.counters, derr, c->eng.fetch);
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[1], sm));
   HIP_TRY(hipGetLastError());
   if (!device_ptrs) {
