@@ -447,7 +447,8 @@ typedef struct mcr_timing {
 } mcr_timing;
 int mcr_last_timing(mcr_ctx *ctx, mcr_timing *out);
 int mcr_set_timing(mcr_ctx *ctx, int enabled);
-/* Path slots of the wavefront (samples in flight). Default 2^21; at least 256. Samples beyond the
+/* Path slots of the wavefront (samples in flight). Default 2^23 (17.5 GB); at least 256, at most
+ * 2^26; a run allocates no more slots than it has samples. Samples beyond the
  * capacity start as earlier ones finish, so the capacity changes memory use, not results. */
 int mcr_set_wavefront_capacity(mcr_ctx *ctx, uint32_t slots);
 /* Replays bounce iterations from a captured HIP graph (default on; off launches each kernel with

@@ -610,7 +610,7 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   if (config) s->cfg = *config;
   else s->cfg.use_graph = 1;
   if (!s->cfg.hbm_budget) s->cfg.hbm_budget = 200ull << 30;
-  if (!s->cfg.slots) s->cfg.slots = DEFAULT_CAPACITY;
+  if (!s->cfg.slots) s->cfg.slots = SERVER_DEFAULT_SLOTS;
   if (s->cfg.slots < 256 || s->cfg.slots > (1u << 26)) return mcr_fail(MCR_E_INVALID, "slots must be in [256, 2^26]");
   if (!s->cfg.window) s->cfg.window = s->cfg.slots;
   if (!s->cfg.max_requests) s->cfg.max_requests = 64;

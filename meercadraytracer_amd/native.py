@@ -14,6 +14,7 @@ from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libmcr.so')
+DEFAULT_SLOTS = 1 << 23  # the library's default path-slot count (csrc/context.h DEFAULT_CAPACITY)
 
 
 class NativeError(RuntimeError):
@@ -163,7 +164,9 @@ class Context:
         """abi.MCR_ISECT_FP64 (default) or abi.MCR_ISECT_WIDE_FP32 for renders on this context."""
         check(lib().mcr_set_intersection(self.handle, int(mode)))
 
-    def set_capacity(self, slots):
+    def set_capacity(self, slots=None):
+        """Path slots of this context's wavefront; None restores the library default."""
+        slots = DEFAULT_SLOTS if slots is None else slots
         check(lib().mcr_set_wavefront_capacity(self.handle, int(slots)))
 
     def last_timing(self):

@@ -230,7 +230,7 @@ def run_batch(ctx, views, jobs, jv, out_ptr, capacity, graph):
 
 
 def autotune(ctx, views, jobs, jv, out_ptr, log=print,
-             capacities=(1 << 19, 1 << 20, 1 << 21, 1 << 22), graphs=(True, False)):
+             capacities=(1 << 20, 1 << 21, 1 << 22, 1 << 23), graphs=(True, False)):
     """Times each (path slots, graph replay) choice on the first quarter of the jobs and returns
     the fastest, with the trial table."""
     sample = jobs[:max(1, len(jobs) // 4)]
@@ -253,12 +253,12 @@ def batched_form(ctx, seed=1, n_jobs=256, n_scenes=16, log=print, tune=True):
     out = native.DeviceBuffer(ctx, floats * 4)
     try:
         best, trials = (autotune(ctx, views, jobs, jv, out.ptr, log) if tune
-                        else ({'slots': 1 << 21, 'graph': True}, []))
+                        else ({'slots': native.DEFAULT_SLOTS, 'graph': True}, []))
         run_batch(ctx, views, jobs, jv, out.ptr, best['slots'], best['graph'])  # warm
         dt, cnt = run_batch(ctx, views, jobs, jv, out.ptr, best['slots'], best['graph'])
     finally:
         out.free()
-        ctx.set_capacity(1 << 21)
+        ctx.set_capacity()
         ctx.set_graph(True)
         for gpu, _, _ in scenes:
             gpu.release()
@@ -290,7 +290,7 @@ def batched_equivalent(ctx, scenes, specs, slots=1 << 21, log=print):
         dt, cnt = run_batch(ctx, views, jobs, jv, out.ptr, slots, False)
     finally:
         out.free()
-        ctx.set_capacity(1 << 21)
+        ctx.set_capacity()
         ctx.set_graph(True)
     rays = cnt.closest_queries + cnt.any_queries
     log(f'batched equivalent: {len(jobs)} tile passes of {len(specs)} requests in {dt:.2f} s')

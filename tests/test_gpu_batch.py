@@ -64,7 +64,7 @@ def test_mixed_views_bit_exact(views, capacity, graph):
     try:
         out = native.render_batch(ctx, vlist, jobs, jv)
     finally:
-        ctx.set_capacity(1 << 21)
+        ctx.set_capacity()
         ctx.set_graph(True)
     for j, (v, t, p, block) in zip(jobs, meta):
         sc, cam, st, w, h = views[v]

@@ -94,7 +94,7 @@ def test_slot_refill_small_capacity(c0):
     try:
         out = s.render_jobs(cam, st, jobs)
     finally:
-        ctx.set_capacity(1 << 21)
+        ctx.set_capacity()
     for j, t in zip(jobs, tiles):
         ref, _ = oracle.render_tile(sc, cam, st, t, 96, 64, j.pass_, j.block)
         got = out[j.out_offset:j.out_offset + t.width * t.height * 4]

@@ -82,7 +82,7 @@ def test_textured_and_plain_views_in_one_batch(capacity, graph):
     try:
         out = native.render_batch(ctx, vlist, jobs, jv)
     finally:
-        ctx.set_capacity(1 << 21)
+        ctx.set_capacity()
         ctx.set_graph(True)
         for r in res:
             r.release()
