@@ -104,8 +104,10 @@ def measure_traffic(args, kernel, launches):
     if not shutil.which('rocprofv3') or launches == 0:
         return None
     outdir = tempfile.mkdtemp(prefix='mcr_pmc_', dir='/tmp')
+    # Only the plain run: the other sections launch the same kernels and would mix into the tail.
     bench_args = ['--steps', str(args.steps), '--warmup', str(args.warmup), '--no-cpu', '--no-traffic',
-                  '--no-batched', '--no-served', '--width', str(args.width), '--height', str(args.height)]
+                  '--no-batched', '--no-served', '--no-precision', '--no-textures',
+                  '--width', str(args.width), '--height', str(args.height)]
     copy_args = ['--copy-calibration', '4096']
     res = {}
     for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
