@@ -102,6 +102,35 @@ def test_slot_refill_small_capacity(c0):
     s.release()
 
 
+def test_default_capacity_full_frame_matches_smaller_capacity():
+    """Five 1920x1080 passes (10.4M samples) fill the default 2^23 path slots; the frames are
+    bit-identical to a run on 2^21 slots (the capacity changes memory use, not results), and a
+    16x16 block of the last pass matches the oracle."""
+    from meercadraytracer_amd import native
+    W, H = 1920, 1080
+    sc, cam, base = scenes.compiled('plain_cpu', 0, build_bvh=oracle.build_bvh, width=W, height=H)
+    st = trace_settings(dict(base, globalIllumination=True, maxBounces=4))
+    ctx = native.default_context()
+    s = _gpu_scene(sc)
+    npx = W * H
+    jobs = [abi.TileJob(Tile(0, 0, W, H).to_c(), W, H, p, 1, p * npx * 4) for p in range(5)]
+    print(f'{len(jobs)} passes of {W}x{H}, default slots', flush=True)
+    ctx.set_capacity()
+    big = s.render_jobs(cam, st, jobs)
+    ctx.set_capacity(1 << 21)
+    try:
+        small = s.render_jobs(cam, st, jobs)
+    finally:
+        ctx.set_capacity()
+    assert np.array_equal(big.view(np.uint32), small.view(np.uint32))
+    t = Tile(952, 532, 16, 16)
+    ref, _ = oracle.render_tile(sc, cam, st, t, W, H, 4, 1)
+    frame = big[4 * npx * 4:5 * npx * 4].reshape(H, W, 4)
+    got = np.ascontiguousarray(frame[t.y:t.y + t.height, t.x:t.x + t.width]).reshape(-1)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    s.release()
+
+
 def test_raytrace_bytes_equal_oracle(c0):
     from meercadraytracer_amd.raytrace import raytrace_compiled
     sc, cam, base = c0
