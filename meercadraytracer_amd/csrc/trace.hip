@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256, MIX ? 4 : 5) void k_extend(const ViewDev *__re
                                                 DevCounters *cnt) {
   __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid == 0) w.counts[cw(C_SHADOW)] = w.counts[cw(C_SHADOW_BACK)] = 0;  // shade of this iteration appends
+  if (gid < 2 * MCR_MAX_LIGHTS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;  // shade of this iteration appends
   LaneStack st;
   st.lds = lds_stack + threadIdx.x;
   st.stride = 256;
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_extend_wide(const ViewDev *__restrict__
                                                         DevCounters *cnt) {
   __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid == 0) w.counts[cw(C_SHADOW)] = w.counts[cw(C_SHADOW_BACK)] = 0;
+  if (gid < 2 * MCR_MAX_LIGHTS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;
   WideStack st{lds_base + threadIdx.x, lds_bits + threadIdx.x, spill + gid, (int)spill_stride, 256, 0};
   extend_loop<MIX, true>(views, w, q, st, cnt);
 }
@@ -954,36 +954,35 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
       nl = V.shadow_slots;
     });
   }
-  // Enqueue the wave's shadow rays (one atomic per counter per block), light by light inside the
-  // wave's range, so a traversal wave mostly holds rays towards one light from neighbouring paths.
-  // Rays of first-hit vertices (neighbouring pixels) go to the front of sref and the rest to the
-  // back, so chunks of the former are not diluted by scattered later bounces. The order of the
-  // queue does not change any result: each ray writes its own (slot, light) entry.
+  // Enqueue the wave's shadow rays (one atomic per counter per block). Each light has a region of
+  // `capacity` entries in sref: rays of first-hit vertices (neighbouring pixels) fill it from the
+  // front and rays of later vertices from the back, so a traversal chunk mostly holds rays towards
+  // one light from neighbouring points rather than a mix diluted by scattered later bounces. The
+  // order of the queue does not change any result: each ray writes its own (slot, light) entry.
   uint32_t mask = 0;
   if (live && nshadow)
     for (uint32_t li = 0; li < nl; li++)
       if (w.status[(size_t)li * P + slot] == LS_RAY) mask |= 1u << li;
-  uint32_t total_first = 0, total_later = 0;
+  constexpr int NSEG = 2 * MCR_MAX_LIGHTS;
+  uint32_t tot[NSEG], got[NSEG];
+  uint32_t *ctr[NSEG];
   for (int li = 0; li < MCR_MAX_LIGHTS; li++) {
     bool want = (mask >> li) & 1u;
-    total_first += (uint32_t)__popcll(__ballot(want && first_hit));
-    total_later += (uint32_t)__popcll(__ballot(want && !first_hit));
+    tot[2 * li] = (uint32_t)__popcll(__ballot(want && first_hit));
+    tot[2 * li + 1] = (uint32_t)__popcll(__ballot(want && !first_hit));
+    ctr[2 * li] = &w.counts[cw(C_SHADOW_SEG + 2 * li)];
+    ctr[2 * li + 1] = &w.counts[cw(C_SHADOW_SEG + 2 * li + 1)];
   }
-  __shared__ uint32_t lds[8];
-  uint32_t *ctr[2] = {&w.counts[cw(C_SHADOW)], &w.counts[cw(C_SHADOW_BACK)]};
-  uint32_t tot[2] = {total_first, total_later}, got[2];
-  block_reserve<2>(ctr, tot, lds, got);
-  uint32_t base_first = got[0], base_later = got[1];
-  const size_t L = (size_t)MCR_MAX_LIGHTS * P;
+  __shared__ uint32_t lds[4 * NSEG];
+  block_reserve<NSEG>(ctr, tot, lds, got);
   uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
   for (int li = 0; li < MCR_MAX_LIGHTS; li++) {
     bool want = (mask >> li) & 1u;
     uint64_t bf = __ballot(want && first_hit), bl = __ballot(want && !first_hit);
     int32_t entry = slot * MCR_MAX_LIGHTS + li;
-    if (want && first_hit) w.sref[base_first + (uint32_t)__popcll(bf & below)] = entry;
-    if (want && !first_hit) w.sref[L - 1 - (base_later + (uint32_t)__popcll(bl & below))] = entry;
-    base_first += (uint32_t)__popcll(bf);
-    base_later += (uint32_t)__popcll(bl);
+    if (want && first_hit) w.sref[(size_t)li * P + got[2 * li] + (uint32_t)__popcll(bf & below)] = entry;
+    if (want && !first_hit)
+      w.sref[(size_t)(li + 1) * P - 1 - (got[2 * li + 1] + (uint32_t)__popcll(bl & below))] = entry;
   }
 }
 
@@ -1056,8 +1055,14 @@ __device__ inline bool visits_before(int2 a, int2 b) {
 template <bool MIX, bool WIDE, bool TEX, class Stack>
 MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &w, Stack &st,
                             double *seen, int2 *seen_meta, double2 *seen_uv, DevCounters *cnt) {
-  uint32_t n_first = w.counts[cw(C_SHADOW)];
-  uint32_t n = n_first + w.counts[cw(C_SHADOW_BACK)];
+  // The queue is the concatenation of its segments: per light, first-hit rays (front of the light's
+  // region of sref), then later ones (back of the region, in reverse).
+  uint32_t seg_end[2 * MCR_MAX_LIGHTS];
+  uint32_t n = 0;
+  for (int k = 0; k < 2 * MCR_MAX_LIGHTS; k++) {
+    n += w.counts[cw(C_SHADOW_SEG + k)];
+    seg_end[k] = n;
+  }
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nthreads = gridDim.x * blockDim.x;
   TravCount tc = {0, 0, 0};
@@ -1073,8 +1078,14 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
   for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
     uint32_t e = b + (threadIdx.x & 63);
     if (e >= n) continue;
-    // Entries [0, n_first) are at the front of sref, the rest at its back in reverse.
-    int32_t ref = e < n_first ? w.sref[e] : w.sref[RS - 1 - (e - n_first)];
+    uint32_t seg = 0, start = 0;
+    for (int k = 0; k < 2 * MCR_MAX_LIGHTS; k++)
+      if (e >= seg_end[k]) {
+        seg = (uint32_t)k + 1;
+        start = seg_end[k];
+      }
+    size_t region = (size_t)(seg >> 1) * P;
+    int32_t ref = (seg & 1) ? w.sref[region + P - 1 - (e - start)] : w.sref[region + (e - start)];
     int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
     size_t idx = (size_t)li * P + slot;
     double ox = nt_ld(&w.sray[0 * RS + idx]), oy = nt_ld(&w.sray[1 * RS + idx]), oz = nt_ld(&w.sray[2 * RS + idx]);

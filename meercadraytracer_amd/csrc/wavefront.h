@@ -59,7 +59,7 @@ enum LightStatus { LS_NONE = 0, LS_VISIBLE = 1, LS_RAY = 2, LS_BLOCKED = 3 };
 enum CountWord {
   C_QUEUE0 = 0,     // live slots in queue 0
   C_QUEUE1 = 1,     // live slots in queue 1
-  C_SHADOW = 2,     // shadow rays of first-hit vertices, queued by shade from the front of sref
+  C_SPARE = 2,      // unused
   C_OVERFLOW = 3,   // OVERFLOW_* bits: a shadow ray met more see-through hits than it can keep, or a
                     // wide traversal met a corrupt node or a full stack
   C_FREE = 4,       // slots on the free list
@@ -67,8 +67,12 @@ enum CountWord {
   C_SORT = 6,       // appends of the queue re-sort (k_sort_gather)
   C_EXTEND_FETCH = 7,                            // MCR_FETCH_COUNTERS words (WaveFetch)
   C_SHADOW_FETCH = C_EXTEND_FETCH + MCR_FETCH_COUNTERS,
-  C_SHADOW_BACK = C_SHADOW_FETCH + MCR_FETCH_COUNTERS,  // shadow rays of later vertices, from the back
-  C_WORDS,
+  // Shadow rays queued by shade, two words per light: word 2*li counts rays of first-hit vertices
+  // (stored from the front of light li's region of sref), word 2*li+1 rays of later vertices
+  // (from the back of that region).
+  C_SHADOW_SEG = C_SHADOW_FETCH + MCR_FETCH_COUNTERS,
+  C_SHADOW_SEG_END = C_SHADOW_SEG + 2 * MCR_MAX_LIGHTS,
+  C_WORDS = C_SHADOW_SEG_END,
 };
 
 enum OverflowBits { OVERFLOW_SEEN = 1, OVERFLOW_WIDE = 2 };
@@ -146,7 +150,7 @@ struct Wavefront {
   // queues
   int32_t *queue[2];         // live slots
   uint32_t *counts;          // CountWord
-  int32_t *sref;             // shadow queue: slot * MCR_MAX_LIGHTS + light
+  int32_t *sref;             // shadow queue, capacity entries per light: slot * MCR_MAX_LIGHTS + light
   int32_t *free_list;        // slots with no sample
   const Control *ctl;
 };
