@@ -65,17 +65,20 @@ def cpu_baseline(sc, cam, st, width, height, seconds=15.0):
 KERNEL_SYMBOLS = {'extend (closest hit)': 'k_extend', 'shadow (any hit)': 'k_shadow'}
 
 
-def _pmc_run(counter, bench_args, outdir):
-    """One rocprofv3 counter pass over a child bench run; returns the per-dispatch values of
-    every kernel, in dispatch order, as {kernel symbol: [KiB, ...]}."""
+def _pmc_run(counters, bench_args, outdir, timeout=240):
+    """One rocprofv3 counter pass (the counters of one hardware pass, space separated) over a
+    child bench run; returns {kernel symbol: {counter: [value per dispatch, ...]}} in dispatch
+    order."""
     import csv
     import subprocess
-    d = os.path.join(outdir, counter)
-    cmd = ['rocprofv3', '--pmc', counter, '--output-format', 'csv', '-d', d, '-o', 'pmc', '--',
+    names = counters.split()
+    d = os.path.abspath(os.path.join(outdir, '_'.join(names)))
+    os.makedirs(d, exist_ok=True)
+    cmd = ['rocprofv3', '--pmc', *names, '--output-format', 'csv', '-d', d, '-o', 'pmc', '--',
            sys.executable, os.path.abspath(__file__), *bench_args]
     env = dict(os.environ, TMPDIR='/tmp')
-    subprocess.run(cmd, cwd='/tmp', env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
-                   timeout=240, check=True)
+    with open(os.path.join(d, 'rocprofv3.log'), 'w') as logf:
+        subprocess.run(cmd, cwd='/tmp', env=env, stdout=logf, stderr=subprocess.STDOUT, timeout=timeout, check=True)
     rows = []
     for root, _, files in os.walk(d):
         for f in files:
@@ -84,7 +87,8 @@ def _pmc_run(counter, bench_args, outdir):
     rows.sort(key=lambda r: int(r['Dispatch_Id']))
     out = {}
     for r in rows:
-        out.setdefault(kernel_symbol(r['Kernel_Name']), []).append(float(r['Counter_Value']))
+        per = out.setdefault(kernel_symbol(r['Kernel_Name']), {})
+        per.setdefault(r['Counter_Name'], []).append(float(r['Counter_Value']))
     return out
 
 
@@ -112,9 +116,10 @@ def measure_traffic(args, kernel, launches):
     res = {}
     for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
         copy = _pmc_run(counter, copy_args, os.path.join(outdir, 'copy'))
-        copied_kib = sum(copy.get('__amd_rocclr_copyBuffer', []))
+        copied_kib = sum(copy.get('__amd_rocclr_copyBuffer', {}).get(counter, []))
         factor = (4096 * 1024) / copied_kib if copied_kib else None
-        vals = _pmc_run(counter, bench_args, outdir).get(KERNEL_SYMBOLS[kernel], [])[-launches:]
+        per = _pmc_run(counter, bench_args, outdir).get(KERNEL_SYMBOLS[kernel], {})
+        vals = (per.get(counter) or next(iter(per.values()), []))[-launches:]
         if not factor or len(vals) < launches:
             return None
         res[counter] = (sum(vals) * factor * 1024 / launches, factor)
@@ -122,6 +127,68 @@ def measure_traffic(args, kernel, launches):
     return {'bytes_per_launch': int(res['FETCH_SIZE'][0] + res['WRITE_SIZE'][0]),
             'read_bytes_per_launch': int(res['FETCH_SIZE'][0]), 'write_bytes_per_launch': int(res['WRITE_SIZE'][0]),
             'fetch_scale': round(res['FETCH_SIZE'][1], 4), 'write_scale': round(res['WRITE_SIZE'][1], 4)}
+
+
+# Counter passes of the traversal evidence: one rocprofv3 run each (a hardware pass holds at most
+# 8 SQ, 4 TCC and 2 GRBM counters, and derived metrics bring their own inputs, so each derived
+# metric runs alone). GRBM_GUI_ACTIVE (GPU cycles per dispatch) weights the per-dispatch
+# percentages into one figure per kernel.
+COUNTER_PASSES = ['GRBM_GUI_ACTIVE', 'TCC_HIT_sum TCC_MISS_sum', 'VALUBusy', 'VALUUtilization',
+                  'OccupancyPercent', 'MemUnitStalled']
+COUNTER_KERNELS = ('k_extend', 'k_shadow', 'k_shade', 'k_resolve')
+
+
+def kernel_counters(args, passes=None, outdir=None):
+    """L2 (TCC) hit rate, VALU busy and utilisation, occupancy and memory-unit stalls of the
+    wavefront kernels over one plain run of `args.steps` passes (every launch of the run),
+    each counter pass a rocprofv3 run of its own over the same command."""
+    import shutil
+    import tempfile
+    if not shutil.which('rocprofv3'):
+        return {'error': 'rocprofv3 not found'}
+    keep = outdir is not None
+    if keep:
+        os.makedirs(outdir, exist_ok=True)
+    else:
+        outdir = tempfile.mkdtemp(prefix='mcr_pmc_', dir='/tmp')
+    bench_args = ['--steps', str(args.steps), '--warmup', '0', '--no-cpu', '--no-traffic',
+                  '--no-batched', '--no-served', '--no-precision', '--no-textures',
+                  '--width', str(args.width), '--height', str(args.height)]
+    raw, errors = {}, {}
+    for counters in (passes or COUNTER_PASSES):
+        log(f'counter pass: {counters} ...')
+        try:
+            res = _pmc_run(counters, bench_args, outdir, timeout=180)
+        except Exception as e:  # an unknown counter on this ROCm fails its own pass only
+            errors[counters] = str(e)[:200]
+            continue
+        for k, per in res.items():
+            if k in COUNTER_KERNELS:
+                raw.setdefault(k, {}).update(per)
+    if not keep:
+        shutil.rmtree(outdir, ignore_errors=True)
+    out = {}
+    for k, per in raw.items():
+        cyc = per.get('GRBM_GUI_ACTIVE')
+        o = {'dispatches': max(len(v) for v in per.values())}
+        if cyc:
+            o['gpu_cycles_total'] = sum(cyc)
+        hit, miss = sum(per.get('TCC_HIT_sum', [])), sum(per.get('TCC_MISS_sum', []))
+        if hit + miss > 0:
+            o['l2_hit_pct'] = round(100.0 * hit / (hit + miss), 2)
+            o['l2_requests_per_dispatch'] = int((hit + miss) / len(per['TCC_HIT_sum']))
+        for name in ('VALUBusy', 'VALUUtilization', 'OccupancyPercent', 'MemUnitStalled'):
+            v = per.get(name)
+            if not v:
+                continue
+            if cyc and len(cyc) == len(v) and sum(cyc) > 0:
+                o[name] = round(sum(a * w for a, w in zip(v, cyc)) / sum(cyc), 2)
+            else:
+                o[name] = round(sum(v) / len(v), 2)
+        out[k] = o
+    return {'workload': f'plain_gpu, {args.width}x{args.height}, {args.steps} passes (every launch of the run)',
+            'kernels': out, 'failed_passes': errors,
+            'note': 'percentages are per-dispatch values weighted by GRBM_GUI_ACTIVE cycles of the same dispatch'}
 
 
 def c0(ctx):
@@ -200,8 +267,10 @@ def main():
     ap.add_argument('--served-slots', type=int, default=0, help='server path slots (0 = default)')
     ap.add_argument('--no-precision', action='store_true', help='skip the precision-form probe measurement')
     ap.add_argument('--no-textures', action='store_true', help='skip the textured-workload measurement')
-    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0', 'precision', 'textures'], default=None,
+    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0', 'precision', 'textures', 'counters'],
+                    default=None,
                     help='run one workload and print its JSON object only')
+    ap.add_argument('--pmc-dir', default=None, help='--only counters: keep the rocprofv3 output here')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB (for scaling HBM counters)')
     args = ap.parse_args()
@@ -219,6 +288,9 @@ def main():
               '(tiles shard across GPUs by the scheduler; only the one-GPU path is measured)', file=sys.stderr)
         sys.exit(2)
 
+    if args.only == 'counters':  # child runs only: this process does not touch the GPU
+        print(json.dumps(kernel_counters(args, outdir=args.pmc_dir)), flush=True)
+        return
     from meercadraytracer_amd import abi, native, scenes, workloads
     from meercadraytracer_amd.compile import compile_camera, compile_scene
     from meercadraytracer_amd.raytrace import render_passes, trace_settings
