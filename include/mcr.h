@@ -338,6 +338,25 @@ int mcr_intersect_any_count(mcr_ctx *ctx, mcr_scene *scene, const double *rays, 
                             double t_min, double t_max, uint32_t *count_out, int device_ptrs,
                             mcr_counters *counters);
 
+/* ---- the one entry point that allocates nothing --------------------------------------------- */
+/* No counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. It is
+ * mcr_intersect_closest (intersectClosest, src/bvh.ts:202-282, over a ray batch) on device
+ * pointers with the same results, except that it allocates nothing, waits for nothing and reports
+ * no counters: the kernels are queued on `stream` (a hipStream_t of the scene's device; NULL =
+ * the default stream) and the outputs are ready once that stream has run them. Its scratch (the
+ * traversal's spilled stack entries for a fixed grid, and its counters) is one static device
+ * array of MCR_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends and located when
+ * mcr_create first runs on that device. Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, two threads) race and must be serialised by the caller, for
+ * example by issuing them on one stream. n is at most MCR_NOALLOC_MAX_RAYS; larger batches are
+ * refused with MCR_E_INVALID. */
+#define MCR_NOALLOC_SCRATCH_BYTES 58729472u   /* 512 x 256 threads x 112 stack words + counters */
+#define MCR_NOALLOC_MAX_RAYS 0x80000000u
+int mcr_intersect_closest_noalloc(mcr_scene *scene, const double *rays, uint32_t n, double t_min,
+                                  double t_max, int cull, int32_t *tri_out, double *t_out,
+                                  double *u_out, double *v_out, uint8_t *front_out, void *stream);
+
 /* ---- precision form: compressed 8-wide BVH, FP32 watertight closest hit ------------------- */
 /* A second layout of a scene's geometry, built on the device from its binary BVH: 80-byte nodes
  * of eight children whose boxes are 8-bit grid coordinates over the node, rounded outward, and

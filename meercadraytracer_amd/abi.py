@@ -200,6 +200,7 @@ class Request(C.Structure):
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 MCR_E_CANCELLED, MCR_E_TIMEOUT, MCR_E_INTERNAL = -6, -7, -8
 MCR_ISECT_FP64, MCR_ISECT_WIDE_FP32 = 0, 1
+MCR_NOALLOC_SCRATCH_BYTES, MCR_NOALLOC_MAX_RAYS = 58729472, 0x80000000  # mcr_intersect_closest_noalloc
 
 
 class RequestStatus(C.Structure):

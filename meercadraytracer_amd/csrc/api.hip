@@ -154,6 +154,43 @@ int32_t child_ref(const int32_t *nodes, uint32_t node, const std::vector<int32_t
 }  // namespace
 
 // =================================================================================================
+// ---- scratch of the entry point that allocates nothing (mcr_intersect_closest_noalloc) ----------
+// The closest-hit probe kernel's scratch for a fixed grid of NOALLOC_GRID blocks of 256 threads:
+// the spilled part of every thread's traversal stack (MCR_STACK_MAX - MCR_STACK_LDS entries),
+// the ray counters (MCR_COUNTER_COPIES lines) and the interleaved work counters. Each GPU the
+// library runs on holds its own copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress gives on that device.
+const uint32_t NOALLOC_GRID = 512;
+const size_t NOALLOC_SPILL_BYTES = sizeof(int32_t) * (size_t)(MCR_STACK_MAX - MCR_STACK_LDS) * NOALLOC_GRID * 256;
+const size_t NOALLOC_BYTES = NOALLOC_SPILL_BYTES + COUNTER_BYTES + FETCH_BYTES;
+static_assert(NOALLOC_BYTES == MCR_NOALLOC_SCRATCH_BYTES, "mcr.h states the scratch size");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+const int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Clears the counter words of the scratch (a fill kernel, not a memset, so the call can be
+// captured into a graph).
+__global__ void k_noalloc_clear(uint32_t *words, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) words[i] = 0;
+}
+}  // namespace
+
+// Once per device, with that device current: the device address of its copy of `scratch`.
+static int noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(MCR_E_INVALID, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return MCR_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(MCR_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_noalloc_ptr[device] = p;
+  return MCR_OK;
+}
+
 extern "C" {
 
 const char *mcr_last_error(void) { return g_error.c_str(); }
@@ -188,6 +225,7 @@ int mcr_create(int device, mcr_ctx **out) {
     return fail(MCR_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
   }
   int rc = engine_init(c->eng, device, c->stream);
+  if (rc == MCR_OK) rc = noalloc_setup(device);
   if (rc) {
     std::string msg = g_error;
     mcr_destroy(c);
@@ -807,6 +845,39 @@ int mcr_intersect_any_count(mcr_ctx *c, mcr_scene *s, const double *rays, uint32
     counters->leaf_visits += hc.leaf_visits;
     counters->triangle_tests += hc.triangle_tests;
   }
+  return MCR_OK;
+}
+
+// As mcr_intersect_closest on device pointers, with no allocation and no wait (see mcr.h).
+int mcr_intersect_closest_noalloc(mcr_scene *s, const double *rays, uint32_t n, double t_min, double t_max,
+                                  int cull, int32_t *tri_out, double *t_out, double *u_out, double *v_out,
+                                  uint8_t *front_out, void *stream) {
+  if (!s || !s->ctx || (!rays && n)) return fail(MCR_E_INVALID, "null argument");
+  if (n > MCR_NOALLOC_MAX_RAYS) return fail(MCR_E_INVALID, "more than 2^31 rays in one call");
+  if (!s->has_geometry) return fail(MCR_E_NO_SCENE, "the scene has no geometry");
+  if (n == 0) return MCR_OK;
+  int device = s->ctx->device;
+  HIP_TRY(hipSetDevice(device));
+  void *p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    p = g_noalloc_ptr[device];
+  }
+  if (!p) {
+    MCR_TRY(noalloc_setup(device));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    p = g_noalloc_ptr[device];
+  }
+  hipStream_t sm = (hipStream_t)stream;
+  int32_t *spill = (int32_t *)p;
+  DevCounters *cnt = (DevCounters *)((char *)p + NOALLOC_SPILL_BYTES);
+  uint32_t *fetch = (uint32_t *)((char *)p + NOALLOC_SPILL_BYTES + COUNTER_BYTES);
+  // The counters start at zero on every call: the previous call on this GPU left its totals.
+  hipLaunchKernelGGL(k_noalloc_clear, dim3(4), dim3(256), 0, sm, (uint32_t *)cnt,
+                     (uint32_t)((COUNTER_BYTES + FETCH_BYTES) / sizeof(uint32_t)));
+  launch_probe_closest(sm, s->d, rays, n, t_min, t_max, cull, tri_out, t_out, u_out, v_out, front_out,
+                       std::min<uint32_t>(NOALLOC_GRID, (n + 255) / 256), spill, NOALLOC_GRID * 256, cnt, fetch);
+  HIP_TRY(hipGetLastError());
   return MCR_OK;
 }
 

@@ -71,6 +71,8 @@ def lib():
             'mcr_encode_tile64': (C.c_int, [P, P, C.c_uint32, abi.Tile, P, P, C.c_uint32, C.c_int]),
             'mcr_intersect_closest': (C.c_int, [P, P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P, P,
                                                 P, C.c_int, P]),
+            'mcr_intersect_closest_noalloc': (C.c_int, [P, P, C.c_uint32, C.c_double, C.c_double, C.c_int, P, P, P,
+                                                        P, P, P]),
             'mcr_scene_build_wide': (C.c_int, [P, C.POINTER(abi.WideInfo)]),
             'mcr_scene_wide_info': (C.c_int, [P, C.POINTER(abi.WideInfo)]),
             'mcr_intersect_closest_wide': (C.c_int, [P, P, P, C.c_uint32, C.c_float, C.c_float, C.c_int, P, P, P,

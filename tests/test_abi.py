@@ -67,3 +67,13 @@ def test_ctypes_layout_matches_header(tmp_path):
     subprocess.run(['gcc', '-std=c11', '-Wall', str(src), '-o', str(exe)], check=True, capture_output=True)
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert got == expect
+
+
+def test_noalloc_entry_refuses_null_scene_without_a_gpu():
+    """mcr_intersect_closest_noalloc checks its arguments before any HIP call."""
+    if not os.path.exists(native.LIB_PATH):
+        pytest.skip('libmcr.so is not built')
+    L = native.lib()
+    rc = L.mcr_intersect_closest_noalloc(None, None, 0, 0.0, 1.0, 1, None, None, None, None, None, None)
+    assert rc == abi.MCR_E_INVALID
+    assert b'null' in L.mcr_last_error()
