@@ -175,6 +175,78 @@ MCR_INLINE double box_entry_t(float bx0, float by0, float bz0, float bx1, float 
   return JM_INF;
 }
 
+// ---- the slab test decided in FP32 first -------------------------------------------------------
+// box_entry_t is the reference's FP64 test. Most boxes are far from the ray's boundary cases, and
+// for those an FP32 evaluation of the same slab distances, t' = fma(b, 1/d, -o/d), already settles
+// the outcome: with |o| and every box coordinate bounded by B (the root box bounds every node box),
+// |t' - t| <= 2^-22.99 (B + |o|) |1/d| per axis, where t is the FP64 value the reference computes
+// ((b - o) * inv rounded twice). The margin M = 2^-19 max over axes of (B + |o|) |1/d| is almost
+// sixteen times that bound. A box is called a hit or a miss in FP32 only when each of the test's
+// three comparisons (exit >= entry, exit >= tmin = 0, entry <= tmax) clears its threshold by the
+// margin; otherwise (a ray grazing a box face within ~M) the FP64 test decides. So every outcome,
+// and every near/far order the closest-hit traversal takes, is the reference's, while the common
+// case costs a third of the FP64 test. The traversals use it for rays with tmin = 0 (every ray
+// the renderer traces) whose inverse direction is finite and moderate.
+struct SlabF32 {
+  float ix, iy, iz;        // fl32(1/d)
+  float oix, oiy, oiz;     // fl32(o * (1/d)) of the FP64 product
+  float M;                 // the margin, rounded up
+  bool ok;                 // the pre-test applies (otherwise the FP64 test only)
+};
+
+// FP32 thresholds for the test entry <= t: an estimate at or below `below` is <= t in FP64, one
+// above `above` is > t in FP64.
+MCR_INLINE void slab_threshold(double t, float M, float &below, float &above) {
+  if (!(jm_fabs(t) < JM_INF)) {  // an unbounded ray: every finite estimate is below it
+    below = above = (float)t;
+    return;
+  }
+  double slack = jm_fabs(t) * 0x1p-20 + (double)M;  // the relative part absorbs the FP64 rounding of t -/+ M
+  below = __double2float_rd(t - slack);
+  above = __double2float_ru(t + slack);
+}
+
+MCR_INLINE SlabF32 slab_setup(const float *root_box, double ox, double oy, double oz, double ix, double iy,
+                              double iz) {
+  SlabF32 r;
+  float B = 0;
+  for (int k = 0; k < 6; k++) B = fmaxf(B, fabsf(root_box[k]));
+  double ax = jm_fabs(ix), ay = jm_fabs(iy), az = jm_fabs(iz);
+  double mx = ((double)B + jm_fabs(ox)) * ax, my = ((double)B + jm_fabs(oy)) * ay, mz = ((double)B + jm_fabs(oz)) * az;
+  double m = __builtin_fmax(__builtin_fmax(mx, my), mz) * 0x1p-19;
+  m = __builtin_fmax(m, 0x1p-100);
+  r.ok = ax <= 0x1p64 && ay <= 0x1p64 && az <= 0x1p64 && m <= 0x1p80;
+  r.ix = (float)ix;
+  r.iy = (float)iy;
+  r.iz = (float)iz;
+  r.oix = (float)(ox * ix);
+  r.oiy = (float)(oy * iy);
+  r.oiz = (float)(oz * iz);
+  r.M = __double2float_ru(m);
+  return r;
+}
+
+// 1: the FP64 test (tmin = 0) hits; 0: it misses; 2: too close to call. `lo` gets the FP32 entry
+// estimate (before clamping to 0). th / tm come from slab_threshold(tmax).
+MCR_INLINE int slab_class(float bx0, float by0, float bz0, float bx1, float by1, float bz1, const SlabF32 &r,
+                          float th, float tm, float &lo_out) {
+  float t0 = __builtin_fmaf(bx0, r.ix, -r.oix), t1 = __builtin_fmaf(bx1, r.ix, -r.oix);
+  float lo = __builtin_fminf(t0, t1), hi = __builtin_fmaxf(t0, t1);
+  t0 = __builtin_fmaf(by0, r.iy, -r.oiy);
+  t1 = __builtin_fmaf(by1, r.iy, -r.oiy);
+  lo = __builtin_fmaxf(lo, __builtin_fminf(t0, t1));
+  hi = __builtin_fminf(hi, __builtin_fmaxf(t0, t1));
+  t0 = __builtin_fmaf(bz0, r.iz, -r.oiz);
+  t1 = __builtin_fmaf(bz1, r.iz, -r.oiz);
+  lo = __builtin_fmaxf(lo, __builtin_fminf(t0, t1));
+  hi = __builtin_fminf(hi, __builtin_fmaxf(t0, t1));
+  lo_out = lo;
+  float d = hi - lo;
+  if (d >= r.M && hi >= r.M && lo <= th) return 1;
+  if (d < -r.M || hi < -r.M || lo > tm) return 0;
+  return 2;
+}
+
 __device__ inline bool finite_inverse(double ix, double iy, double iz) {
   return jm_fabs(ix) < JM_INF && jm_fabs(iy) < JM_INF && jm_fabs(iz) < JM_INF;
 }
@@ -197,6 +269,16 @@ struct LaneStack {
     return spill[(size_t)(sp - MCR_STACK_LDS) * spill_stride];
   }
 };
+
+// Declares a traversal kernel's per-lane stack: MCR_STACK_LDS entries per lane in LDS (blocks of
+// 256 threads), the rest in the lane's part of the global spill area.
+#define MCR_LANE_STACK(st, spill_base, spill_stride_)                     \
+  __shared__ int32_t lds_stack_[MCR_STACK_LDS * 256];                     \
+  LaneStack st;                                                          \
+  st.lds = lds_stack_ + threadIdx.x;                                      \
+  st.stride = 256;                                                        \
+  st.spill = (spill_base) + (blockIdx.x * blockDim.x + threadIdx.x);      \
+  st.spill_stride = (int)(spill_stride_)
 
 struct TriHit {
   int tri;
@@ -232,8 +314,29 @@ MCR_INLINE bool mt_test(const float4 a, const float4 b, const float4 c, double o
   return true;
 }
 
+// How a traversal evaluates the reference's FP64 slab test (bvh.ts:340-356). Every mode gives the
+// reference's outcomes; they differ in cost and in the rays they apply to.
+enum SlabMode {
+  SLAB_JS = 0,    // FP64 with JS Math.min/max: rays whose inverse direction overflows (NaN possible)
+  SLAB_FP64 = 1,  // FP64 with the hardware's min/max (identical without NaN)
+  SLAB_FP32 = 2,  // the FP32 pre-test, FP64 where it cannot decide (slab_setup's ok holds)
+};
+
+// The FP64 inverse direction for the FP64 test: SLAB_FP32 traversals recompute it where an FP32
+// outcome was too close to call (rare), which gives the same bits and keeps six registers free
+// over the loop. `salt` varies with the loop (a box word): adding 0 * salt (a signed zero; every
+// component of d is nonzero in this mode) leaves d unchanged but ties the divisions to the loop,
+// so the compiler does not hoist them out and keep their results live after all.
+MCR_INLINE void fresh_inverse(double dx, double dy, double dz, int32_t salt, double &ix, double &iy, double &iz) {
+  double z = 0.0 * (double)salt;
+  ix = safe_inverse(dx + z);
+  iy = safe_inverse(dy + z);
+  iz = safe_inverse(dz + z);
+}
+
 // bvh.ts:202-282 over the device layout. cull: drop back faces of FrontSide and front faces of
-// BackSide triangles.
+// BackSide triangles. Near/far orders use the FP64 entry distances (exact-t ties between
+// triangles are broken by visit order, so the order must be the reference's).
 template <bool EXACT>
 MCR_INLINE bool closest_impl(const DevScene &s, double ox, double oy, double oz, double dx,
                                     double dy, double dz, double ix, double iy, double iz,
@@ -312,17 +415,50 @@ MCR_INLINE bool intersect_closest(const DevScene &s, double ox, double oy, doubl
   return closest_impl<true>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, cull, hit, st, cnt);
 }
 
+// The pre-test's parameters for an any-hit ray (tmin = 0), or ok = false where only FP64 applies.
+MCR_INLINE SlabF32 slab_for(const DevScene &s, double ox, double oy, double oz, double ix, double iy, double iz,
+                            double tmin) {
+  SlabF32 sl;
+  sl.ok = false;
+  if (tmin == 0 && finite_inverse(ix, iy, iz)) sl = slab_setup(s.root_box, ox, oy, oz, ix, iy, iz);
+  return sl;
+}
+
+MCR_INLINE int slab_mode(const SlabF32 &sl, double ix, double iy, double iz) {
+  if (sl.ok) return SLAB_FP32;
+  return finite_inverse(ix, iy, iz) ? SLAB_FP64 : SLAB_JS;
+}
+
+// Outcome of one child box of an any-hit traversal (fixed tmin, tmax); `entry` estimates where
+// the ray enters it (used to order visits only).
+template <int MODE>
+MCR_INLINE bool child_hit(float bx0, float by0, float bz0, float bx1, float by1, float bz1, const SlabF32 &sl,
+                          float th, float tm, double ox, double oy, double oz, double dx, double dy, double dz,
+                          double ix, double iy, double iz, double tmin, double tmax, float &entry) {
+  if (MODE == SLAB_FP32) {
+    int c = slab_class(bx0, by0, bz0, bx1, by1, bz1, sl, th, tm, entry);
+    if (c != 2) return c == 1;
+    fresh_inverse(dx, dy, dz, __float_as_int(bx0), ix, iy, iz);
+  }
+  double t = box_entry_t<MODE == SLAB_JS>(bx0, by0, bz0, bx1, by1, bz1, ox, oy, oz, ix, iy, iz, tmin, tmax);
+  entry = (float)t;
+  return t < JM_INF;
+}
+
 // bvh.ts:289-332 over the device layout: every hit in (tmin, tmax), no culling, in the
 // reference's depth-first order (right child first). Both child boxes are tested when their parent
 // is visited; a box test does not depend on when it runs, so the visit order is unchanged.
 // visit(k, t, u, v, front) returns false to stop.
-template <bool EXACT, class Visit>
+template <int MODE, class Visit>
 MCR_INLINE void any_impl(const DevScene &s, double ox, double oy, double oz, double dx,
-                                double dy, double dz, double ix, double iy, double iz, double tmin,
-                                double tmax, LaneStack &st, TravCount &cnt, Visit &visit) {
+                                double dy, double dz, double ix, double iy, double iz, const SlabF32 &sl,
+                                double tmin, double tmax, LaneStack &st, TravCount &cnt, Visit &visit) {
   const float *rb = s.root_box;
-  if (!(box_entry_t<EXACT>(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF))
+  if (!(box_entry_t<MODE == SLAB_JS>(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) <
+        JM_INF))
     return;
+  float th = 0, tm = 0;
+  if (MODE == SLAB_FP32) slab_threshold(tmax, sl.M, th, tm);
   st.sp = 0;
   int32_t ref = s.root_ref;
   for (;;) {
@@ -343,8 +479,11 @@ MCR_INLINE void any_impl(const DevScene &s, double ox, double oy, double oz, dou
       const float4 *r = s.inner + (size_t)ref * 4;
       float4 p = r[0], q = r[1], w = r[2];
       int4 m = *reinterpret_cast<const int4 *>(r + 3);
-      bool hl = box_entry_t<EXACT>(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
-      bool hr = box_entry_t<EXACT>(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF;
+      float el, er;
+      bool hl = child_hit<MODE>(p.x, p.y, p.z, p.w, q.x, q.y, sl, th, tm, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin,
+                                tmax, el);
+      bool hr = child_hit<MODE>(q.z, q.w, w.x, w.y, w.z, w.w, sl, th, tm, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin,
+                                tmax, er);
       if (hr) {
         if (hl) st.push(m.x);
         ref = m.y;
@@ -366,28 +505,34 @@ MCR_INLINE void intersect_any(const DevScene &s, double ox, double oy, double oz
                                      TravCount &cnt, Visit &visit) {
   if (s.root_ref == -1) return;
   double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
-  if (finite_inverse(ix, iy, iz))
-    any_impl<false>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, visit);
-  else
-    any_impl<true>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, visit);
+  SlabF32 sl = slab_for(s, ox, oy, oz, ix, iy, iz, tmin);
+  switch (slab_mode(sl, ix, iy, iz)) {
+    case SLAB_FP32: any_impl<SLAB_FP32>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, sl, tmin, tmax, st, cnt, visit); break;
+    case SLAB_FP64: any_impl<SLAB_FP64>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, sl, tmin, tmax, st, cnt, visit); break;
+    default: any_impl<SLAB_JS>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, sl, tmin, tmax, st, cnt, visit);
+  }
 }
 
 // Shadow-ray query (tracer.ts:529-594 over bvh.ts:289-332). The reference visits every hit in
 // (tmin, tmax) in depth-first order, right child first, and stops at the first opaque one. The
 // set of triangles it can visit depends only on box tests with the fixed (tmin, tmax), never on
 // the order, and an opaque hit anywhere in that set makes the light invisible whatever came
-// before it. So this traversal takes the nearer child first and stops at the first opaque hit;
-// see-through hits are handed to on_hit with their leaf's first triangle, so the caller can replay
-// them in the reference's order (leaves by descending first triangle, then ascending index).
+// before it. So this traversal takes the nearer child first (by entry estimates, which only order
+// the visits) and stops at the first opaque hit; see-through hits are handed to on_hit with their
+// leaf's first triangle, so the caller can replay them in the reference's order (leaves by
+// descending first triangle, then ascending index).
 // on_hit(k, leaf_first, t, u, v, front, record) returns true for an opaque hit.
-template <bool EXACT, class OnHit>
+template <int MODE, class OnHit>
 MCR_INLINE bool occluded_impl(const DevScene &s, double ox, double oy, double oz, double dx,
                                      double dy, double dz, double ix, double iy, double iz,
-                                     double tmin, double tmax, LaneStack &st, TravCount &cnt,
-                                     OnHit &on_hit) {
+                                     const SlabF32 &sl, double tmin, double tmax,
+                                     LaneStack &st, TravCount &cnt, OnHit &on_hit) {
   const float *rb = s.root_box;
-  if (!(box_entry_t<EXACT>(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) < JM_INF))
+  if (!(box_entry_t<MODE == SLAB_JS>(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ox, oy, oz, ix, iy, iz, tmin, tmax) <
+        JM_INF))
     return false;
+  float th = 0, tm = 0;
+  if (MODE == SLAB_FP32) slab_threshold(tmax, sl.M, th, tm);
   st.sp = 0;
   int32_t ref = s.root_ref;
   for (;;) {
@@ -408,11 +553,13 @@ MCR_INLINE bool occluded_impl(const DevScene &s, double ox, double oy, double oz
       const float4 *r = s.inner + (size_t)ref * 4;
       float4 p = r[0], q = r[1], w = r[2];
       int4 m = *reinterpret_cast<const int4 *>(r + 3);
-      double tl = box_entry_t<EXACT>(p.x, p.y, p.z, p.w, q.x, q.y, ox, oy, oz, ix, iy, iz, tmin, tmax);
-      double tr = box_entry_t<EXACT>(q.z, q.w, w.x, w.y, w.z, w.w, ox, oy, oz, ix, iy, iz, tmin, tmax);
-      bool hl = tl < JM_INF, hr = tr < JM_INF;
+      float el, er;
+      bool hl = child_hit<MODE>(p.x, p.y, p.z, p.w, q.x, q.y, sl, th, tm, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin,
+                                tmax, el);
+      bool hr = child_hit<MODE>(q.z, q.w, w.x, w.y, w.z, w.w, sl, th, tm, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin,
+                                tmax, er);
       if (hl && hr) {
-        bool left_first = tl <= tr;
+        bool left_first = el <= er;
         st.push(left_first ? m.y : m.x);
         ref = left_first ? m.x : m.y;
         continue;
@@ -438,9 +585,15 @@ MCR_INLINE bool occluded(const DevScene &s, double ox, double oy, double oz, dou
                                 TravCount &cnt, OnHit &on_hit) {
   if (s.root_ref == -1) return false;
   double ix = safe_inverse(dx), iy = safe_inverse(dy), iz = safe_inverse(dz);
-  if (finite_inverse(ix, iy, iz))
-    return occluded_impl<false>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, on_hit);
-  return occluded_impl<true>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax, st, cnt, on_hit);
+  SlabF32 sl = slab_for(s, ox, oy, oz, ix, iy, iz, tmin);
+  switch (slab_mode(sl, ix, iy, iz)) {
+    case SLAB_FP32:
+      return occluded_impl<SLAB_FP32>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, sl, tmin, tmax, st, cnt, on_hit);
+    case SLAB_FP64:
+      return occluded_impl<SLAB_FP64>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, sl, tmin, tmax, st, cnt, on_hit);
+    default:
+      return occluded_impl<SLAB_JS>(s, ox, oy, oz, dx, dy, dz, ix, iy, iz, sl, tmin, tmax, st, cnt, on_hit);
+  }
 }
 
 // Sum over the 64 lanes of a wave.

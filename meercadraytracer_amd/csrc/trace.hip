@@ -270,14 +270,9 @@ template <bool MIX>
 __global__ __launch_bounds__(256, MIX ? 4 : 5) void k_extend(const ViewDev *__restrict__ views, Wavefront w, int q,
                                                 int32_t *spill, uint32_t spill_stride,
                                                 DevCounters *cnt) {
-  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid < 2 * MCR_MAX_LIGHTS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;  // shade of this iteration appends
-  LaneStack st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = 256;
-  st.spill = spill + gid;
-  st.spill_stride = (int)spill_stride;
+  MCR_LANE_STACK(st, spill, spill_stride);
   extend_loop<MIX, false>(views, w, q, st, cnt);
 }
 
@@ -1018,10 +1013,10 @@ template <bool TEX>
 struct ShadowCollector {
   const DevScene *sc;
   bool castOnly;
-  double *t;        // this thread's records, stride `stride`
+  double *t;        // records of every traversal thread, this thread's at lane + i * stride
   int2 *meta;       // (leaf first triangle, triangle index | front << 31)
   double2 *uv;      // the hit's barycentrics, kept for textured materials (null without textures)
-  size_t stride;
+  uint32_t lane, stride;
   int n;
   bool overflow;
   __device__ bool operator()(int k, int leaf_first, double tk, double u, double v, bool front, const float4 &c) {
@@ -1034,9 +1029,10 @@ struct ShadowCollector {
     // the replay does: the hit is kept here and dropped there, outside the traversal loop.
     if (!textured && m.alpha_test > 0 && m.opacity < m.alpha_test) return false;
     if (n < MCR_SHADOW_HIT_CAP) {
-      t[(size_t)n * stride] = tk;
-      meta[(size_t)n * stride] = make_int2(leaf_first, (int)((uint32_t)k | (front ? 0x80000000u : 0u)));
-      if (textured) uv[(size_t)n * stride] = make_double2(u, v);
+      size_t at = lane + (size_t)n * stride;
+      t[at] = tk;
+      meta[at] = make_int2(leaf_first, (int)((uint32_t)k | (front ? 0x80000000u : 0u)));
+      if (textured) uv[at] = make_double2(u, v);
       n++;
     } else {
       overflow = true;
@@ -1072,7 +1068,7 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
   bool overflow = false, bad = false;
   double *T = seen + gid;
   int2 *M = seen_meta + gid;
-  double2 *UV = TEX ? seen_uv + gid : nullptr;
+  double2 *UV = TEX ? seen_uv + gid : nullptr;  // the replay's view of this thread's records
   const size_t S = nthreads;
   WaveFetch fetch(&w.counts[cw(C_SHADOW_FETCH)], n);
   for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
@@ -1095,8 +1091,8 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
     const DevScene &sc = V.sc;
     // The environment-diffuse ray (slot light_slots) is transmittance(..., false) (tracer.ts:346).
-    ShadowCollector<TEX> col{&sc, V.st.shadows == MCR_SHADOWS_CAST && (uint32_t)li != V.light_slots, T, M, UV, S, 0,
-                        false};
+    ShadowCollector<TEX> col{&sc, V.st.shadows == MCR_SHADOWS_CAST && (uint32_t)li != V.light_slots, seen, seen_meta,
+                             TEX ? seen_uv : nullptr, gid, (uint32_t)S, 0, false};
     bool blocked;
     if constexpr (WIDE) {
       WideRay wr;
@@ -1186,17 +1182,12 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
 }
 
 template <bool MIX, bool TEX>
-__global__ __launch_bounds__(256) void k_shadow(const ViewDev *__restrict__ views, Wavefront w,
+__global__ __launch_bounds__(256, 5) void k_shadow(const ViewDev *__restrict__ views, Wavefront w,
                                                 int32_t *spill, uint32_t spill_stride,
                                                 double *seen, int2 *seen_meta, double2 *seen_uv,
                                                 DevCounters *cnt) {
-  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  LaneStack st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = 256;
-  st.spill = spill + gid;
-  st.spill_stride = (int)spill_stride;
+  MCR_LANE_STACK(st, spill, spill_stride);
   shadow_loop<MIX, false, TEX>(views, w, st, seen, seen_meta, seen_uv, cnt);
 }
 
@@ -1389,13 +1380,8 @@ __global__ __launch_bounds__(256) void k_probe_closest(DevScene sc, const double
                                                        uint8_t *front, int32_t *spill,
                                                        uint32_t spill_stride, DevCounters *cnt,
                                                        uint32_t *fetch_counters) {
-  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  LaneStack st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = 256;
-  st.spill = spill + gid;
-  st.spill_stride = (int)spill_stride;
+  MCR_LANE_STACK(st, spill, spill_stride);
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
   WaveFetch fetch(fetch_counters, n);
@@ -1427,13 +1413,8 @@ __global__ __launch_bounds__(256) void k_probe_any_count(DevScene sc, const doub
                                                          double tmin, double tmax, uint32_t *count,
                                                          int32_t *spill, uint32_t spill_stride,
                                                          DevCounters *cnt, uint32_t *fetch_counters) {
-  __shared__ int32_t lds_stack[MCR_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  LaneStack st;
-  st.lds = lds_stack + threadIdx.x;
-  st.stride = 256;
-  st.spill = spill + gid;
-  st.spill_stride = (int)spill_stride;
+  MCR_LANE_STACK(st, spill, spill_stride);
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
   WaveFetch fetch(fetch_counters, n);

@@ -111,3 +111,47 @@ def test_bvh_reference_cases_on_gpu(ctx):
     hit = tri >= 0
     assert hit.sum() > 100 and np.all(np.abs(t[hit] - bt[hit]) < 5e-4)
     assert np.array_equal(s.intersect_any_count(rays), ball)
+
+
+def _grazing_rays(g, n, seed):
+    """Rays aimed exactly at corners of BVH node boxes and at triangle vertices, some starting on a
+    box face and some ending (tmax) at the corner: the cases where an FP32 slab estimate cannot
+    decide and the FP64 test must."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = g.bvh.bounds.reshape(-1, 6).astype(np.float64)
+    tv = g.triangles.reshape(-1, 9).astype(np.float64)[:, :3]
+    idx = rng.integers(0, len(b), n)
+    corner = np.where(rng.integers(0, 2, (n, 3)) == 1, b[idx, 3:], b[idx, :3])
+    verts = tv[rng.integers(0, len(tv), n)]
+    target = np.where(rng.integers(0, 2, (n, 1)) == 1, corner, verts)
+    o = target + rng.normal(0, 5, (n, 3))
+    on_face = rng.integers(0, 4, n) == 0  # start on a face of the box: one coordinate copied
+    ax = rng.integers(0, 3, n)
+    o[on_face, ax[on_face]] = corner[on_face, ax[on_face]]
+    d = target - o
+    dist = np.linalg.norm(d, axis=1)
+    d /= dist[:, None]
+    return np.ascontiguousarray(np.concatenate([o, d], 1)), dist
+
+
+@pytest.mark.parametrize('case', ['lcg3000', 'soup20k', 'clustered', 'flat'])
+def test_any_count_exact_grazing(ctx, case):
+    """Any-hit traversals decide most boxes with an FP32 estimate and hand the rest to the FP64
+    test; rays that graze box corners and faces must still visit exactly the reference's hits."""
+    tris, n = BVH_CASES[case]()
+    g, _ = geometry_from(tris, n, oracle.build_bvh)
+    s = _scene(ctx, g)
+    rays, dist = _grazing_rays(g, 6000, 7)
+    for tmax in (np.inf, 20.0):
+        gc = s.intersect_any_count(rays, 0.0, tmax)
+        oc, _ = oracle.intersect_any_count(g, rays, 0.0, tmax)
+        assert np.array_equal(gc, oc), (case, tmax, int((gc != oc).sum()))
+    # tmax at the aimed point: the box whose corner it is sits on the boundary of the segment.
+    for k in range(0, 6000, 750):
+        gc = s.intersect_any_count(rays[k:k + 750], 0.0, float(dist[k]))
+        oc, _ = oracle.intersect_any_count(g, rays[k:k + 750], 0.0, float(dist[k]))
+        assert np.array_equal(gc, oc)
+    gt, gtt, *_ = s.intersect_closest(rays, cull=False)
+    ot, ott, *_ = oracle.intersect_closest(g, rays, cull=False)
+    assert np.array_equal(gt, ot) and np.array_equal(gtt[ot >= 0].view(np.uint64), ott[ot >= 0].view(np.uint64))
+    s.release()
