@@ -132,8 +132,10 @@ typedef struct mcr_light {
 
 /* Everything of CompiledScene but the geometry (protocol.ts:19-27 ShadingPayload), with the
  * textures message's arrays (protocol.ts:31-35). Texture and environment indices are checked
- * against the counts; at most 15 punctual lights when environments are present (one light slot
- * carries the environment-diffuse ray of tracer.ts:343-353). */
+ * against the counts; at most MCR_MAX_LIGHTS_PER_SCENE lights (the lights array, ambient and hemisphere
+ * entries included), one fewer when environments are present (one light slot carries the
+ * environment-diffuse ray of tracer.ts:343-353). */
+#define MCR_MAX_LIGHTS_PER_SCENE 64
 typedef struct mcr_shading {
   uint32_t material_count, light_count;
   const mcr_material *materials;
@@ -272,6 +274,8 @@ typedef struct mcr_server_config {
   uint32_t window;          /* unstarted samples kept queued ahead of the slots; 0 = slots */
   uint32_t max_requests;    /* admitted requests at once; 0 = 64 */
   int32_t use_graph;        /* replay captured bounce iterations (1) or launch directly (0) */
+  uint32_t max_lights;      /* light slots per path: the most lights (plus 1 with environments) a
+                               scene added to the server may have; 0 = 16, at most MCR_MAX_LIGHTS_PER_SCENE */
 } mcr_server_config;
 typedef struct mcr_request {
   mcr_scene *scene;         /* added with mcr_server_add_scene */

@@ -259,6 +259,7 @@ int mcr_set_wavefront_capacity(mcr_ctx *c, uint32_t slots) {
     hipFree(e.w_block);
     e.w_block = nullptr;
     e.capacity = 0;
+    e.lights = 0;
   }
   return MCR_OK;
 }
@@ -588,7 +589,7 @@ ViewDev make_view(const mcr_scene *s, const mcr_camera &cam, const mcr_settings 
   v.wide = s->wide;
   v.cam = cam;
   v.st = st;
-  v.light_slots = std::min<uint32_t>(s->d.light_count, MCR_MAX_LIGHTS);
+  v.light_slots = s->d.light_count;  // at most MCR_MAX_LIGHTS (checked on upload)
   v.shadow_slots = v.light_slots + (s->d.environment_count ? 1 : 0);
   v.shadow_tex = s->shadow_tex ? 1 : 0;
   return v;

@@ -40,10 +40,14 @@ hipError_t dalloc(T **p, size_t count) {
 // Grid of persistent traversal blocks: enough waves to fill 256 CUs several times over.
 const uint32_t TRAVERSE_GRID = 256 * 5;
 const uint32_t TRAVERSE_THREADS = TRAVERSE_GRID * 256;
-// Path slots (2,081 bytes each with their light slots, so 2^23 slots hold 17.5 GB). More slots
-// keep more waves per kernel busy: a 1080p, 64 spp render of 1.38M triangles runs 1,604, 1,688,
-// 1,798 and 1,812 Mrays/s at 2M, 3M, 8M and 12M slots.
+// Path slots (337 bytes each plus 109 bytes per light slot: 2^23 slots of a five-light scene hold
+// 7.4 GB). More slots keep more waves per kernel busy: a 1080p, 64 spp render of 1.38M triangles
+// runs 1,604, 1,688, 1,798 and 1,812 Mrays/s at 2M, 3M, 8M and 12M slots.
 const uint32_t DEFAULT_CAPACITY = 1u << 23;
+// Light slots take 109 bytes each per path slot (unshadowed term, shadow ray and colour, status,
+// queue entry); scenes with many lights get fewer path slots so that they stay within 24 GB.
+const uint64_t LIGHT_SLOT_BYTES = 109;
+const uint64_t LIGHT_SLOT_BUDGET = 24ull << 30;
 // A server holds all its slots from the start and its latency grows with them: 2^21 by default.
 const uint32_t SERVER_DEFAULT_SLOTS = 1u << 21;
 
@@ -55,6 +59,7 @@ struct Engine {
   Wavefront w = {};
   void *w_block = nullptr;
   uint32_t capacity = 0;
+  uint32_t lights = 0;             // light slots per path slot of the allocated wavefront
   uint32_t capacity_limit = DEFAULT_CAPACITY;
   int32_t *spill = nullptr;        // traversal stack overflow, per traversal thread
   double *seen = nullptr;          // see-through shadow hits, per traversal thread
@@ -87,8 +92,11 @@ struct Engine {
 
 int engine_init(Engine &e, int device, hipStream_t stream);
 void engine_free(Engine &e);
-// Path slots, views and jobs the next run needs (buffers only grow; the capacity is clamped).
-int engine_reserve(Engine &e, uint64_t samples, uint32_t nviews, uint32_t njobs, bool track_done);
+// Path slots, light slots, views and jobs the next run needs (buffers only grow; the capacity is
+// clamped to capacity_limit and to the light-slot budget).
+int engine_reserve(Engine &e, uint64_t samples, uint32_t nviews, uint32_t njobs, bool track_done, uint32_t lights);
+// The path slots a wavefront with `lights` light slots per path slot may hold.
+uint32_t engine_capacity_for(const Engine &e, uint32_t lights);
 // Host arrays must stay alive and unchanged until the next engine_sync.
 int engine_upload_views(Engine &e, const ViewDev *views, uint32_t n);
 int engine_upload_jobs(Engine &e, const JobDev *jobs, uint32_t first, uint32_t n);

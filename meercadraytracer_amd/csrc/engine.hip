@@ -12,13 +12,16 @@ namespace {
 
 uint32_t blocks_of(uint64_t n) { return (uint32_t)((n + 255) / 256); }
 
-int alloc_wavefront(Engine &e, uint32_t capacity) {
+int alloc_wavefront(Engine &e, uint32_t capacity, uint32_t lights) {
   hipFree(e.w_block);
   e.w_block = nullptr;
   e.capacity = 0;
+  e.lights = 0;
   engine_drop_graphs(e);
   size_t P = capacity;
-  size_t L = (size_t)MCR_MAX_LIGHTS * P;
+  size_t L = (size_t)lights * P;
+  uint32_t regions = std::min<uint32_t>(lights, MCR_SHADOW_REGIONS);
+  uint32_t region_cap = (uint32_t)(((lights + MCR_SHADOW_REGIONS - 1) / MCR_SHADOW_REGIONS) * P);
   size_t bytes = 0;
   auto take = [&](size_t n) {
     size_t off = (bytes + 255) / 256 * 256;
@@ -35,12 +38,15 @@ int alloc_wavefront(Engine &e, uint32_t capacity) {
   size_t o_q0 = take(sizeof(int32_t) * P);
   size_t o_q1 = take(sizeof(int32_t) * P);
   size_t o_counts = take(sizeof(uint32_t) * COUNT_WORDS_ALLOC);
-  size_t o_sref = take(sizeof(int32_t) * L);
+  size_t o_sref = take(sizeof(uint32_t) * regions * (size_t)region_cap);
   size_t o_free = take(sizeof(int32_t) * P);
+  size_t o_deep = take(sizeof(double) * (size_t)MCR_DEEP_HITS * MCR_DEEP_THREADS);
   HIP_TRY(hipMalloc(&e.w_block, bytes));
   char *b = (char *)e.w_block;
   Wavefront &w = e.w;
   w.capacity = capacity;
+  w.lights = lights;
+  w.region_cap = region_cap;
   w.f = (double *)(b + o_f);
   w.i = (int32_t *)(b + o_i);
   w.mode = (uint8_t *)(b + o_mode);
@@ -51,10 +57,12 @@ int alloc_wavefront(Engine &e, uint32_t capacity) {
   w.queue[0] = (int32_t *)(b + o_q0);
   w.queue[1] = (int32_t *)(b + o_q1);
   w.counts = (uint32_t *)(b + o_counts);
-  w.sref = (int32_t *)(b + o_sref);
+  w.sref = (uint32_t *)(b + o_sref);
   w.free_list = (int32_t *)(b + o_free);
+  w.deep_seen = (double *)(b + o_deep);
   w.ctl = e.ctl;
   e.capacity = capacity;
+  e.lights = lights;
   return MCR_OK;
 }
 
@@ -122,11 +130,18 @@ void engine_free(Engine &e) {
   e = Engine();
 }
 
-int engine_reserve(Engine &e, uint64_t samples, uint32_t nviews, uint32_t njobs, bool track_done) {
-  uint32_t cap = (uint32_t)std::min<uint64_t>(samples, e.capacity_limit);
+uint32_t engine_capacity_for(const Engine &e, uint32_t lights) {
+  uint64_t by_lights = LIGHT_SLOT_BUDGET / ((uint64_t)std::max<uint32_t>(lights, 1) * LIGHT_SLOT_BYTES);
+  return (uint32_t)std::max<uint64_t>(256, std::min<uint64_t>(e.capacity_limit, by_lights));
+}
+
+int engine_reserve(Engine &e, uint64_t samples, uint32_t nviews, uint32_t njobs, bool track_done, uint32_t lights) {
+  lights = std::max<uint32_t>(lights, 1);
+  uint32_t limit = engine_capacity_for(e, std::max(lights, e.lights));
+  uint32_t cap = (uint32_t)std::min<uint64_t>(samples, limit);
   cap = std::max<uint32_t>(cap, 256);
-  cap = std::min(std::max(cap, e.capacity), e.capacity_limit);
-  if (e.capacity < cap) MCR_TRY(alloc_wavefront(e, cap));
+  cap = std::min(std::max(cap, e.capacity), limit);
+  if (e.capacity < cap || e.lights < lights) MCR_TRY(alloc_wavefront(e, cap, std::max(lights, e.lights)));
   const void *old_views = e.views;
   MCR_TRY(grow(e.views, e.views_cap, std::max<uint32_t>(nviews, 1)));
   if (old_views != e.views) engine_drop_graphs(e);
@@ -188,9 +203,11 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
   launch_shade(sm, e.mix, e.views, w, q, live, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[2], sm));
   uint64_t max_shadow = (uint64_t)live * max_lights;
-  if (max_shadow)
+  if (max_shadow) {
     launch_shadow(sm, e.mix, e.wide, e.tex, e.views, w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(max_shadow)), e.spill,
                   TRAVERSE_THREADS, e.seen, e.seen_meta, e.seen_uv, e.counters);
+    launch_shadow_deep(sm, e.mix, e.tex, e.views, w, e.spill, TRAVERSE_THREADS, e.counters);
+  }
   if (timed) HIP_TRY(hipEventRecord(e.ev[3], sm));
   launch_resolve(sm, e.mix, e.views, w, q, live);
   HIP_TRY(hipGetLastError());
@@ -231,9 +248,11 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
         launch_extend(cs, e.mix, e.wide, e.views, e.w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(cap)), e.spill,
                       TRAVERSE_THREADS, e.counters);
         launch_shade(cs, e.mix, e.views, e.w, q, cap, e.counters);
-        if (max_lights)
+        if (max_lights) {
           launch_shadow(cs, e.mix, e.wide, e.tex, e.views, e.w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of((uint64_t)cap * max_lights)),
                         e.spill, TRAVERSE_THREADS, e.seen, e.seen_meta, e.seen_uv, e.counters);
+          launch_shadow_deep(cs, e.mix, e.tex, e.views, e.w, e.spill, TRAVERSE_THREADS, e.counters);
+        }
         launch_resolve(cs, e.mix, e.views, e.w, q, cap);
       }
       err = hipStreamEndCapture(cs, &g);
@@ -262,8 +281,8 @@ int engine_check(Engine &e) {
   if (o & OVERFLOW_WIDE)
     return mcr_fail(MCR_E_INTERNAL, "a wide-BVH traversal met a corrupt node or overflowed its stack");
   if (o)
-    return mcr_fail(MCR_E_OVERFLOW,
-                    "a shadow ray crossed more see-through surfaces than the per-ray list holds (64)");
+    return mcr_fail(MCR_E_OVERFLOW, "a shadow ray crossed more than " + std::to_string(MCR_DEEP_HITS) +
+                                        " counted see-through surfaces");
   return MCR_OK;
 }
 
@@ -313,13 +332,13 @@ int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<J
                uint32_t total, float *out, mcr_counters *counters) {
   if (jobs.empty() || total == 0) return MCR_OK;
   uint32_t nviews = (uint32_t)views.size(), njobs = (uint32_t)jobs.size();
-  MCR_TRY(engine_reserve(e, total, nviews, njobs, false));
   uint32_t max_lights = 0;
   e.tex = false;
   for (const ViewDev &v : views) {
     max_lights = std::max(max_lights, v.shadow_slots);
     e.tex |= v.shadow_tex != 0;
   }
+  MCR_TRY(engine_reserve(e, total, nviews, njobs, false, max_lights));
   e.mix = nviews > 1;
   hipStream_t sm = e.stream;
   MCR_TRY(engine_set_control(e, out, njobs, total, false));

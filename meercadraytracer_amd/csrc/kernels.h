@@ -24,6 +24,10 @@ void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront
 void launch_shadow(hipStream_t s, bool mix, bool wide, bool tex, const ViewDev *views, const Wavefront &w, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta, double2 *seen_uv,
                    DevCounters *cnt);
+// Retraces in the reference's order the shadow rays k_shadow marked LS_DEEP (spill_stride: the
+// spill area's thread count, at least MCR_DEEP_THREADS).
+void launch_shadow_deep(hipStream_t s, bool mix, bool tex, const ViewDev *views, const Wavefront &w, int32_t *spill,
+                        uint32_t spill_stride, DevCounters *cnt);
 void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                     uint32_t max_live);
 void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,

@@ -488,7 +488,7 @@ int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
   uint32_t n = (uint32_t)s->table.size();
   if (n == 0 && e.counts_host[cw(s->q)] == 0) return deliver(s, progress);
   progress = true;
-  MCR_TRY(engine_reserve(e, e.capacity_limit, (uint32_t)s->views.size(), n, true));
+  MCR_TRY(engine_reserve(e, e.capacity_limit, (uint32_t)s->views.size(), n, true, s->cfg.max_lights));
   MCR_TRY(ensure_host(s, std::max<uint32_t>(n, 1)));
   bool mixed = false;
   for (const SrvJob &j : s->table) mixed |= j.dev.view != 0;
@@ -614,6 +614,8 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   if (s->cfg.slots < 256 || s->cfg.slots > (1u << 26)) return mcr_fail(MCR_E_INVALID, "slots must be in [256, 2^26]");
   if (!s->cfg.window) s->cfg.window = s->cfg.slots;
   if (!s->cfg.max_requests) s->cfg.max_requests = 64;
+  if (!s->cfg.max_lights) s->cfg.max_lights = 16;
+  if (s->cfg.max_lights > MCR_MAX_LIGHTS) return mcr_fail(MCR_E_INVALID, "max_lights beyond MCR_MAX_LIGHTS_PER_SCENE");
   HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&s->encoded, hipEventDisableTiming));
@@ -632,7 +634,9 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   s->enc_cap = 1u << 16;
   HIP_TRY(hipHostMalloc(&s->enc_host, s->enc_cap * sizeof(EncodeItem), hipHostMallocDefault));
   HIP_TRY(hipMalloc(&s->enc_dev, s->enc_cap * sizeof(EncodeItem)));
-  MCR_TRY(engine_reserve(s->eng, s->cfg.slots, s->cfg.max_requests, 1024, true));
+  MCR_TRY(engine_reserve(s->eng, s->cfg.slots, s->cfg.max_requests, 1024, true, s->cfg.max_lights));
+  if (engine_capacity_for(s->eng, s->cfg.max_lights) < s->cfg.slots)
+    return mcr_fail(MCR_E_INVALID, "slots x max_lights exceeds the light-slot memory budget; lower one of them");
   MCR_TRY(engine_reset(s->eng));
   HIP_TRY(counters_clear(s->eng.counters, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -679,6 +683,11 @@ int mcr_server_add_scene(mcr_server *s, mcr_scene *scene) {
   if (!s || !scene) return mcr_fail(MCR_E_INVALID, "null argument");
   if (!scene->has_geometry || !scene->has_shading) return mcr_fail(MCR_E_NO_SCENE, "the scene has no geometry or shading");
   if (scene->ctx != s->ctx) return mcr_fail(MCR_E_INVALID, "the scene belongs to another context");
+  uint32_t slots_needed = scene->d.light_count + (scene->d.environment_count ? 1 : 0);
+  if (slots_needed > s->cfg.max_lights)
+    return mcr_fail(MCR_E_UNSUPPORTED, "the scene needs " + std::to_string(slots_needed) +
+                                           " light slots; the server was created with max_lights " +
+                                           std::to_string(s->cfg.max_lights));
   std::lock_guard<std::mutex> lk(s->mu);
   if (std::find(s->scenes.begin(), s->scenes.end(), scene) != s->scenes.end()) return MCR_OK;
   uint64_t b = mcr_scene_bytes(scene);

@@ -271,7 +271,8 @@ __global__ __launch_bounds__(256, MIX ? 4 : 5) void k_extend(const ViewDev *__re
                                                 int32_t *spill, uint32_t spill_stride,
                                                 DevCounters *cnt) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid < 2 * MCR_MAX_LIGHTS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;  // shade of this iteration appends
+  if (gid < 2 * MCR_SHADOW_REGIONS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;  // shade of this iteration appends
+  if (gid == 0) w.counts[cw(C_DEEP)] = 0;                                       // shadow of this iteration marks
   MCR_LANE_STACK(st, spill, spill_stride);
   extend_loop<MIX, false>(views, w, q, st, cnt);
 }
@@ -282,7 +283,8 @@ __global__ __launch_bounds__(256) void k_extend_wide(const ViewDev *__restrict__
                                                         DevCounters *cnt) {
   __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid < 2 * MCR_MAX_LIGHTS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;
+  if (gid < 2 * MCR_SHADOW_REGIONS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;
+  if (gid == 0) w.counts[cw(C_DEEP)] = 0;
   WideStack st{lds_base + threadIdx.x, lds_bits + threadIdx.x, spill + gid, (int)spill_stride, 256, 0};
   extend_loop<MIX, true>(views, w, q, st, cnt);
 }
@@ -663,7 +665,7 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
             }
             if (r + g + b <= 0) continue;
             size_t ls_idx = (size_t)li * P + slot;
-            const size_t RS = (size_t)MCR_MAX_LIGHTS * P;  // one row of light slots
+            const size_t RS = (size_t)w.lights * P;  // one row of light slots
             nt_st(&w.contrib[0 * RS + ls_idx], r);
             nt_st(&w.contrib[1 * RS + ls_idx], g);
             nt_st(&w.contrib[2 * RS + ls_idx], b);
@@ -725,7 +727,7 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
               if (ex * gx + ey * gy + ez * gz > 0) {
                 double env[3];
                 env_sample(sc.environments[environment], sc.textures, ex, ey, ez, env);
-                const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
+                const size_t RS = (size_t)w.lights * P;
                 size_t ls_idx = (size_t)nl * P + slot;
                 nt_st(&w.contrib[0 * RS + ls_idx], env[0] * environmentScale * diffR);
                 nt_st(&w.contrib[1 * RS + ls_idx], env[1] * environmentScale * diffG);
@@ -949,35 +951,51 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
       nl = V.shadow_slots;
     });
   }
-  // Enqueue the wave's shadow rays (one atomic per counter per block). Each light has a region of
-  // `capacity` entries in sref: rays of first-hit vertices (neighbouring pixels) fill it from the
-  // front and rays of later vertices from the back, so a traversal chunk mostly holds rays towards
-  // one light from neighbouring points rather than a mix diluted by scattered later bounces. The
-  // order of the queue does not change any result: each ray writes its own (slot, light) entry.
-  uint32_t mask = 0;
+  // Enqueue the wave's shadow rays (one atomic per counter per block). The queue has a region per
+  // light (light li in region li % 16 when there are more than 16), `region_cap` entries each: rays
+  // of first-hit vertices (neighbouring pixels) fill it from the front and rays of later vertices
+  // from the back, so a traversal chunk mostly holds rays towards one light from neighbouring points
+  // rather than a mix diluted by scattered later bounces. The order of the queue does not change
+  // any result: each ray writes its own (slot, light) entry.
+  uint64_t mask = 0;
   if (live && nshadow)
     for (uint32_t li = 0; li < nl; li++)
-      if (w.status[(size_t)li * P + slot] == LS_RAY) mask |= 1u << li;
-  constexpr int NSEG = 2 * MCR_MAX_LIGHTS;
+      if (w.status[(size_t)li * P + slot] == LS_RAY) mask |= 1ull << li;
+  constexpr int NSEG = 2 * MCR_SHADOW_REGIONS;
   uint32_t tot[NSEG], got[NSEG];
   uint32_t *ctr[NSEG];
-  for (int li = 0; li < MCR_MAX_LIGHTS; li++) {
-    bool want = (mask >> li) & 1u;
-    tot[2 * li] = (uint32_t)__popcll(__ballot(want && first_hit));
-    tot[2 * li + 1] = (uint32_t)__popcll(__ballot(want && !first_hit));
-    ctr[2 * li] = &w.counts[cw(C_SHADOW_SEG + 2 * li)];
-    ctr[2 * li + 1] = &w.counts[cw(C_SHADOW_SEG + 2 * li + 1)];
+  const uint32_t L = w.lights;  // uniform bound of every lane's light loop
+  // Regions in an unrolled loop (register arrays stay indexed by constants), the lights of a
+  // region (li = r, r + 16, ...) in a loop of their own.
+#pragma unroll
+  for (int r = 0; r < MCR_SHADOW_REGIONS; r++) {
+    uint32_t nf = 0, nb = 0;
+    for (uint32_t li = r; li < L; li += MCR_SHADOW_REGIONS) {
+      bool want = (mask >> li) & 1ull;
+      nf += (uint32_t)__popcll(__ballot(want && first_hit));
+      nb += (uint32_t)__popcll(__ballot(want && !first_hit));
+    }
+    tot[2 * r] = nf;
+    tot[2 * r + 1] = nb;
+    ctr[2 * r] = &w.counts[cw(C_SHADOW_SEG + 2 * r)];
+    ctr[2 * r + 1] = &w.counts[cw(C_SHADOW_SEG + 2 * r + 1)];
   }
   __shared__ uint32_t lds[4 * NSEG];
   block_reserve<NSEG>(ctr, tot, lds, got);
   uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
-  for (int li = 0; li < MCR_MAX_LIGHTS; li++) {
-    bool want = (mask >> li) & 1u;
-    uint64_t bf = __ballot(want && first_hit), bl = __ballot(want && !first_hit);
-    int32_t entry = slot * MCR_MAX_LIGHTS + li;
-    if (want && first_hit) w.sref[(size_t)li * P + got[2 * li] + (uint32_t)__popcll(bf & below)] = entry;
-    if (want && !first_hit)
-      w.sref[(size_t)(li + 1) * P - 1 - (got[2 * li + 1] + (uint32_t)__popcll(bl & below))] = entry;
+  const size_t RC = w.region_cap;
+#pragma unroll
+  for (int r = 0; r < MCR_SHADOW_REGIONS; r++) {
+    uint32_t front = got[2 * r], back = got[2 * r + 1];
+    for (uint32_t li = r; li < L; li += MCR_SHADOW_REGIONS) {
+      bool want = (mask >> li) & 1ull;
+      uint64_t bf = __ballot(want && first_hit), bl = __ballot(want && !first_hit);
+      uint32_t entry = sref_entry((uint32_t)slot, li);
+      if (want && first_hit) w.sref[(size_t)r * RC + front + (uint32_t)__popcll(bf & below)] = entry;
+      if (want && !first_hit) w.sref[(size_t)(r + 1) * RC - 1 - (back + (uint32_t)__popcll(bl & below))] = entry;
+      front += (uint32_t)__popcll(bf);
+      back += (uint32_t)__popcll(bl);
+    }
   }
 }
 
@@ -1048,14 +1066,27 @@ __device__ inline bool visits_before(int2 a, int2 b) {
   return (a.y & 0x7fffffff) < (b.y & 0x7fffffff);
 }
 
+// Entry e of the shadow queue, whose segments end at seg_end (two per region: first-hit rays
+// from the front of the region, later ones from its back).
+MCR_INLINE uint32_t shadow_queue_entry(const Wavefront &w, const uint32_t *seg_end, uint32_t e) {
+  uint32_t seg = 0, start = 0;
+  for (int k = 0; k < 2 * MCR_SHADOW_REGIONS; k++)
+    if (e >= seg_end[k]) {
+      seg = (uint32_t)k + 1;
+      start = seg_end[k];
+    }
+  size_t region = (size_t)(seg >> 1) * w.region_cap;
+  return (seg & 1) ? w.sref[region + w.region_cap - 1 - (e - start)] : w.sref[region + (e - start)];
+}
+
 template <bool MIX, bool WIDE, bool TEX, class Stack>
 MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &w, Stack &st,
                             double *seen, int2 *seen_meta, double2 *seen_uv, DevCounters *cnt) {
-  // The queue is the concatenation of its segments: per light, first-hit rays (front of the light's
+  // The queue is the concatenation of its segments: per region, first-hit rays (front of the
   // region of sref), then later ones (back of the region, in reverse).
-  uint32_t seg_end[2 * MCR_MAX_LIGHTS];
+  uint32_t seg_end[2 * MCR_SHADOW_REGIONS];
   uint32_t n = 0;
-  for (int k = 0; k < 2 * MCR_MAX_LIGHTS; k++) {
+  for (int k = 0; k < 2 * MCR_SHADOW_REGIONS; k++) {
     n += w.counts[cw(C_SHADOW_SEG + k)];
     seg_end[k] = n;
   }
@@ -1064,8 +1095,9 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
   TravCount tc = {0, 0, 0};
   unsigned long long queries = 0;
   size_t P = w.capacity;
-  const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
-  bool overflow = false, bad = false;
+  const size_t RS = (size_t)w.lights * P;
+  bool bad = false;
+  unsigned long long ndeep = 0;
   double *T = seen + gid;
   int2 *M = seen_meta + gid;
   double2 *UV = TEX ? seen_uv + gid : nullptr;  // the replay's view of this thread's records
@@ -1074,20 +1106,14 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
   for (uint32_t b = fetch.next(); b < n; b = fetch.next()) {
     uint32_t e = b + (threadIdx.x & 63);
     if (e >= n) continue;
-    uint32_t seg = 0, start = 0;
-    for (int k = 0; k < 2 * MCR_MAX_LIGHTS; k++)
-      if (e >= seg_end[k]) {
-        seg = (uint32_t)k + 1;
-        start = seg_end[k];
-      }
-    size_t region = (size_t)(seg >> 1) * P;
-    int32_t ref = (seg & 1) ? w.sref[region + P - 1 - (e - start)] : w.sref[region + (e - start)];
-    int slot = ref / MCR_MAX_LIGHTS, li = ref % MCR_MAX_LIGHTS;
+    uint32_t ref = shadow_queue_entry(w, seg_end, e);
+    uint32_t slot = sref_slot(ref), li = sref_light(ref);
     size_t idx = (size_t)li * P + slot;
     double ox = nt_ld(&w.sray[0 * RS + idx]), oy = nt_ld(&w.sray[1 * RS + idx]), oz = nt_ld(&w.sray[2 * RS + idx]);
     double dx = nt_ld(&w.sray[3 * RS + idx]), dy = nt_ld(&w.sray[4 * RS + idx]), dz = nt_ld(&w.sray[5 * RS + idx]);
     double maxT = nt_ld(&w.sray[6 * RS + idx]);
     double c[3] = {1, 1, 1};
+    bool deep = false;
     with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
     const DevScene &sc = V.sc;
     // The environment-diffuse ray (slot light_slots) is transmittance(..., false) (tracer.ts:346).
@@ -1105,11 +1131,13 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     } else {
       blocked = occluded(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, col);
     }
-    overflow |= col.overflow;
+    // More see-through hits than the list keeps: k_shadow_deep traces the ray again in the
+    // reference's order. An opaque hit settles the ray whatever else it crosses.
+    deep = !blocked && col.overflow;
     double depth[3] = {0, 0, 0};
     if (blocked) {
       c[0] = c[1] = c[2] = 0;
-    } else if (col.n > 0) {
+    } else if (col.n > 0 && !deep) {
       // Replay the see-through hits in the reference's visit order (insertion sort, few hits).
       for (int i = 1; i < col.n; i++) {
         double ti = T[i * S];
@@ -1170,15 +1198,121 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     }
     });
     queries++;
+    if (deep) {
+      w.status[idx] = LS_DEEP;
+      ndeep++;
+      continue;
+    }
     bool visible = c[0] + c[1] + c[2] > 0;
     nt_st(&w.shadow[0 * RS + idx], c[0]);
     nt_st(&w.shadow[1 * RS + idx], c[1]);
     nt_st(&w.shadow[2 * RS + idx], c[2]);
     w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
   }
-  if (overflow) atomicOr(&w.counts[cw(C_OVERFLOW)], OVERFLOW_SEEN);
+  uint32_t wave_deep = (uint32_t)wave_sum(ndeep);
+  if ((threadIdx.x & 63) == 0 && wave_deep) atomicAdd(&w.counts[cw(C_DEEP)], wave_deep);
   if (bad) atomicOr(&w.counts[cw(C_OVERFLOW)], OVERFLOW_WIDE);
   flush_counts(cnt, tc, 0, queries);
+}
+
+// attenuateShadow (tracer.ts:544-594) applied to each hit as intersectAny visits it, for shadow
+// rays retraced in the reference's order: the distances of the surfaces already counted live in
+// this thread's MCR_DEEP_HITS words of deep_seen.
+template <bool TEX>
+struct DeepShadow {
+  const DevScene *sc;
+  bool castOnly;
+  double *seen;     // this thread's counted distances at lane + j * MCR_DEEP_THREADS
+  uint32_t lane;
+  int n;
+  bool overflow;
+  double c[3];
+  double depth[3];
+  __device__ bool operator()(int k, double t, double u, double v, bool front) {
+    float4 rec = sc->tri[(size_t)k * 3 + 2];
+    int flags = (int)((__float_as_uint(rec.y) >> 8) & 0xffu);
+    if (castOnly && (flags & MCR_TRI_CAST_SHADOW) == 0) return true;
+    if (flags & MCR_TRI_OPAQUE) {
+      c[0] = c[1] = c[2] = 0;
+      return false;
+    }
+    const DevMaterial &m = sc->materials[__float_as_uint(rec.z)];
+    double alpha = m.opacity, cr = m.color[0], cg = m.color[1], cb = m.color[2];
+    if (TEX && (m.map >= 0 || m.alpha_map >= 0)) shadow_texel(*sc, m, k, u, v, alpha, cr, cg, cb);
+    if (m.alpha_test > 0 && alpha < m.alpha_test) return true;
+    if (!m.transparent) alpha = 1;
+    double eps = sc->epsilon;
+    for (int j = 0; j < n; j++)
+      if (jm_fabs(seen[lane + (size_t)j * MCR_DEEP_THREADS] - t) <= eps) return true;
+    if (n == MCR_DEEP_HITS) {  // the one capacity left: reported, never approximated
+      overflow = true;
+      return false;
+    }
+    seen[lane + (size_t)n * MCR_DEEP_THREADS] = t;
+    n++;
+    if (m.thickness > 0) {
+      double sgn = front ? -t : t;
+      depth[0] = depth[0] + m.attenuation[0] * sgn;
+      depth[1] = depth[1] + m.attenuation[1] * sgn;
+      depth[2] = depth[2] + m.attenuation[2] * sgn;
+    }
+    double open = 1 - alpha, through = m.transmission;
+    c[0] = c[0] * (open + alpha * through * cr);
+    c[1] = c[1] * (open + alpha * through * cg);
+    c[2] = c[2] * (open + alpha * through * cb);
+    if (c[0] + c[1] + c[2] < 1e-4) {
+      c[0] = c[1] = c[2] = 0;
+      return false;
+    }
+    return true;
+  }
+};
+
+// The reference-order pass over the shadow rays k_shadow marked LS_DEEP (none in most iterations:
+// the kernel then returns at once). MCR_DEEP_THREADS threads stride over the queue; each marked
+// ray is traced with intersectAny's depth-first order on the FP64 binary BVH and attenuateShadow
+// per hit, exactly as transmittance does (tracer.ts:529-542).
+template <bool MIX, bool TEX>
+__global__ __launch_bounds__(256) void k_shadow_deep(const ViewDev *__restrict__ views, Wavefront w,
+                                                     int32_t *spill, uint32_t spill_stride, DevCounters *cnt) {
+  if (w.counts[cw(C_DEEP)] == 0) return;
+  uint32_t seg_end[2 * MCR_SHADOW_REGIONS];
+  uint32_t n = 0;
+  for (int k = 0; k < 2 * MCR_SHADOW_REGIONS; k++) {
+    n += w.counts[cw(C_SHADOW_SEG + k)];
+    seg_end[k] = n;
+  }
+  MCR_LANE_STACK(st, spill, spill_stride);
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  size_t P = w.capacity;
+  const size_t RS = (size_t)w.lights * P;
+  TravCount tc = {0, 0, 0};
+  bool overflow = false;
+  for (uint32_t e = gid; e < n; e += MCR_DEEP_THREADS) {
+    uint32_t ref = shadow_queue_entry(w, seg_end, e);
+    uint32_t slot = sref_slot(ref), li = sref_light(ref);
+    size_t idx = (size_t)li * P + slot;
+    if (w.status[idx] != LS_DEEP) continue;
+    double ox = w.sray[0 * RS + idx], oy = w.sray[1 * RS + idx], oz = w.sray[2 * RS + idx];
+    double dx = w.sray[3 * RS + idx], dy = w.sray[4 * RS + idx], dz = w.sray[5 * RS + idx];
+    double maxT = w.sray[6 * RS + idx];
+    double c[3];
+    with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
+      const DevScene &sc = V.sc;
+      DeepShadow<TEX> vis{&sc, V.st.shadows == MCR_SHADOWS_CAST && li != V.light_slots, w.deep_seen, gid, 0, false,
+                          {1, 1, 1}, {0, 0, 0}};
+      intersect_any(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, vis);
+      overflow |= vis.overflow;
+      for (int k = 0; k < 3; k++) c[k] = vis.depth[k] > 0 ? vis.c[k] * jm_exp(-vis.depth[k]) : vis.c[k];
+    });
+    bool visible = c[0] + c[1] + c[2] > 0;
+    w.shadow[0 * RS + idx] = c[0];
+    w.shadow[1 * RS + idx] = c[1];
+    w.shadow[2 * RS + idx] = c[2];
+    w.status[idx] = visible ? LS_VISIBLE : LS_BLOCKED;
+  }
+  if (overflow) atomicOr(&w.counts[cw(C_OVERFLOW)], OVERFLOW_SEEN);
+  flush_counts(cnt, tc, 0, 0);
 }
 
 template <bool MIX, bool TEX>
@@ -1210,7 +1344,7 @@ MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control
   const DevScene &sc = V.sc;
   const mcr_settings &st = V.st;
   size_t P = w.capacity;
-  const size_t RS = (size_t)MCR_MAX_LIGHTS * P;
+  const size_t RS = (size_t)w.lights * P;
   double *f = w.f;
   if (mode == MODE_ADD_EXTEND || mode == MODE_ADD_FINISH) {
     double oR = f[F_AR * P + slot], oG = f[F_AG * P + slot], oB = f[F_AB * P + slot];
@@ -1563,6 +1697,17 @@ void launch_shadow(hipStream_t s, bool mix, bool wide, bool tex, const ViewDev *
                    double2 *seen_uv, DevCounters *cnt) {
   if (tex) launch_shadow_tex<true>(s, mix, wide, views, w, grid, spill, spill_stride, seen, seen_meta, seen_uv, cnt);
   else launch_shadow_tex<false>(s, mix, wide, views, w, grid, spill, spill_stride, seen, seen_meta, seen_uv, cnt);
+}
+void launch_shadow_deep(hipStream_t s, bool mix, bool tex, const ViewDev *views, const Wavefront &w, int32_t *spill,
+                        uint32_t spill_stride, DevCounters *cnt) {
+  const uint32_t grid = MCR_DEEP_THREADS / 256;
+  if (tex) {
+    if (mix) hipLaunchKernelGGL((k_shadow_deep<true, true>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride, cnt);
+    else hipLaunchKernelGGL((k_shadow_deep<false, true>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride, cnt);
+  } else {
+    if (mix) hipLaunchKernelGGL((k_shadow_deep<true, false>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride, cnt);
+    else hipLaunchKernelGGL((k_shadow_deep<false, false>), dim3(grid), dim3(256), 0, s, views, w, spill, spill_stride, cnt);
+  }
 }
 void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                     uint32_t max_live) {

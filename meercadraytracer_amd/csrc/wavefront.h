@@ -10,9 +10,13 @@
 #include "../../include/mcr.h"
 #include "wide.h"
 
-#define MCR_MAX_LIGHTS 16          // punctual lights traced per scene
+#define MCR_MAX_LIGHTS MCR_MAX_LIGHTS_PER_SCENE  // lights per scene (mcr.h)
+#define MCR_SHADOW_REGIONS 16      // regions of the shadow-ray queue; light li queues in region li % 16
 #define MCR_FETCH_COUNTERS 8       // interleaved work counters of the traversal kernels
-#define MCR_SHADOW_HIT_CAP 64      // see-through surfaces remembered per shadow ray (tracer.ts:48)
+#define MCR_SHADOW_HIT_CAP 64      // see-through hits a shadow traversal keeps for its ordered replay
+#define MCR_DEEP_HITS 1024         // counted see-through surfaces a replayed shadow ray may cross
+#define MCR_DEEP_THREADS (64 * 256)  // threads of the shadow kernel's reference-order pass
+#define MCR_SLOT_BITS 26           // a shadow-queue entry: light << 26 | slot (capacity <= 2^26)
 
 // Double fields of a path slot.
 enum PathField {
@@ -52,8 +56,9 @@ enum PathMode {
   MODE_FINISH = 4,      // write the sample
 };
 
-// Light-slot status after shade / shadow.
-enum LightStatus { LS_NONE = 0, LS_VISIBLE = 1, LS_RAY = 2, LS_BLOCKED = 3 };
+// Light-slot status after shade / shadow. LS_DEEP: the shadow ray crossed more see-through surfaces
+// than the traversal keeps for its ordered replay; the reference-order pass traces it again.
+enum LightStatus { LS_NONE = 0, LS_VISIBLE = 1, LS_RAY = 2, LS_BLOCKED = 3, LS_DEEP = 4 };
 
 // Counter words of Wavefront::counts.
 enum CountWord {
@@ -65,17 +70,23 @@ enum CountWord {
   C_FREE = 4,       // slots on the free list
   C_NEXT = 5,       // next sample to start (may run past Control::total; the host clamps it)
   C_SORT = 6,       // appends of the queue re-sort (k_sort_gather)
-  C_EXTEND_FETCH = 7,                            // MCR_FETCH_COUNTERS words (WaveFetch)
+  C_DEEP = 7,       // shadow rays of this iteration marked LS_DEEP
+  C_EXTEND_FETCH = 8,                            // MCR_FETCH_COUNTERS words (WaveFetch)
   C_SHADOW_FETCH = C_EXTEND_FETCH + MCR_FETCH_COUNTERS,
-  // Shadow rays queued by shade, two words per light: word 2*li counts rays of first-hit vertices
-  // (stored from the front of light li's region of sref), word 2*li+1 rays of later vertices
-  // (from the back of that region).
+  // Shadow rays queued by shade, two words per region: word 2*r counts rays of first-hit vertices
+  // (stored from the front of region r of sref), word 2*r+1 rays of later vertices (from the back
+  // of that region).
   C_SHADOW_SEG = C_SHADOW_FETCH + MCR_FETCH_COUNTERS,
-  C_SHADOW_SEG_END = C_SHADOW_SEG + 2 * MCR_MAX_LIGHTS,
+  C_SHADOW_SEG_END = C_SHADOW_SEG + 2 * MCR_SHADOW_REGIONS,
   C_WORDS = C_SHADOW_SEG_END,
 };
 
 enum OverflowBits { OVERFLOW_SEEN = 1, OVERFLOW_WIDE = 2 };
+
+// Shadow-queue entries: the ray of light li from path slot `slot`.
+__host__ __device__ constexpr inline uint32_t sref_entry(uint32_t slot, uint32_t li) { return (li << MCR_SLOT_BITS) | slot; }
+__host__ __device__ constexpr inline uint32_t sref_slot(uint32_t e) { return e & ((1u << MCR_SLOT_BITS) - 1u); }
+__host__ __device__ constexpr inline uint32_t sref_light(uint32_t e) { return e >> MCR_SLOT_BITS; }
 
 // Each counter word sits on a 128-byte cache line of its own: atomics on one line serialise, and
 // the traversal kernels' work counters must not queue behind each other or behind the queue
@@ -131,7 +142,7 @@ struct ViewDev {
   WideScene wide;            // the precision form's layout, when the run uses it
   mcr_camera cam;
   mcr_settings st;
-  uint32_t light_slots;      // min(light count, MCR_MAX_LIGHTS)
+  uint32_t light_slots;      // the scene's light count
   uint32_t shadow_slots;     // light_slots, plus the environment-diffuse slot when the scene has environments
   uint32_t shadow_tex;       // a material has map or alphaMap, which shadow rays sample
   uint32_t pad;
@@ -139,10 +150,12 @@ struct ViewDev {
 
 struct Wavefront {
   uint32_t capacity;         // path slots
+  uint32_t lights;           // light slots per path slot (the largest shadow_slots of the views it runs)
+  uint32_t region_cap;       // entries of each shadow-queue region: ceil(lights / 16) * capacity
   double *f;                 // F_COUNT * capacity
   int32_t *i;                // I_COUNT * capacity
   uint8_t *mode;             // capacity
-  // light slots, indexed (row * MCR_MAX_LIGHTS + light) * capacity + slot
+  // light slots, indexed (row * lights + light) * capacity + slot
   double *contrib;           // 3 rows: the unshadowed r, g, b of the light at this vertex
   double *shadow;            // 3 rows: shadowColor after the shadow ray
   double *sray;              // 7 rows: shadow ray origin, direction, maxT
@@ -150,8 +163,9 @@ struct Wavefront {
   // queues
   int32_t *queue[2];         // live slots
   uint32_t *counts;          // CountWord
-  int32_t *sref;             // shadow queue, capacity entries per light: slot * MCR_MAX_LIGHTS + light
+  uint32_t *sref;            // shadow queue: min(lights, 16) regions of region_cap sref_entry values
   int32_t *free_list;        // slots with no sample
+  double *deep_seen;         // MCR_DEEP_HITS counted distances per thread of the reference-order pass
   const Control *ctl;
 };
 

@@ -144,8 +144,8 @@ def test_textured_workload_bit_exact():
 
 def test_bad_texture_data_is_refused():
     """Indices out of range are refused when the shading is set, with a message naming the part;
-    a scene with environments keeps one light slot for the environment-diffuse ray (at most 15
-    punctual lights)."""
+    a scene with environments keeps one light slot for the environment-diffuse ray (at most 63
+    lights)."""
     import dataclasses
     from meercadraytracer_amd.types import CompiledLight
     sc, cam = _compiled('environment')
@@ -159,21 +159,21 @@ def test_bad_texture_data_is_refused():
                                  + sc.shading.environments[1:]), 'environment 0'),
             (dataclasses.replace(sc.shading, environment=len(sc.shading.environments)), 'scene environment'),
             (dataclasses.replace(sc.shading, lights=[CompiledLight('point', position=(0.0, 3.0, float(i)))
-                                                     for i in range(16)]), 'more than 15 lights'),
+                                                     for i in range(64)]), 'more than 63 lights'),
         ]
         for shading, words in cases:
             with pytest.raises(native.NativeError) as e:
                 s.set_shading(shading, 7)
             assert words in str(e.value), (words, str(e.value))
-        # 15 punctual lights with environments still trace
+        # 63 lights with environments still trace
         ok = dataclasses.replace(sc.shading, lights=[CompiledLight('point', color=(0.2, 0.2, 0.2),
                                                                    position=(float(i % 5) - 2, 3.0, float(i // 5)))
-                                                     for i in range(15)])
+                                                     for i in range(63)])
         s.set_shading(ok, 8)
         st = trace_settings({})
         g = s.render_tile(cam, st, Tile(0, 0, 24, 16), W, H, 0, 1)
         from meercadraytracer_amd.types import CompiledScene
         ref, _ = oracle.render_tile(CompiledScene(sc.geometry, ok), cam, st, Tile(0, 0, 24, 16), W, H, 0, 1)
-        _same(g, ref, '15 lights with environments')
+        _same(g, ref, '63 lights with environments')
     finally:
         s.release()
