@@ -286,6 +286,8 @@ typedef struct mcr_request {
   uint32_t tile_size;       /* 0 = 64 (RaytracingRenderer.ts:105) */
   uint32_t preview_block;   /* <= 1: no preview; 0 = 4 */
   int32_t srgb;             /* outputColorSpace 'srgb' (1) or 'srgb-linear' (0) */
+  uint32_t geometry_version, shading_version;  /* the scene versions the request is for, as a tile
+                                                  message names them (protocol.ts:37-45); 0 = any */
 } mcr_request;
 enum { MCR_REQ_QUEUED = 0, MCR_REQ_RUNNING = 1, MCR_REQ_DONE = 2, MCR_REQ_CANCELLED = 3, MCR_REQ_FAILED = 4 };
 typedef struct mcr_request_status {
@@ -306,7 +308,13 @@ typedef struct mcr_server_stats {
 } mcr_server_stats;
 int mcr_server_create(mcr_ctx *ctx, const mcr_server_config *config, mcr_server **out);
 void mcr_server_destroy(mcr_server *server);
+/* Makes a resident scene available to requests. While a server holds a scene, its geometry,
+ * shading and wide BVH cannot change (those calls return MCR_E_INVALID) and mcr_scene_release
+ * is deferred until the last server lets it go: requests render the arrays admitted with them. */
 int mcr_server_add_scene(mcr_server *server, mcr_scene *scene);
+/* Lets a scene go (SceneSync sending a new version starts here: remove, change, add again).
+ * MCR_E_INVALID while a request on the scene is queued, running or still draining. */
+int mcr_server_remove_scene(mcr_server *server, mcr_scene *scene);
 int mcr_server_submit(mcr_server *server, const mcr_request *request, uint64_t *id_out);
 int mcr_server_cancel(mcr_server *server, uint64_t id);
 int mcr_server_status(mcr_server *server, uint64_t id, mcr_request_status *out);

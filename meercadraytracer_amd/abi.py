@@ -194,7 +194,8 @@ class ServerConfig(C.Structure):
 class Request(C.Structure):
     _fields_ = [('scene', C.c_void_p), ('camera', Camera), ('settings', Settings), ('width', C.c_uint32),
                 ('height', C.c_uint32), ('samples', C.c_uint32), ('tile_size', C.c_uint32),
-                ('preview_block', C.c_uint32), ('srgb', C.c_int32)]
+                ('preview_block', C.c_uint32), ('srgb', C.c_int32), ('geometry_version', C.c_uint32),
+                ('shading_version', C.c_uint32)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)

@@ -191,6 +191,34 @@ static int noalloc_setup(int device) {
   return MCR_OK;
 }
 
+// ---- server holds on scenes ---------------------------------------------------------------------
+static std::mutex g_scene_refs_mu;
+static void scene_free(mcr_scene *s);
+
+void scene_hold(mcr_scene *s) {
+  std::lock_guard<std::mutex> lk(g_scene_refs_mu);
+  s->server_refs++;
+}
+
+void scene_drop(mcr_scene *s) {
+  bool free_now = false;
+  {
+    std::lock_guard<std::mutex> lk(g_scene_refs_mu);
+    s->server_refs--;
+    free_now = s->server_refs == 0 && s->release_pending;
+  }
+  if (free_now) scene_free(s);
+}
+
+bool scene_held(const mcr_scene *s) {
+  std::lock_guard<std::mutex> lk(g_scene_refs_mu);
+  return s->server_refs > 0;
+}
+
+int scene_held_error() {
+  return fail(MCR_E_INVALID, "the scene is held by a server; mcr_server_remove_scene it first");
+}
+
 extern "C" {
 
 const char *mcr_last_error(void) { return g_error.c_str(); }
@@ -295,6 +323,17 @@ int mcr_scene_create(mcr_ctx *c, mcr_scene **out) {
 
 void mcr_scene_release(mcr_scene *s) {
   if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(g_scene_refs_mu);
+    if (s->server_refs > 0) {  // freed when the last server holding it lets it go (scene_drop)
+      s->release_pending = true;
+      return;
+    }
+  }
+  scene_free(s);
+}
+
+static void scene_free(mcr_scene *s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   free_geometry(s);
@@ -310,6 +349,7 @@ uint64_t mcr_scene_bytes(const mcr_scene *s) { return s ? s->bytes_geometry + s-
 
 int mcr_scene_set_geometry(mcr_scene *s, const mcr_geometry *g, uint32_t version) {
   if (!s || !g) return fail(MCR_E_INVALID, "null argument");
+  if (scene_held(s)) return scene_held_error();
   uint32_t n = g->triangle_count, nodes = g->node_count;
   if (n >= MCR_MAX_TRIANGLES) return fail(MCR_E_OVERFLOW, "too many triangles for this build (2^26 max)");
   if (n > 0 && (!g->triangles || !g->normals || !g->materials || !g->flags || !g->sides || !g->bvh_bounds ||
@@ -424,6 +464,7 @@ int mcr_scene_set_geometry(mcr_scene *s, const mcr_geometry *g, uint32_t version
 
 int mcr_scene_set_shading(mcr_scene *s, const mcr_shading *sh, uint32_t version) {
   if (!s || !sh) return fail(MCR_E_INVALID, "null argument");
+  if (scene_held(s)) return scene_held_error();
   std::string why;
   if (!shading_supported(sh, why)) return fail(MCR_E_UNSUPPORTED, why);
   mcr_ctx *c = s->ctx;

@@ -535,6 +535,7 @@ int mcr_compile_geometry(mcr_ctx *c, const mcr_mesh *meshes, uint32_t mesh_count
 int mcr_scene_compile_geometry(mcr_scene *s, const mcr_mesh *meshes, uint32_t mesh_count, uint32_t version,
                                mcr_compile_info *info) {
   if (!s || (!meshes && mesh_count)) return fail(MCR_E_INVALID, "null argument");
+  if (scene_held(s)) return scene_held_error();
   uint64_t tris;
   bool any_colors;
   MCR_TRY(count_meshes(meshes, mesh_count, tris, any_colors));

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -144,6 +145,11 @@ struct mcr_ctx {
 
 struct mcr_scene {
   mcr_ctx *ctx = nullptr;
+  // Servers holding the scene (mcr_server_add_scene): while any does, its device arrays are in use
+  // by requests it may admit, so geometry, shading and wide-BVH changes are refused and a release
+  // is deferred until the last server lets the scene go.
+  int server_refs = 0;
+  bool release_pending = false;
   DevScene d = {};
   uint32_t geometry_version = 0, shading_version = 0;
   bool has_geometry = false, has_shading = false;
@@ -168,6 +174,13 @@ struct mcr_scene {
   WideScene wide = {};
   mcr_wide_info wide_info = {};
 };
+
+// Server holds on a scene (under one library-wide lock). scene_held: refused changes report it.
+void scene_hold(mcr_scene *s);
+// Drops one hold; frees the scene if it was released while held and this was the last hold.
+void scene_drop(mcr_scene *s);
+bool scene_held(const mcr_scene *s);
+int scene_held_error();
 
 // Frees the scene's wide BVH (wide.hip).
 void wide_free(mcr_scene *s);

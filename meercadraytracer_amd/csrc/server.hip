@@ -155,7 +155,8 @@ struct mcr_server {
   uint32_t total = 0;              // samples [0, total) may be started
   int q = 0;                       // queue holding the live paths
   uint64_t bytes_used = 0;
-  std::vector<mcr_scene *> scenes;
+  std::vector<mcr_scene *> scenes;  // held (scene_hold) until removed or the server is destroyed
+  std::vector<uint64_t> scene_bytes;  // what each counts against the HBM budget
   EncodeItem *enc_host = nullptr, *enc_dev = nullptr;
   uint32_t enc_cap = 0;
   JobDev *jobs_host = nullptr;     // pinned staging of the job list
@@ -667,6 +668,7 @@ void mcr_server_destroy(mcr_server *s) {
     hipFree(kv.second->sum);
   }
   engine_free(s->eng);
+  for (mcr_scene *scene : s->scenes) scene_drop(scene);
   hipFree(s->arena_dev);
   hipFree(s->enc_dev);
   if (s->enc_host) hipHostFree(s->enc_host);
@@ -694,6 +696,28 @@ int mcr_server_add_scene(mcr_server *s, mcr_scene *scene) {
   if (s->bytes_used + b > s->cfg.hbm_budget) return mcr_fail(MCR_E_OVERFLOW, "the scene does not fit the HBM budget");
   s->bytes_used += b;
   s->scenes.push_back(scene);
+  s->scene_bytes.push_back(b);
+  scene_hold(scene);
+  return MCR_OK;
+}
+
+int mcr_server_remove_scene(mcr_server *s, mcr_scene *scene) {
+  if (!s || !scene) return mcr_fail(MCR_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = std::find(s->scenes.begin(), s->scenes.end(), scene);
+  if (it == s->scenes.end()) return mcr_fail(MCR_E_INVALID, "the scene was not added to the server");
+  // A request holds the scene from submission until its last sample has drained (retire()).
+  for (auto &kv : s->reqs) {
+    const SrvRequest &r = *kv.second;
+    bool live = r.state == MCR_REQ_QUEUED || r.state == MCR_REQ_RUNNING || r.view >= 0;
+    if (live && r.r.scene == scene)
+      return mcr_fail(MCR_E_INVALID, "requests on the scene are still live; cancel them or wait for them first");
+  }
+  size_t k = (size_t)(it - s->scenes.begin());
+  s->bytes_used -= s->scene_bytes[k];
+  s->scenes.erase(it);
+  s->scene_bytes.erase(s->scene_bytes.begin() + (long)k);
+  scene_drop(scene);  // may free it, if it was released while held
   return MCR_OK;
 }
 
@@ -708,6 +732,12 @@ int mcr_server_submit(mcr_server *s, const mcr_request *req, uint64_t *id_out) {
   if (s->fatal) return mcr_fail(s->fatal, s->fatal_msg);
   if (std::find(s->scenes.begin(), s->scenes.end(), r.scene) == s->scenes.end())
     return mcr_fail(MCR_E_INVALID, "the scene was not added to the server");
+  // protocol.ts:76-78: the request names the scene versions it was made for (0: any). A held scene
+  // cannot change, so what matches now matches until the request is done.
+  const mcr_scene *sc = r.scene;
+  if (!sc->has_geometry || !sc->has_shading || (r.geometry_version && r.geometry_version != sc->geometry_version) ||
+      (r.shading_version && r.shading_version != sc->shading_version))
+    return mcr_fail(MCR_E_NO_SCENE, "the tile refers to a scene this worker has not received");
   uint64_t need = request_bytes(r);
   if (need + (uint64_t)s->arena_floats * 4 > s->cfg.hbm_budget)
     return mcr_fail(MCR_E_OVERFLOW, "the request can never fit the HBM budget");

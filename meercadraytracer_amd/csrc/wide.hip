@@ -279,6 +279,7 @@ extern "C" {
 int mcr_scene_build_wide(mcr_scene *s, mcr_wide_info *info_out) {
   if (!s) return fail(MCR_E_INVALID, "null argument");
   if (!s->has_geometry) return fail(MCR_E_NO_SCENE, "the scene has no geometry");
+  if (scene_held(s)) return scene_held_error();
   mcr_ctx *c = s->ctx;
   std::lock_guard<std::mutex> lock(c->mu);
   HIP_TRY(hipSetDevice(c->device));

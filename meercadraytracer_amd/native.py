@@ -60,6 +60,7 @@ def lib():
             'mcr_server_create': (C.c_int, [P, P, C.POINTER(P)]),
             'mcr_server_destroy': (None, [P]),
             'mcr_server_add_scene': (C.c_int, [P, P]),
+            'mcr_server_remove_scene': (C.c_int, [P, P]),
             'mcr_server_submit': (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
             'mcr_server_cancel': (C.c_int, [P, C.c_uint64]),
             'mcr_server_status': (C.c_int, [P, C.c_uint64, P]),
@@ -440,9 +441,13 @@ class Server:
     def add_scene(self, scene):
         check(lib().mcr_server_add_scene(self.handle, scene.handle))
 
-    def submit(self, scene, camera, settings, width, height, samples, tile_size=64, preview_block=4, srgb=True):
+    def remove_scene(self, scene):
+        check(lib().mcr_server_remove_scene(self.handle, scene.handle))
+
+    def submit(self, scene, camera, settings, width, height, samples, tile_size=64, preview_block=4, srgb=True,
+               geometry_version=0, shading_version=0):
         r = abi.Request(scene.handle, camera.to_c(), settings.to_c(), width, height, samples, tile_size,
-                        preview_block, int(bool(srgb)))
+                        preview_block, int(bool(srgb)), int(geometry_version), int(shading_version))
         rid = C.c_uint64()
         check(lib().mcr_server_submit(self.handle, C.byref(r), C.byref(rid)))
         self.sizes[rid.value] = (width, height)

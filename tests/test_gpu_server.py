@@ -171,3 +171,92 @@ def test_errors(world):
         assert srv.wait(rid, 30000) == abi.REQ_CANCELLED
     finally:
         srv.close()
+
+
+def test_scene_versions_between_requests():
+    """A served scene moves to new shading, then new geometry, between requests, the way
+    SceneSync re-sends what changed (progressive.ts:80-157) and a worker replaces what it holds
+    (protocol.ts:64-74): remove the scene from the server, change it, add it again. Each request
+    names the versions it is for (protocol.ts:76-78) and its finished image is the oracle's
+    raytrace() of that version, byte for byte. While the server holds the scene, changes are
+    refused; while a request on it is live, so is its removal."""
+    import dataclasses
+    from meercadraytracer_amd.types import CompiledScene
+    sc, cam, o = scenes.compiled('plain_cpu', 0, build_bvh=oracle.build_bvh, width=48, height=32)
+    sc_b, _, _ = scenes.compiled('plain_cpu', 1, build_bvh=oracle.build_bvh, width=48, height=32)
+    st = trace_settings(o)
+    shading2 = dataclasses.replace(sc.shading, materials=[dataclasses.replace(m, color=(m.color[2], m.color[0], m.color[1]))
+                                                          for m in sc.shading.materials])
+    s = native.Scene()
+    s.set_geometry(sc.geometry, 1)
+    s.set_shading(sc.shading, 1)
+    srv = native.Server(slots=1 << 16)
+    try:
+        srv.add_scene(s)
+        with pytest.raises(native.NativeError) as e:
+            s.set_shading(shading2, 2)
+        assert 'held by a server' in str(e.value)
+        with pytest.raises(native.NativeError) as e:
+            srv.submit(s, cam, st, 48, 32, 2, geometry_version=1, shading_version=2)
+        assert 'has not received' in str(e.value)
+
+        def served(gv, sv, expect_scene, samples=2):
+            rid = srv.submit(s, cam, st, 48, 32, samples, tile_size=16, geometry_version=gv, shading_version=sv)
+            assert srv.wait(rid, 60000) == abi.REQ_DONE
+            got = srv.read(rid)
+            srv.release(rid)
+            ref, _ = oracle.raytrace(expect_scene, cam, st, 48, 32, samples=samples, srgb=True)
+            assert np.array_equal(got, ref), (gv, sv)
+
+        served(1, 1, sc)
+        srv.remove_scene(s)
+        s.set_shading(shading2, 2)
+        srv.add_scene(s)
+        served(1, 2, CompiledScene(sc.geometry, shading2))
+        # a long request holds the scene: its removal is refused until it has drained
+        rid = srv.submit(s, cam, st, 48, 32, 4096, tile_size=16)
+        with pytest.raises(native.NativeError) as e:
+            srv.remove_scene(s)
+        assert 'still live' in str(e.value)
+        srv.cancel(rid)
+        assert srv.wait(rid, 60000) == abi.REQ_CANCELLED
+        t0 = time.time()
+        while True:
+            try:
+                srv.remove_scene(s)
+                break
+            except native.NativeError:
+                assert time.time() - t0 < 30, 'the cancelled request never drained'
+                time.sleep(0.01)
+        srv.release(rid)
+        s.set_geometry(sc_b.geometry, 2)
+        srv.add_scene(s)
+        served(2, 2, CompiledScene(sc_b.geometry, shading2))
+        served(0, 0, CompiledScene(sc_b.geometry, shading2), samples=1)  # 0: whatever the scene holds
+    finally:
+        srv.close()
+        s.release()
+
+
+def test_release_of_a_held_scene_waits_for_the_server():
+    """mcr_scene_release on a scene a server holds is deferred: requests on it still render (their
+    arrays stay allocated) and the scene is freed when the server lets it go."""
+    sc, cam, o = scenes.compiled('plain_cpu', 0, build_bvh=oracle.build_bvh, width=32, height=32)
+    st = trace_settings(o)
+    s = native.Scene()
+    s.set_geometry(sc.geometry, 1)
+    s.set_shading(sc.shading, 1)
+    srv = native.Server(slots=1 << 16)
+    try:
+        srv.add_scene(s)
+        rid = srv.submit(s, cam, st, 32, 32, 3, tile_size=16)
+        handle = s.handle
+        native.lib().mcr_scene_release(handle)  # deferred: the server holds it
+        assert srv.wait(rid, 60000) == abi.REQ_DONE
+        ref, _ = oracle.raytrace(sc, cam, st, 32, 32, samples=3, srgb=True)
+        assert np.array_equal(srv.read(rid), ref)
+        srv.release(rid)
+        native.check(native.lib().mcr_server_remove_scene(srv.handle, handle))  # frees it now
+        s.handle = None
+    finally:
+        srv.close()
