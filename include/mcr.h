@@ -386,7 +386,9 @@ typedef struct mcr_wide_info {
 int mcr_scene_build_wide(mcr_scene *scene, mcr_wide_info *info_out);
 int mcr_scene_wide_info(const mcr_scene *scene, mcr_wide_info *out);
 /* rays: n*6 floats (origin, direction). Outputs as mcr_intersect_closest, in FP32; triangle ids
- * are BVH-order indices, as there. */
+ * are BVH-order indices, as there. The winner is found in FP32; its t, u and v are then the
+ * reference's FP64 test on that triangle and the widened ray (bvh.ts:228-252), rounded to FP32,
+ * except where that test rejects it (a ray within FP32 rounding of an edge). */
 #define MCR_ISECT_FP64 0       /* the reference's FP64 arithmetic on its binary BVH (default) */
 #define MCR_ISECT_WIDE_FP32 1  /* the precision form: FP32 on the 8-wide BVH */
 /* How renders on this context (mcr_render_tile/tiles/batch) intersect rays: closest hits

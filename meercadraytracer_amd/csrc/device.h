@@ -334,6 +334,21 @@ MCR_INLINE void fresh_inverse(double dx, double dy, double dz, int32_t salt, dou
   iz = safe_inverse(dz + z);
 }
 
+// The precision form's refinement of a first hit found in FP32: triangle k's FP64 Möller–Trumbore
+// (bvh.ts:228-252) on the FP64 ray gives t, u and v exactly as the reference computes them for that
+// triangle. Returns false (values untouched) where the FP64 test rejects the triangle that the FP32
+// watertight test accepted: a ray through an edge, within FP32 rounding of it.
+MCR_INLINE bool refine_hit(const float4 *tri, int k, double ox, double oy, double oz, double dx, double dy,
+                           double dz, double &t, double &u, double &v) {
+  const float4 *r = tri + (size_t)k * 3;
+  double tt, uu, vv, det;
+  if (!mt_test(r[0], r[1], r[2], ox, oy, oz, dx, dy, dz, 2, tt, uu, vv, det)) return false;
+  t = tt;
+  u = uu;
+  v = vv;
+  return true;
+}
+
 // bvh.ts:202-282 over the device layout. cull: drop back faces of FrontSide and front faces of
 // BackSide triangles. Near/far orders use the FP64 entry distances (exact-t ties between
 // triangles are broken by visit order, so the order must be the reference's).

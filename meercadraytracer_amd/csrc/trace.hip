@@ -251,6 +251,7 @@ MCR_INLINE void extend_loop(const ViewDev *__restrict__ views, const Wavefront &
         h.u = wh.u;
         h.v = wh.v;
         h.front = wh.front;
+        if (wh.tri >= 0) refine_hit(V.sc.tri, wh.tri, ox, oy, oz, dx, dy, dz, h.t, h.u, h.v);
       } else {
         intersect_closest(V.sc, ox, oy, oz, dx, dy, dz, 0, JM_INF, (flags & PF_CULL) != 0, h, st, tc);
       }

@@ -219,9 +219,9 @@ __global__ __launch_bounds__(256) void k_wide_level(WideBuild a, uint32_t lo, ui
 }
 
 // Closest hits of caller rays (6 floats each) over the wide BVH.
-__global__ __launch_bounds__(256) void k_wide_closest(WideScene ws, const float *rays, uint32_t n, float tmin,
-                                                      float tmax, int cull, int32_t *tri, float *t, float *u,
-                                                      float *v, uint8_t *front, int32_t *spill,
+__global__ __launch_bounds__(256) void k_wide_closest(WideScene ws, const float4 *tri64, const float *rays, uint32_t n,
+                                                      float tmin, float tmax, int cull, int32_t *tri, float *t,
+                                                      float *u, float *v, uint8_t *front, int32_t *spill,
                                                       uint32_t spill_stride, DevCounters *cnt, uint32_t *err,
                                                       uint32_t *fetch_counters) {
   __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
@@ -250,10 +250,13 @@ __global__ __launch_bounds__(256) void k_wide_closest(WideScene ws, const float 
     if (wide_ray(r[0], r[1], r[2], r[3], r[4], r[5], wr))
       wide_closest(ws, wr, tmin, tmax, cull != 0, h, st, tc, bad);
     queries++;
+    // The winner's t, u, v from the reference's FP64 test on the same (widened) ray.
+    double rt = h.t, ru = h.u, rv = h.v;
+    if (h.tri >= 0) refine_hit(tri64, h.tri, r[0], r[1], r[2], r[3], r[4], r[5], rt, ru, rv);
     if (tri) tri[i] = h.tri;
-    if (t) t[i] = h.t;
-    if (u) u[i] = h.u;
-    if (v) v[i] = h.v;
+    if (t) t[i] = (float)rt;
+    if (u) u[i] = (float)ru;
+    if (v) v[i] = (float)rv;
     if (front) front[i] = h.front ? 1 : 0;
   }
   if (bad) *err = 1;
@@ -412,7 +415,7 @@ int mcr_intersect_closest_wide(mcr_ctx *c, mcr_scene *s, const float *rays, uint
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[0], sm));
   uint32_t grid = std::min<uint32_t>(TRAVERSE_GRID, (n + 255) / 256);
   HIP_TRY(hipMemsetAsync(c->eng.fetch, 0, FETCH_BYTES, sm));
-  hipLaunchKernelGGL(k_wide_closest, dim3(grid), dim3(256), 0, sm, s->wide, drays, n, t_min, t_max, cull, dtri,
+  hipLaunchKernelGGL(k_wide_closest, dim3(grid), dim3(256), 0, sm, s->wide, s->d.tri, drays, n, t_min, t_max, cull, dtri,
                      dt, du, dv, dfront, c->eng.spill, TRAVERSE_THREADS, c->eng.counters, derr, c->eng.fetch);
   if (c->eng.timing) HIP_TRY(hipEventRecord(c->eng.ev[1], sm));
   HIP_TRY(hipGetLastError());

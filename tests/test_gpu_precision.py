@@ -41,12 +41,12 @@ def probe():
     for cull in (True, False):
         print(f'oracle probe hits, cull={cull}', flush=True)
         ref[cull] = _oracle_closest(sc.geometry, rays.astype(np.float64), cull)
-    yield gpu, rays, ref
+    yield gpu, rays, ref, sc.geometry
     gpu.release()
 
 
 def test_probe_set_1m(probe):
-    gpu, rays, ref = probe
+    gpu, rays, ref, _ = probe
     r64 = rays.astype(np.float64)
     report = {}
     for cull in (True, False):
@@ -66,12 +66,66 @@ def test_probe_set_1m(probe):
     print('PRECISION ' + json.dumps(report), flush=True)
 
 
+def _mt64(tris, k, rays):
+    """The reference's FP64 Möller-Trumbore (bvh.ts:228-252) of triangle k[i] against ray i, without
+    the range tests: (t, u, v), the barycentric weights of v1 and v2, as the reference computes them.
+    Used to classify first-hit disagreements, not to decide hits."""
+    t9 = tris.reshape(-1, 9)[k].astype(np.float64)
+    o, d = rays[:, :3], rays[:, 3:]
+    e1, e2 = t9[:, 3:6], t9[:, 6:9]
+    p = np.cross(d, e2)
+    det = np.einsum('ij,ij->i', e1, p)
+    inv = 1.0 / np.where(det == 0, np.nan, det)
+    s_ = o - t9[:, 0:3]
+    u = np.einsum('ij,ij->i', s_, p) * inv
+    q = np.cross(s_, e1)
+    v = np.einsum('ij,ij->i', d, q) * inv
+    t = np.einsum('ij,ij->i', e2, q) * inv
+    return t, u, v
+
+
+def _classify(geometry, rays, w_tri, o_tri, o_t, edge_tol=1e-4, tie_tol=1e-6):
+    """Sorts the rays whose first-hit triangle differs between the wide form and the FP64 oracle:
+    'tie' - both triangles lie at the same distance along the ray (within tie_tol relative), so
+    the choice between them is the order of an exact tie; 'edge' - the ray passes within edge_tol
+    (barycentric) of an edge of the triangle one side hit and the other did not, where FP32 and
+    FP64 rounding of the inclusive edge tests (bvh.ts:241-246) differ; 'real' - anything else."""
+    idx = np.flatnonzero(w_tri != o_tri)
+    out = {'tie': 0, 'edge': 0, 'real': 0}
+    if idx.size == 0:
+        return out, []
+    r = rays[idx].astype(np.float64)
+    real = []
+    for j, i in enumerate(idx):
+        wk, ok = int(w_tri[i]), int(o_tri[i])
+        rr = r[j:j + 1]
+        near_edge = False
+        tw = to = None
+        if wk >= 0:
+            t, u, v = _mt64(geometry.triangles, np.array([wk]), rr)
+            tw = t[0]
+            near_edge |= min(u[0], v[0], 1 - u[0] - v[0]) > -edge_tol and min(u[0], v[0], 1 - u[0] - v[0]) < edge_tol
+        if ok >= 0:
+            t, u, v = _mt64(geometry.triangles, np.array([ok]), rr)
+            to = t[0]
+            near_edge |= min(u[0], v[0], 1 - u[0] - v[0]) < edge_tol
+        if tw is not None and to is not None and abs(tw - to) <= tie_tol * abs(to):
+            out['tie'] += 1
+        elif near_edge:
+            out['edge'] += 1
+        else:
+            out['real'] += 1
+            real.append(int(i))
+    return out, real
+
+
 def test_probe_set_1m_wide(probe):
-    """The FP32 wide form against the FP64 oracle: the rate of rays whose first-hit triangle
-    differs, and the relative t error where it agrees (largest, and 99.99th percentile). FP32
-    rounding of the vertices relative to the ray origin moves t by a few parts in 10^7; where two
-    triangles meet it can choose the neighbour."""
-    gpu, rays, ref = probe
+    """The FP32 wide form against the FP64 oracle. The first hit is found in FP32 and its t, u, v
+    refined with the reference's FP64 test on that triangle, so where the ids agree t is the
+    oracle's rounded to FP32 (relative error <= 2^-24, inside the 1e-6 bound). Rays whose id
+    differs are classified: ties between triangles at the same distance, and rays through an
+    edge within FP32 rounding of it, are the only disagreements allowed."""
+    gpu, rays, ref, geometry = probe
     info = gpu.build_wide()
     report = {'wide_bvh': info}
     for cull in (True, False):
@@ -81,11 +135,12 @@ def test_probe_set_1m_wide(probe):
         mismatch = int(np.count_nonzero(w_tri != o_tri))
         both = (w_tri == o_tri) & (w_tri >= 0)
         rel = np.abs(w_t[both].astype(np.float64) - o_t[both]) / np.abs(o_t[both])
+        classes, real = _classify(geometry, rays, w_tri, o_tri, o_t)
         report[f'cull_{cull}'] = {
             'rays': len(rays), 'hits': int(both.sum()), 'id_mismatch_rate': mismatch / len(rays),
-            'max_rel_t_error': float(rel.max()), 'p9999_rel_t_error': float(np.percentile(rel, 99.99)),
-            'frac_rel_t_within_1e-6': float((rel <= 1e-6).mean()),
+            'id_mismatches': classes, 'max_rel_t_error': float(rel.max()),
             'front_mismatch': int(np.count_nonzero(w_f[both] != o_f[both]))}
-        assert mismatch / len(rays) <= 1e-3
-        assert np.percentile(rel, 99.99) <= 1e-5
+        assert classes['real'] == 0, real[:10]
+        assert rel.max() <= 1e-6
+        assert mismatch / len(rays) <= 1e-4
     print('PRECISION_WIDE ' + json.dumps(report), flush=True)
