@@ -32,13 +32,43 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sc, cam, st, width, height, seconds=15.0):
+def host_cpu():
+    """The host CPU's model, the cores this process may run on (its affinity mask) and the thread
+    count a CPU baseline uses: the affinity's size, at most 16 (the host share of one GPU)."""
+    model = 'unknown'
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    return {'model': model, 'affinity': _ranges(cpus), 'threads': max(1, min(16, len(cpus)))}
+
+
+def _ranges(cpus):
+    out, start = [], None
+    for i, c in enumerate(cpus):
+        if start is None:
+            start = c
+        if i + 1 == len(cpus) or cpus[i + 1] != c + 1:
+            out.append(f'{start}-{c}' if c != start else f'{c}')
+            start = None
+    return ','.join(out)
+
+
+def cpu_baseline(sc, cam, st, width, height, seconds=15.0, single_seconds=5.0):
     """Times the oracle on a sample of the frame's (tile, pass) work items in the reference's
     progressive order (64x64 tiles centre-first, pass-major from pass 1) with one thread per host
-    core, sized to about `seconds` of work."""
+    core, sized to about `seconds` of work; then one thread alone on about `single_seconds`."""
     import oracle
     from meercadraytracer_amd import abi
-    threads = os.cpu_count() or 1
+    host = host_cpu()
+    threads = host['threads']
     tiles = oracle.make_tiles(width, height, 64)
 
     def jobs_for(k):
@@ -56,7 +86,13 @@ def cpu_baseline(sc, cam, st, width, height, seconds=15.0):
     jobs = jobs_for(k)
     secs, cnt = oracle.render_items_mt(sc, cam, st, jobs, threads)
     rays = cnt['closest_queries'] + cnt['any_queries']
+    k1 = int(max(1, min(k, single_seconds / max(per_tile, 1e-6))))
+    secs1, cnt1 = oracle.render_items_mt(sc, cam, st, jobs_for(k1), 1)
+    rays1 = cnt1['closest_queries'] + cnt1['any_queries']
     return {'value': rays / secs / 1e6, 'unit': 'Mrays/s', 'cores': threads, 'kind': 'port',
+            'cpu_model': host['model'], 'affinity': host['affinity'],
+            'single_thread': {'value': round(rays1 / secs1 / 1e6, 3), 'unit': 'Mrays/s', 'items': k1,
+                              'seconds': round(secs1, 2)},
             'sample': f'{k} (tile, pass) items = {k / len(tiles):.2f} passes of {len(tiles)} centre-first 64x64 tiles '
                       f'of the same 1920x1080 frame from pass 1, '
                       f'{rays} queries in {secs:.2f} s (portable-math oracle, std::thread pool)'}
@@ -110,7 +146,7 @@ def measure_traffic(args, kernel, launches):
     outdir = tempfile.mkdtemp(prefix='mcr_pmc_', dir='/tmp')
     # Only the plain run: the other sections launch the same kernels and would mix into the tail.
     bench_args = ['--steps', str(args.steps), '--warmup', str(args.warmup), '--no-cpu', '--no-traffic',
-                  '--no-batched', '--no-served', '--no-precision', '--no-textures',
+                  '--no-batched', '--no-served', '--no-precision', '--no-textures', '--no-c0',
                   '--width', str(args.width), '--height', str(args.height)]
     copy_args = ['--copy-calibration', '4096']
     res = {}
@@ -152,7 +188,7 @@ def kernel_counters(args, passes=None, outdir=None):
     else:
         outdir = tempfile.mkdtemp(prefix='mcr_pmc_', dir='/tmp')
     bench_args = ['--steps', str(args.steps), '--warmup', '0', '--no-cpu', '--no-traffic',
-                  '--no-batched', '--no-served', '--no-precision', '--no-textures',
+                  '--no-batched', '--no-served', '--no-precision', '--no-textures', '--no-c0',
                   '--width', str(args.width), '--height', str(args.height)]
     raw, errors = {}, {}
     for counters in (passes or COUNTER_PASSES):
@@ -207,7 +243,7 @@ def c0(ctx):
     gpu_rays = cnt.closest_queries + cnt.any_queries
     tiles = oracle.make_tiles(256, 256, 64)
     jobs = [abi.TileJob(abi.Tile(*tl), 256, 256, p, 1, 0) for p in range(16) for tl in tiles]
-    threads = os.cpu_count() or 1
+    threads = host_cpu()['threads']
     secs, c = oracle.render_items_mt(sc, cam, trace_settings(o), jobs, threads)
     cpu_rays = c['closest_queries'] + c['any_queries']
     return {'workload': 'plain_cpu: 10,242 triangles, 256x256, 16 spp, 4 bounces, GI off, ACES, sRGB',
@@ -215,6 +251,41 @@ def c0(ctx):
                     'note': 'raytrace_compiled: scene upload, 16 passes, encode and readback'},
             'cpu': {'seconds': round(secs, 3), 'rays': int(cpu_rays), 'mrays_s': round(cpu_rays / secs / 1e6, 2),
                     'cores': threads, 'kind': 'port'}}
+
+
+def builder(ctx, repeats=10):
+    """The binned-SAH builder on the plain GPU scene's 1,376,032 triangles already in HBM
+    (mcr_build_bvh_device), `repeats` builds after one warm-up: host wall time per build."""
+    from meercadraytracer_amd import native, scenes
+    from meercadraytracer_amd.compile import compile_scene
+    captured = {}
+
+    def keep(tris, count):
+        captured['tris'], captured['count'] = tris.copy(), count
+        return ctx.build_bvh(tris, count)
+    scene, _, _ = scenes.plain_gpu_scene(0)
+    compile_scene(scene, keep)
+    tris, n = captured['tris'], captured['count']
+    dt = native.DeviceBuffer(ctx, tris.nbytes)
+    dt.upload(tris)
+    db = native.DeviceBuffer(ctx, (2 * n - 1) * 24)
+    dn = native.DeviceBuffer(ctx, (2 * n - 1) * 12)
+    do = native.DeviceBuffer(ctx, n * 4)
+    import ctypes as C
+    nc = C.c_uint32()
+    L = native.lib()
+    walls = []
+    for i in range(repeats + 1):
+        t = time.perf_counter()
+        native.check(L.mcr_build_bvh_device(ctx.handle, dt.ptr, n, db.ptr, dn.ptr, do.ptr, C.byref(nc)))
+        if i:
+            walls.append((time.perf_counter() - t) * 1e3)
+    for b in (dt, db, dn, do):
+        b.free()
+    walls.sort()
+    return {'workload': 'plain_gpu triangles (1,376,032) in HBM, mcr_build_bvh_device', 'triangles': int(n),
+            'nodes': int(nc.value), 'builds': repeats, 'ms_median': round(walls[len(walls) // 2], 2),
+            'ms_min': round(walls[0], 2), 'ms_max': round(walls[-1], 2)}
 
 
 def served(args, ctx, workloads):
@@ -267,7 +338,8 @@ def main():
     ap.add_argument('--served-slots', type=int, default=0, help='server path slots (0 = default)')
     ap.add_argument('--no-precision', action='store_true', help='skip the precision-form probe measurement')
     ap.add_argument('--no-textures', action='store_true', help='skip the textured-workload measurement')
-    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0', 'precision', 'textures', 'counters'],
+    ap.add_argument('--no-c0', action='store_true', help='skip the CPU configuration (C0) on GPU and host')
+    ap.add_argument('--only', choices=['plain', 'batched', 'served', 'c0', 'precision', 'textures', 'counters', 'build'],
                     default=None,
                     help='run one workload and print its JSON object only')
     ap.add_argument('--pmc-dir', default=None, help='--only counters: keep the rocprofv3 output here')
@@ -306,6 +378,9 @@ def main():
         return
     if args.only == 'c0':
         print(json.dumps(c0(ctx)), flush=True)
+        return
+    if args.only == 'build':
+        print(json.dumps(builder(ctx)), flush=True)
         return
     if args.only == 'textures':
         print(json.dumps(workloads.textured_form(ctx, W, H, log=log)), flush=True)
@@ -436,6 +511,9 @@ def main():
     if not args.no_served:
         log('served mix ...')
         result['served'] = served(args, ctx, workloads)
+    if not args.no_c0:
+        log('C0 (the CPU configuration) on the GPU and on the host ...')
+        result['c0'] = c0(ctx)
     if not args.no_cpu:
         log('cpu baseline ...')
         result['cpu_baseline'] = cpu_baseline(sc, cam, st, W, H, args.cpu_seconds)

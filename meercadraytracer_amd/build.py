@@ -43,5 +43,26 @@ def build(verbose=False):
     return LIB
 
 
+NAPI_SRC = os.path.join(HERE, 'napi', 'mcr_napi.c')
+NAPI_LIB = os.path.join(HERE, 'napi', 'mcr.node')
+NODE_INCLUDE = '/usr/include/node'
+
+
+def build_napi():
+    """Builds the Node-API addon napi/mcr.node against libmcr.so (gcc; Node's headers must be
+    installed). Returns its path, or None where the headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, 'node_api.h')):
+        return None
+    tmp = NAPI_LIB + '.tmp'
+    cmd = ['gcc', '-O2', '-Wall', '-shared', '-fPIC', '-I' + NODE_INCLUDE, NAPI_SRC, '-L' + HERE, '-lmcr',
+           '-Wl,-rpath,$ORIGIN/..', '-o', tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f'addon build failed: {r.stdout}\n{r.stderr}')
+    os.replace(tmp, NAPI_LIB)
+    return NAPI_LIB
+
+
 if __name__ == '__main__':
     print(build(verbose='-v' in sys.argv))
+    print(build_napi())

@@ -57,11 +57,24 @@ def test_tone_mapping_operators(libm):  # color.test.ts:34-57
     assert abs(grey[0] - grey[1]) < 5e-3 and abs(grey[1] - grey[2]) < 5e-3
 
 
+def _dfg_table():
+    """The 512 half floats of csrc/dfg_lut.h (the table the device and the oracle both compile)."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'meercadraytracer_amd', 'csrc',
+                        'dfg_lut.h')
+    body = open(path).read()
+    body = body[body.index('{'):body.rindex('}')]
+    vals = [int(h, 16) for h in re.findall(r'0x([0-9a-fA-F]{4})', body)]
+    assert len(vals) == 512
+    return vals
+
+
 def test_dfg_texel_centres():  # color.test.ts:61-70
     lib = oracle.load()
     out = np.zeros(2)
     lib.orc_dfg(3.5 / 16, 5.5 / 16, out.ctypes.data)
-    from meercadraytracer_amd.csrc_data import DFG_LUT_HALF
+    DFG_LUT_HALF = _dfg_table()
     assert abs(out[0] - lib.orc_half_to_float(DFG_LUT_HALF[(5 * 16 + 3) * 2])) < 5e-7
     assert abs(out[1] - lib.orc_half_to_float(DFG_LUT_HALF[(5 * 16 + 3) * 2 + 1])) < 5e-7
     lib.orc_dfg(0, 1, out.ctypes.data)
