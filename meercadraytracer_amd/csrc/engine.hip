@@ -112,6 +112,7 @@ int engine_init(Engine &e, int device, hipStream_t stream) {
 void engine_free(Engine &e) {
   if (e.stream) hipStreamSynchronize(e.stream);
   engine_drop_graphs(e);
+  if (e.capture_stream) hipStreamDestroy(e.capture_stream);
   hipFree(e.w_block);
   hipFree(e.spill);
   hipFree(e.seen);
@@ -238,8 +239,8 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
   if (!graph) {
     // Grids for every slot: a replay does not know how many paths are live. Blocks past the live
     // count return at once, and traversal waves leave as soon as the shared counter runs out.
-    hipStream_t cs;
-    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    if (!e.capture_stream) HIP_TRY(hipStreamCreateWithFlags(&e.capture_stream, hipStreamNonBlocking));
+    hipStream_t cs = e.capture_stream;
     hipGraph_t g = nullptr;
     hipError_t err = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (err == hipSuccess) {
@@ -259,7 +260,6 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
     }
     if (err == hipSuccess) err = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
     if (g) hipGraphDestroy(g);
-    hipStreamDestroy(cs);
     if (err != hipSuccess) {
       graph = nullptr;
       return mcr_fail(MCR_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(err));
