@@ -343,6 +343,8 @@ def main():
                     default=None,
                     help='run one workload and print its JSON object only')
     ap.add_argument('--pmc-dir', default=None, help='--only counters: keep the rocprofv3 output here')
+    ap.add_argument('--shade-sort', choices=['on', 'off'], default=None,
+                    help='material-sorted shading on or off (default: the library default)')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB (for scaling HBM counters)')
     args = ap.parse_args()
@@ -370,6 +372,8 @@ def main():
 
     W, H = args.width, args.height
     ctx = native.default_context()
+    if args.shade_sort:
+        ctx.set_shade_sort(args.shade_sort == 'on')
     if args.only == 'batched':
         print(json.dumps(workloads.batched_form(ctx, log=log)), flush=True)
         return
@@ -483,6 +487,11 @@ def main():
                                            'resident': round(resident_s * 1e3, 1)}},
         'roofline': roofline,
     }
+    if args.shade_sort == 'on':
+        lanes = c['shade_class_lanes']
+        result['config']['shade_sort'] = {
+            'class_lanes': lanes,
+            'class_mixed_wave_frac': round(c['class_mixed_waves'] / max(1, sum(lanes) / 64), 4)}
     if not args.no_traffic:
         log('HBM traffic passes (rocprofv3 FETCH_SIZE / WRITE_SIZE) ...')
         try:

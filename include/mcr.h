@@ -197,6 +197,10 @@ typedef struct mcr_counters {
   uint64_t closest_node_visits, closest_triangle_tests;  /* split by query kind, for the */
   uint64_t any_node_visits, any_triangle_tests;          /* per-kernel algorithmic bytes   */
   uint64_t shade_waves, mixed_view_waves;  /* batched runs: shade waves, and those mixing views */
+  /* material-sorted shading (mcr_set_shade_sort): paths shaded per shade class (0 miss, 1 basic,
+   * 2 lambert/toon, 3 phong, 4 physical, 5 physical with maps, 6 transmissive, 7 see-through), and
+   * the shade waves that would mix classes in the unsorted queue order */
+  uint64_t shade_class_lanes[8], class_mixed_waves;
 } mcr_counters;
 
 typedef struct mcr_ctx mcr_ctx;
@@ -363,7 +367,7 @@ int mcr_intersect_any_count(mcr_ctx *ctx, mcr_scene *scene, const double *rays, 
  * overlap on one GPU (two streams, two threads) race and must be serialised by the caller, for
  * example by issuing them on one stream. n is at most MCR_NOALLOC_MAX_RAYS; larger batches are
  * refused with MCR_E_INVALID. */
-#define MCR_NOALLOC_SCRATCH_BYTES 58729472u   /* 512 x 256 threads x 112 stack words + counters */
+#define MCR_NOALLOC_SCRATCH_BYTES 58737664u   /* 512 x 256 threads x 112 stack words + counters */
 #define MCR_NOALLOC_MAX_RAYS 0x80000000u
 int mcr_intersect_closest_noalloc(mcr_scene *scene, const double *rays, uint32_t n, double t_min,
                                   double t_max, int cull, int32_t *tri_out, double *t_out,
@@ -487,6 +491,12 @@ int mcr_set_wavefront_capacity(mcr_ctx *ctx, uint32_t slots);
 /* Replays bounce iterations from a captured HIP graph (default on; off launches each kernel with
  * grids sized to the live paths). Timing mode always launches directly. Results are the same. */
 int mcr_set_graph(mcr_ctx *ctx, int enabled);
+/* Material-sorted shading (default on): before each shade launch the live paths are binned by the
+ * branch their vertex takes (a miss, the material model, maps, transmission, see-through), so the
+ * lanes of a wave shade alike, and misses are shaded by a light kernel of their own. Each sample
+ * keeps its own random stream, so results are the same either way. No counterpart in the
+ * reference (tracer.ts shades one sample at a time). */
+int mcr_set_shade_sort(mcr_ctx *ctx, int enabled);
 
 #ifdef __cplusplus
 }

@@ -137,10 +137,12 @@ class Counters(C.Structure):
         ('closest_node_visits', C.c_uint64), ('closest_triangle_tests', C.c_uint64),
         ('any_node_visits', C.c_uint64), ('any_triangle_tests', C.c_uint64),
         ('shade_waves', C.c_uint64), ('mixed_view_waves', C.c_uint64),
+        ('shade_class_lanes', C.c_uint64 * 8), ('class_mixed_waves', C.c_uint64),
     ]
 
     def as_dict(self):
-        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+        return {name: (list(map(int, getattr(self, name))) if name == 'shade_class_lanes' else int(getattr(self, name)))
+                for name, _ in self._fields_}
 
 
 class Timing(C.Structure):
@@ -201,7 +203,7 @@ class Request(C.Structure):
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 MCR_E_CANCELLED, MCR_E_TIMEOUT, MCR_E_INTERNAL = -6, -7, -8
 MCR_ISECT_FP64, MCR_ISECT_WIDE_FP32 = 0, 1
-MCR_NOALLOC_SCRATCH_BYTES, MCR_NOALLOC_MAX_RAYS = 58729472, 0x80000000  # mcr_intersect_closest_noalloc
+MCR_NOALLOC_SCRATCH_BYTES, MCR_NOALLOC_MAX_RAYS = 58737664, 0x80000000  # mcr_intersect_closest_noalloc
 
 
 class RequestStatus(C.Structure):

@@ -305,6 +305,13 @@ int mcr_set_graph(mcr_ctx *c, int enabled) {
   return MCR_OK;
 }
 
+int mcr_set_shade_sort(mcr_ctx *c, int enabled) {
+  if (!c) return fail(MCR_E_INVALID, "ctx is null");
+  std::lock_guard<std::mutex> lock(c->mu);
+  c->eng.sort_shade = enabled != 0;
+  return MCR_OK;
+}
+
 int mcr_last_timing(mcr_ctx *c, mcr_timing *out) {
   if (!c || !out) return fail(MCR_E_INVALID, "null argument");
   *out = c->eng.last_timing;

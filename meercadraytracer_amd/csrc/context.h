@@ -83,6 +83,10 @@ struct Engine {
   // Two captured bounce iterations (queue 0 then queue 1), replayed while paths are live; one
   // graph per kernel variant (single or mixed views, FP64 binary or FP32 wide traversal).
   bool use_graph = true;
+  // Bins the live queue by shade class (miss, material model, transmissive, see-through) before
+  // shade, so a wave's lanes take the same branch and misses are shaded by a light kernel of
+  // their own; results do not depend on it.
+  bool sort_shade = true;
   hipStream_t capture_stream = nullptr;  // captures the graphs; lives as long as the engine
   hipGraphExec_t graph[8] = {};
   const void *graph_key[8][4] = {};

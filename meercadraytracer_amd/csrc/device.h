@@ -103,13 +103,16 @@ struct DevCounters {  // accumulated per wave, flushed with one atomic per field
   unsigned long long closest_queries, any_queries, node_visits, leaf_visits, triangle_tests, paths;
   unsigned long long closest_node_visits, closest_triangle_tests, any_node_visits, any_triangle_tests;
   unsigned long long shade_waves, mixed_view_waves;  // shade waves, and those holding several views
+  // Material-sorted shading (k_shade_bin): live paths per shade class, and the waves of the
+  // unsorted queue order whose lanes hold more than one class.
+  unsigned long long shade_class_lanes[8], class_mixed_waves;
 };
 
-// The counters live in MCR_COUNTER_COPIES copies, 128 bytes apart, and block b adds to copy
+// The counters live in MCR_COUNTER_COPIES copies, 256 bytes apart, and block b adds to copy
 // b % MCR_COUNTER_COPIES: thousands of waves flushing at a kernel's end then spread over as many
 // cache lines instead of queueing on one. Readers sum the copies.
 #define MCR_COUNTER_COPIES 64
-#define MCR_COUNTER_STRIDE 128
+#define MCR_COUNTER_STRIDE 256
 static_assert(sizeof(DevCounters) <= MCR_COUNTER_STRIDE, "one counter copy per stride");
 
 __device__ inline DevCounters *counter_copy(DevCounters *base) {

@@ -41,6 +41,7 @@ int alloc_wavefront(Engine &e, uint32_t capacity, uint32_t lights) {
   size_t o_sref = take(sizeof(uint32_t) * regions * (size_t)region_cap);
   size_t o_free = take(sizeof(int32_t) * P);
   size_t o_deep = take(sizeof(double) * (size_t)MCR_DEEP_HITS * MCR_DEEP_THREADS);
+  size_t o_bins = take(sizeof(int32_t) * MCR_SHADE_CLASSES * P);
   HIP_TRY(hipMalloc(&e.w_block, bytes));
   char *b = (char *)e.w_block;
   Wavefront &w = e.w;
@@ -60,6 +61,7 @@ int alloc_wavefront(Engine &e, uint32_t capacity, uint32_t lights) {
   w.sref = (uint32_t *)(b + o_sref);
   w.free_list = (int32_t *)(b + o_free);
   w.deep_seen = (double *)(b + o_deep);
+  w.bins = (int32_t *)(b + o_bins);
   w.ctl = e.ctl;
   e.capacity = capacity;
   e.lights = lights;
@@ -201,7 +203,8 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
   launch_extend(sm, e.mix, e.wide, e.views, w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(live)), e.spill,
                 TRAVERSE_THREADS, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[1], sm));
-  launch_shade(sm, e.mix, e.views, w, q, live, e.counters);
+  if (e.sort_shade) launch_shade_bin(sm, e.mix, e.views, w, q, live, e.counters);
+  launch_shade(sm, e.mix, e.sort_shade, e.views, w, q, live, e.counters);
   if (timed) HIP_TRY(hipEventRecord(e.ev[2], sm));
   uint64_t max_shadow = (uint64_t)live * max_lights;
   if (max_shadow) {
@@ -229,7 +232,7 @@ int engine_iteration(Engine &e, int q, uint32_t live, uint32_t max_lights, doubl
 int engine_iteration_pair(Engine &e, uint32_t max_lights) {
   int v = (e.mix ? 1 : 0) + (e.wide ? 2 : 0) + (e.tex ? 4 : 0);
   const void *key[4] = {e.w_block, e.views, e.counters,
-                        (const void *)(uintptr_t)((uint64_t)e.capacity * 64u + max_lights)};
+                        (const void *)(uintptr_t)(((uint64_t)e.capacity * 64u + max_lights) * 2u + e.sort_shade)};
   hipGraphExec_t &graph = e.graph[v];
   if (graph && memcmp(key, e.graph_key[v], sizeof(key)) != 0) {
     HIP_TRY(hipStreamSynchronize(e.stream));  // a launch of the old graph may still be running
@@ -248,7 +251,8 @@ int engine_iteration_pair(Engine &e, uint32_t max_lights) {
       for (int q = 0; q < 2; q++) {
         launch_extend(cs, e.mix, e.wide, e.views, e.w, q, std::min<uint32_t>(TRAVERSE_GRID, blocks_of(cap)), e.spill,
                       TRAVERSE_THREADS, e.counters);
-        launch_shade(cs, e.mix, e.views, e.w, q, cap, e.counters);
+        if (e.sort_shade) launch_shade_bin(cs, e.mix, e.views, e.w, q, cap, e.counters);
+        launch_shade(cs, e.mix, e.sort_shade, e.views, e.w, q, cap, e.counters);
         if (max_lights) {
           launch_shadow(cs, e.mix, e.wide, e.tex, e.views, e.w, std::min<uint32_t>(TRAVERSE_GRID, blocks_of((uint64_t)cap * max_lights)),
                         e.spill, TRAVERSE_THREADS, e.seen, e.seen_meta, e.seen_uv, e.counters);
@@ -326,6 +330,8 @@ void engine_add_counters(const DevCounters &hc, uint64_t paths, mcr_counters *co
   counters->any_triangle_tests += hc.any_triangle_tests;
   counters->shade_waves += hc.shade_waves;
   counters->mixed_view_waves += hc.mixed_view_waves;
+  for (int c = 0; c < 8; c++) counters->shade_class_lanes[c] += hc.shade_class_lanes[c];
+  counters->class_mixed_waves += hc.class_mixed_waves;
 }
 
 int engine_run(Engine &e, const std::vector<ViewDev> &views, const std::vector<JobDev> &jobs,

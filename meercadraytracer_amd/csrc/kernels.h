@@ -19,7 +19,11 @@ void launch_sort_queue(hipStream_t s, const Wavefront &w, int q, uint32_t live);
 // wide: the run traces with the precision form's FP32 wide BVH (every view's scene has one).
 void launch_extend(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w, int q,
                    uint32_t grid, int32_t *spill, uint32_t spill_stride, DevCounters *cnt);
-void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+// Bins the live slots of queue q by shade class into Wavefront::bins (counts in C_BIN words).
+void launch_shade_bin(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                      uint32_t max_live, DevCounters *cnt);
+// binned: shade walks the bins of launch_shade_bin instead of queue q.
+void launch_shade(hipStream_t s, bool mix, bool binned, const ViewDev *views, const Wavefront &w, int q,
                   uint32_t max_live, DevCounters *cnt);
 void launch_shadow(hipStream_t s, bool mix, bool wide, bool tex, const ViewDev *views, const Wavefront &w, uint32_t grid,
                    int32_t *spill, uint32_t spill_stride, double *seen, int2 *seen_meta, double2 *seen_uv,

@@ -273,6 +273,7 @@ __global__ __launch_bounds__(256, MIX ? 4 : 5) void k_extend(const ViewDev *__re
                                                 DevCounters *cnt) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid < 2 * MCR_SHADOW_REGIONS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;  // shade of this iteration appends
+  if (gid < MCR_SHADE_CLASSES) w.counts[cw(C_BIN + gid)] = 0;                  // its shade bins
   if (gid == 0) w.counts[cw(C_DEEP)] = 0;                                       // shadow of this iteration marks
   MCR_LANE_STACK(st, spill, spill_stride);
   extend_loop<MIX, false>(views, w, q, st, cnt);
@@ -285,6 +286,7 @@ __global__ __launch_bounds__(256) void k_extend_wide(const ViewDev *__restrict__
   __shared__ uint32_t lds_base[MCR_WIDE_STACK_LDS * 256], lds_bits[MCR_WIDE_STACK_LDS * 256];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid < 2 * MCR_SHADOW_REGIONS) w.counts[cw(C_SHADOW_SEG + gid)] = 0;
+  if (gid < MCR_SHADE_CLASSES) w.counts[cw(C_BIN + gid)] = 0;
   if (gid == 0) w.counts[cw(C_DEEP)] = 0;
   WideStack st{lds_base + threadIdx.x, lds_bits + threadIdx.x, spill + gid, (int)spill_stride, 256, 0};
   extend_loop<MIX, true>(views, w, q, st, cnt);
@@ -382,6 +384,32 @@ MCR_INLINE void background(const ViewDev &V, const Control &ctl, uint32_t smp, i
 // Writes the vertex's unshadowed light terms and shadow rays into the light slots, the values
 // resolve needs to finish `lr += tr * (outR + er)`, and the continuation ray (the continuation
 // uses only the random stream and the material, never the shadow results, so it runs here).
+// A path whose ray met nothing: the camera ray's background, written as the sample, or the last
+// surface's environment added to the radiance before the sample is written (tracer.ts:119-124).
+// Returns the slot's PathMode.
+MCR_INLINE int shade_miss(const ViewDev &V, const Wavefront &w, const Control &ctl, int slot, int flags,
+                          double dx, double dy, double dz, double tr, double tg, double tb) {
+  size_t P = w.capacity;
+  if (!(flags & PF_SURFACE)) {
+    double c[4];
+    uint32_t smp = (uint32_t)w.i[I_SAMPLE * P + slot];
+    int job = job_of(ctl.jobs, ctl.njobs, smp, w.i[I_JOB * P + slot]);
+    background(V, ctl, smp, job, dx, dy, dz, c);
+    write_sample(ctl.jobs[job], ctl.out, smp, c);
+    return MODE_DONE;
+  }
+  int le = w.i[I_ENV * P + slot];
+  if (le >= 0) {
+    double env[3];
+    double scale = w.f[F_ENVS * P + slot];
+    env_sample(V.sc.environments[le], V.sc.textures, dx, dy, dz, env);
+    w.f[F_LR * P + slot] += tr * env[0] * scale;
+    w.f[F_LG * P + slot] += tg * env[1] * scale;
+    w.f[F_LB * P + slot] += tb * env[2] * scale;
+  }
+  return MODE_FINISH;
+}
+
 MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Control &ctl,
                                      int slot) {
   const DevScene &sc = V.sc;
@@ -405,25 +433,7 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
     int mode;
     for (uint32_t l = 0; l < V.shadow_slots; l++) w.status[(size_t)l * P + slot] = LS_NONE;
     if (k < 0) {
-      if (!(flags & PF_SURFACE)) {
-        double c[4];
-        uint32_t smp = (uint32_t)iv[I_SAMPLE * P + slot];
-        int job = job_of(ctl.jobs, ctl.njobs, smp, iv[I_JOB * P + slot]);
-        background(V, ctl, smp, job, dx, dy, dz, c);
-        write_sample(ctl.jobs[job], ctl.out, smp, c);
-        mode = MODE_DONE;
-      } else {
-        int le = iv[I_ENV * P + slot];
-        if (le >= 0) {  // the last surface's environment (tracer.ts:119-124)
-          double env[3];
-          double scale = f[F_ENVS * P + slot];
-          env_sample(sc.environments[le], sc.textures, dx, dy, dz, env);
-          f[F_LR * P + slot] += tr * env[0] * scale;
-          f[F_LG * P + slot] += tg * env[1] * scale;
-          f[F_LB * P + slot] += tb * env[2] * scale;
-        }
-        mode = MODE_FINISH;
-      }
+      mode = shade_miss(V, w, ctl, slot, flags, dx, dy, dz, tr, tg, tb);
     } else {
       double t = f[F_HT * P + slot];
       double mr = f[F_MR * P + slot], mg = f[F_MG * P + slot], mb = f[F_MB * P + slot];
@@ -912,11 +922,129 @@ MCR_INLINE uint32_t shade_slot(const ViewDev &V, const Wavefront &w, const Contr
   return nshadow;
 }
 
+// ---- shade classes --------------------------------------------------------------------------------
+// The branch of shade_slot a path takes at its vertex: 0 a miss (background or environment,
+// tracer.ts:119-124), 1 basic, 2 lambert or toon, 3 phong, 4 physical, 5 physical with maps,
+// 6 physical with transmission (tracer.ts:254-389), 7 a material the path may pass straight
+// through (alphaTest, transparent: tracer.ts:170-176).
+__device__ inline uint32_t shade_class(const DevScene &sc, int k) {
+  if (k < 0) return 0;
+  const DevMaterial &m = sc.materials[__float_as_uint(sc.tri[(size_t)k * 3 + 2].z)];
+  if (m.alpha_test > 0 || m.transparent) return 7;
+  if (m.model == MCR_MODEL_BASIC) return 1;
+  if (m.model == MCR_MODEL_LAMBERT || m.model == MCR_MODEL_TOON) return 2;
+  if (m.model == MCR_MODEL_PHONG) return 3;
+  return m.transmission > 0 ? 6 : m.textured ? 5 : 4;
+}
+
+// Bins the live slots of queue q by shade class: class c's slots go to region c of w.bins, in
+// queue order within each block, and word C_BIN + c counts them. Shade then walks the regions one
+// after another, so all but at most MCR_SHADE_CLASSES - 1 of its waves hold one class. Every
+// sample keeps its own random stream, so the order shade visits slots in changes no result.
 template <bool MIX>
+__global__ __launch_bounds__(256) void k_shade_bin(const ViewDev *__restrict__ views, Wavefront w, int q,
+                                                   DevCounters *cnt) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = w.counts[cw(q)];
+  if (blockIdx.x * blockDim.x >= n) return;  // every thread of a block that goes on reaches the append
+  bool live = gid < n;
+  size_t P = w.capacity;
+  int slot = live ? w.queue[q][gid] : 0;
+  uint32_t cls = 0;
+  if (live) {
+    int k = w.i[I_HTRI * P + slot];
+    uint32_t view = MIX ? (uint32_t)w.i[I_VIEW * P + slot] : 0;
+    with_view<MIX>(views, view, [&](const ViewDev &V) { cls = shade_class(V.sc, k); });
+  }
+  uint32_t *ctr[MCR_SHADE_CLASSES];
+  bool want[MCR_SHADE_CLASSES];
+  uint32_t pos[MCR_SHADE_CLASSES];
+#pragma unroll
+  for (int c = 0; c < MCR_SHADE_CLASSES; c++) {
+    ctr[c] = &w.counts[cw(C_BIN + c)];
+    want[c] = live && cls == (uint32_t)c;
+  }
+  __shared__ uint32_t lds[4 * MCR_SHADE_CLASSES];
+  block_append<MCR_SHADE_CLASSES>(ctr, want, lds, pos);
+  // The class mix shade would see in queue order (a wave's lanes in more than one class), and the
+  // lanes per class: block totals from the wave ballots, one atomic per class per block.
+  {
+    uint32_t kinds = 0;
+#pragma unroll
+    for (int c = 0; c < MCR_SHADE_CLASSES; c++) kinds += __ballot(want[c]) != 0;
+    __shared__ uint32_t mixed_waves, lanes[MCR_SHADE_CLASSES];
+    if (threadIdx.x < MCR_SHADE_CLASSES) lanes[threadIdx.x] = 0;
+    if (threadIdx.x == 0) mixed_waves = 0;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+      if (kinds > 1) atomicAdd(&mixed_waves, 1u);
+    }
+    if (live) atomicAdd(&lanes[cls], 1u);
+    __syncthreads();
+    DevCounters *cc = counter_copy(cnt);
+    if (threadIdx.x < MCR_SHADE_CLASSES && lanes[threadIdx.x])
+      atomicAdd(&cc->shade_class_lanes[threadIdx.x], (unsigned long long)lanes[threadIdx.x]);
+    if (threadIdx.x == 0 && mixed_waves) atomicAdd(&cc->class_mixed_waves, (unsigned long long)mixed_waves);
+  }
+  uint32_t at = 0;
+#pragma unroll
+  for (int c = 0; c < MCR_SHADE_CLASSES; c++)
+    if (cls == (uint32_t)c) at = pos[c];
+  if (live) w.bins[(size_t)cls * P + at] = slot;
+}
+
+// Entry i of the binned queue: the regions of w.bins one after another.
+__device__ inline int binned_slot(const Wavefront &w, uint32_t i) {
+  uint32_t base = 0, c = 0;
+  for (; c + 1 < MCR_SHADE_CLASSES; c++) {
+    uint32_t k = w.counts[cw(C_BIN + c)];
+    if (i < base + k) break;
+    base += k;
+  }
+  return w.bins[(size_t)c * w.capacity + (i - base)];
+}
+
+// The misses of a binned queue (class 0, region 0 of w.bins): the miss branch of shade_slot on its
+// own, in a kernel light enough to run many waves per SIMD. The light-slot statuses of these slots
+// are left as they are: resolve reads them only for vertices that add light, and the shadow kernels
+// only for rays shade queued in this iteration.
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_shade_miss(const ViewDev *__restrict__ views, Wavefront w,
+                                                    DevCounters *cnt) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = w.counts[cw(C_BIN)];
+  if (gid >= n) return;
+  size_t P = w.capacity;
+  int slot = w.bins[gid];
+  uint32_t view = MIX ? (uint32_t)w.i[I_VIEW * P + slot] : 0;
+  if (MIX) {  // shade waves, as k_shade counts them
+    uint64_t lanes = __ballot(true);
+    int leader = __ffsll((unsigned long long)lanes) - 1;
+    uint32_t v0 = __shfl(view, leader, 64);
+    bool mixed = __ballot(view != v0) != 0;
+    if ((int)(threadIdx.x & 63) == leader) {
+      DevCounters *cc = counter_copy(cnt);
+      atomicAdd(&cc->shade_waves, 1ull);
+      if (mixed) atomicAdd(&cc->mixed_view_waves, 1ull);
+    }
+  }
+  int flags = w.i[I_FLAGS * P + slot];
+  double dx = w.f[F_DX * P + slot], dy = w.f[F_DY * P + slot], dz = w.f[F_DZ * P + slot];
+  double tr = w.f[F_TR * P + slot], tg = w.f[F_TG * P + slot], tb = w.f[F_TB * P + slot];
+  const Control &ctl = *w.ctl;
+  int mode = MODE_DONE;
+  with_view<MIX>(views, view, [&](const ViewDev &V) { mode = shade_miss(V, w, ctl, slot, flags, dx, dy, dz, tr, tg, tb); });
+  w.mode[slot] = (uint8_t)mode;
+}
+
+// BINNED: the slots come from k_shade_bin's regions instead of queue q, past the misses of region
+// 0 (k_shade_miss shades those).
+template <bool MIX, bool BINNED>
 __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views, Wavefront w, int q,
                                                DevCounters *cnt) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = w.counts[cw(q)];
+  uint32_t skip = BINNED ? w.counts[cw(C_BIN)] : 0;
+  uint32_t n = w.counts[cw(q)] - skip;
   if (gid == 0) {
     w.counts[cw(q ^ 1)] = 0;  // resolve of this iteration queues into it
     for (int k = 0; k < MCR_FETCH_COUNTERS; k++) {
@@ -928,7 +1056,7 @@ __global__ __launch_bounds__(256) void k_shade(const ViewDev *__restrict__ views
   // Every thread of the block reaches the queue append below, so the bound check guards the work only.
   bool live = gid < n;
   size_t P = w.capacity;
-  int slot = live ? w.queue[q][gid] : 0;
+  int slot = !live ? 0 : BINNED ? binned_slot(w, gid + skip) : w.queue[q][gid];
   uint32_t nshadow = 0, nl = 0;
   bool first_hit = live && w.i[I_BOUNCE * P + slot] == 0;  // the vertex a camera ray found
   uint32_t view = live ? (uint32_t)w.i[I_VIEW * P + slot] : 0;
@@ -1669,9 +1797,21 @@ void launch_extend(hipStream_t s, bool mix, bool wide, const ViewDev *views, con
   if (wide) MCR_LAUNCH_MIX(k_extend_wide, grid, s, views, w, q, spill, spill_stride, cnt);
   else MCR_LAUNCH_MIX(k_extend, grid, s, views, w, q, spill, spill_stride, cnt);
 }
-void launch_shade(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+void launch_shade_bin(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
+                      uint32_t max_live, DevCounters *cnt) {
+  MCR_LAUNCH_MIX(k_shade_bin, blocks_for(max_live ? max_live : 1), s, views, w, q, cnt);
+}
+void launch_shade(hipStream_t s, bool mix, bool binned, const ViewDev *views, const Wavefront &w, int q,
                   uint32_t max_live, DevCounters *cnt) {
-  MCR_LAUNCH_MIX(k_shade, blocks_for(max_live ? max_live : 1), s, views, w, q, cnt);
+  uint32_t grid = blocks_for(max_live ? max_live : 1);
+  if (binned) {
+    MCR_LAUNCH_MIX(k_shade_miss, grid, s, views, w, cnt);
+    if (mix) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, true>), dim3(grid), dim3(256), 0, s, views, w, q, cnt);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>), dim3(grid), dim3(256), 0, s, views, w, q, cnt);
+  } else {
+    if (mix) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), dim3(grid), dim3(256), 0, s, views, w, q, cnt);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, false>), dim3(grid), dim3(256), 0, s, views, w, q, cnt);
+  }
 }
 template <bool TEX>
 static void launch_shadow_tex(hipStream_t s, bool mix, bool wide, const ViewDev *views, const Wavefront &w,

@@ -17,6 +17,7 @@
 #define MCR_DEEP_HITS 1024         // counted see-through surfaces a replayed shadow ray may cross
 #define MCR_DEEP_THREADS (64 * 256)  // threads of the shadow kernel's reference-order pass
 #define MCR_SLOT_BITS 26           // a shadow-queue entry: light << 26 | slot (capacity <= 2^26)
+#define MCR_SHADE_CLASSES 8        // shade classes the live queue is binned by before shade
 
 // Double fields of a path slot.
 enum PathField {
@@ -78,7 +79,10 @@ enum CountWord {
   // of that region).
   C_SHADOW_SEG = C_SHADOW_FETCH + MCR_FETCH_COUNTERS,
   C_SHADOW_SEG_END = C_SHADOW_SEG + 2 * MCR_SHADOW_REGIONS,
-  C_WORDS = C_SHADOW_SEG_END,
+  // Live slots binned by shade class before shade (k_shade_bin), one word per class.
+  C_BIN = C_SHADOW_SEG_END,
+  C_BIN_END = C_BIN + MCR_SHADE_CLASSES,
+  C_WORDS = C_BIN_END,
 };
 
 enum OverflowBits { OVERFLOW_SEEN = 1, OVERFLOW_WIDE = 2 };
@@ -165,6 +169,7 @@ struct Wavefront {
   uint32_t *counts;          // CountWord
   uint32_t *sref;            // shadow queue: min(lights, 16) regions of region_cap sref_entry values
   int32_t *free_list;        // slots with no sample
+  int32_t *bins;             // MCR_SHADE_CLASSES regions of capacity entries: the live queue by shade class
   double *deep_seen;         // MCR_DEEP_HITS counted distances per thread of the reference-order pass
   const Control *ctl;
 };

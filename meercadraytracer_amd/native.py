@@ -56,6 +56,7 @@ def lib():
             'mcr_render_tiles': (C.c_int, [P, P, P, P, P, C.c_uint32, P, C.c_int, P]),
             'mcr_render_batch': (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, P, C.c_int, P]),
             'mcr_set_graph': (C.c_int, [P, C.c_int]),
+            'mcr_set_shade_sort': (C.c_int, [P, C.c_int]),
             'mcr_set_intersection': (C.c_int, [P, C.c_int]),
             'mcr_server_create': (C.c_int, [P, P, C.POINTER(P)]),
             'mcr_server_destroy': (None, [P]),
@@ -162,6 +163,10 @@ class Context:
 
     def set_graph(self, on=True):
         check(lib().mcr_set_graph(self.handle, int(on)))
+
+    def set_shade_sort(self, on=True):
+        """Material-sorted shading (bin live paths by shade class before shade); same results."""
+        check(lib().mcr_set_shade_sort(self.handle, int(on)))
 
     def set_intersection(self, mode):
         """abi.MCR_ISECT_FP64 (default) or abi.MCR_ISECT_WIDE_FP32 for renders on this context."""
