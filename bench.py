@@ -504,6 +504,8 @@ def main():
             roofline['traffic_detail'] = t
     for b in (data, acc, image):
         b.free()
+    if args.only == 'plain':  # the plain workload alone (and the CPU baseline unless --no-cpu)
+        args.no_precision = args.no_textures = args.no_batched = args.no_served = args.no_c0 = True
     if not args.no_precision or args.only == 'precision':
         log('precision form (1M probe rays, FP64 binary vs FP32 wide) ...')
         result['precision'] = workloads.precision_form(ctx, gpu, sc.geometry, cam, W, H, settings=st, log=log)

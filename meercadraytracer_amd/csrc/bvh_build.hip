@@ -629,10 +629,10 @@ __global__ __launch_bounds__(256) void k_create(const int32_t *active, uint32_t 
 
 // ---- small subtrees ------------------------------------------------------------------------------
 // One wave builds the whole subtree of a node of at most SMALL primitives, with the decisions the
-// level-parallel kernels take: the 16 x 3 bins with lane 16 * axis + bin holding one bin and
-// reading every primitive of the node from LDS (all lanes read the same word, a broadcast),
-// decide_wave, and the closed-form two-cursor partition on the LDS copy. A child's bounds and
-// centroid bounds come from its parent's bins (exact: min and max), a half split's children take
+// level-parallel kernels take: the 16 x 3 bins filled in LDS by the node's primitives (one lane
+// each, LDS atomics), read back with lane 16 * axis + bin holding one bin, decide_wave, and the
+// closed-form two-cursor partition on the LDS copy. A child's bounds and centroid bounds come from
+// its parent's bins (exact: min and max), a half split's children take
 // wave reductions. The node's primitives sit in buffer depth % 2 (the level that created it
 // scattered them there and no later level touches them).
 //
@@ -664,6 +664,7 @@ __global__ __launch_bounds__(64) void k_small(const int32_t *list, const uint32_
   __shared__ int sfirst[SMALL_STACK], sn[SMALL_STACK], sdepth[SMALL_STACK], sglob[SMALL_STACK];
   __shared__ uint32_t skey[SMALL_STACK][12];  // bkey[6], ckey[6]; skey[.][0] == 0: to be reduced
   __shared__ int fr[SMALL], bl[SMALL];
+  __shared__ uint32_t sbin[3 * BINS * NB];  // the current node's bins
   const int lane = (int)threadIdx.x;
   const int32_t root = list[blockIdx.x];
   const int first = nodes[root].first, n = nodes[root].n, depth0 = nodes[root].depth;
@@ -709,6 +710,29 @@ __global__ __launch_bounds__(64) void k_small(const int32_t *list, const uint32_
     NodeGeom geo = node_geom(nckey);
     if (!leaf) {
       const int axis = lane >> 4, bin = lane & 15;
+      // The node's bins in LDS: cleared, then every primitive adds its count and keys to its bin
+      // on each axis with LDS atomics (min, max and sums do not depend on their order).
+      for (int i = lane; i < 3 * BINS * NB; i += 64) {
+        int f = i % NB;
+        sbin[i] = f == 0 ? 0u : ((f - 1) / 3 % 2 == 0 ? KEY_POS_INF : KEY_NEG_INF);
+      }
+      __syncthreads();
+      if (in) {
+        uint32_t v[9];
+        for (int j = 0; j < 9; j++) v[j] = fkey(pf[j][p]);
+        for (int a = 0; a < 3; a++) {
+          if (!geo.valid[a]) continue;
+          uint32_t *wb = sbin + (a * BINS + bin_of((double)pf[6 + a][p], geo.lo[a], geo.scale[a])) * NB;
+          atomicAdd(&wb[0], 1u);
+          for (int j = 0; j < 3; j++) {
+            atomicMin(&wb[1 + j], v[j]);
+            atomicMax(&wb[4 + j], v[3 + j]);
+            atomicMin(&wb[7 + j], v[6 + j]);
+            atomicMax(&wb[10 + j], v[6 + j]);
+          }
+        }
+      }
+      __syncthreads();
       int cnt = 0;
       uint32_t kb[6], kc[6];
       for (int j = 0; j < 3; j++) {
@@ -716,19 +740,13 @@ __global__ __launch_bounds__(64) void k_small(const int32_t *list, const uint32_
         kb[3 + j] = kc[3 + j] = KEY_NEG_INF;
       }
       if (axis < 3 && pick3(geo.valid, axis)) {
-        double lo = pick3(geo.lo, axis), scale = pick3(geo.scale, axis);
-        const float *cen = pf[6 + axis];
-#pragma unroll 4
-        for (int k = f0; k < f0 + nn; k++) {
-          if (bin_of((double)cen[k], lo, scale) != bin) continue;
-          cnt++;
-          for (int j = 0; j < 3; j++) {
-            kb[j] = min(kb[j], fkey(pf[j][k]));
-            kb[3 + j] = max(kb[3 + j], fkey(pf[3 + j][k]));
-            uint32_t c = fkey(pf[6 + j][k]);
-            kc[j] = min(kc[j], c);
-            kc[3 + j] = max(kc[3 + j], c);
-          }
+        const uint32_t *wb = sbin + (axis * BINS + bin) * NB;
+        cnt = (int)wb[0];
+        for (int j = 0; j < 3; j++) {
+          kb[j] = wb[1 + j];
+          kb[3 + j] = wb[4 + j];
+          kc[j] = wb[7 + j];
+          kc[3 + j] = wb[10 + j];
         }
       }
       decide_wave(nn, nbkey, nckey, lane, cnt, kb, kc, dec, bad);

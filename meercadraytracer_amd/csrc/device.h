@@ -183,10 +183,15 @@ MCR_INLINE double box_entry_t(float bx0, float by0, float bz0, float bx1, float 
 // for those an FP32 evaluation of the same slab distances, t' = fma(b, 1/d, -o/d), already settles
 // the outcome: with |o| and every box coordinate bounded by B (the root box bounds every node box),
 // |t' - t| <= 2^-22.99 (B + |o|) |1/d| per axis, where t is the FP64 value the reference computes
-// ((b - o) * inv rounded twice). The margin M = 2^-19 max over axes of (B + |o|) |1/d| is almost
-// sixteen times that bound. A box is called a hit or a miss in FP32 only when each of the test's
-// three comparisons (exit >= entry, exit >= tmin = 0, entry <= tmax) clears its threshold by the
-// margin; otherwise (a ray grazing a box face within ~M) the FP64 test decides. So every outcome,
+// ((b - o) * inv rounded twice). Entry and exit are maxima and minima of these, so each is within
+// E = 2^-22.99 max over axes of (B + |o|) |1/d| of its FP64 value, and their FP32 difference
+// within 2E plus its own rounding (2^-24 relative). The margin M = 2^-21 max over axes of
+// (B + |o|) |1/d| is 3.97 E, so a comparison that clears it is settled: exit - entry >= M in FP32
+// gives at least M (1 - 2^-24) - 2E > 0 in FP64, exit >= M gives at least M - E > 0, and the
+// thresholds against tmax carry a slack of at least M. A box is called a hit or a miss in FP32
+// only when each of the test's three comparisons (exit >= entry, exit >= tmin = 0,
+// entry <= tmax) clears its threshold by the margin; otherwise (a ray grazing a box face within
+// ~M) the FP64 test decides. So every outcome,
 // and every near/far order the closest-hit traversal takes, is the reference's, while the common
 // case costs a third of the FP64 test. The traversals use it for rays with tmin = 0 (every ray
 // the renderer traces) whose inverse direction is finite and moderate.
@@ -216,7 +221,7 @@ MCR_INLINE SlabF32 slab_setup(const float *root_box, double ox, double oy, doubl
   for (int k = 0; k < 6; k++) B = fmaxf(B, fabsf(root_box[k]));
   double ax = jm_fabs(ix), ay = jm_fabs(iy), az = jm_fabs(iz);
   double mx = ((double)B + jm_fabs(ox)) * ax, my = ((double)B + jm_fabs(oy)) * ay, mz = ((double)B + jm_fabs(oz)) * az;
-  double m = __builtin_fmax(__builtin_fmax(mx, my), mz) * 0x1p-19;
+  double m = __builtin_fmax(__builtin_fmax(mx, my), mz) * 0x1p-21;
   m = __builtin_fmax(m, 0x1p-100);
   r.ok = ax <= 0x1p64 && ay <= 0x1p64 && az <= 0x1p64 && m <= 0x1p80;
   r.ix = (float)ix;

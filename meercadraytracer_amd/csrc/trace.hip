@@ -1208,6 +1208,70 @@ MCR_INLINE uint32_t shadow_queue_entry(const Wavefront &w, const uint32_t *seg_e
   return (seg & 1) ? w.sref[region + w.region_cap - 1 - (e - start)] : w.sref[region + (e - start)];
 }
 
+// attenuateShadow (tracer.ts:544-594) over the see-through hits a shadow traversal kept (n of
+// them, this thread's records at T/M/UV[i * S]): the hits are put in the reference's visit order
+// and applied one by one; c is the light's colour factor (1, 1, 1 on entry).
+template <bool TEX>
+MCR_INLINE void shadow_replay(const DevScene &sc, double *T, int2 *M, double2 *UV, size_t S, int n, double c[3]) {
+  for (int i = 1; i < n; i++) {  // insertion sort, few hits
+    double ti = T[i * S];
+    int2 mi = M[i * S];
+    double2 ui = TEX ? UV[i * S] : make_double2(0, 0);
+    int j = i - 1;
+    while (j >= 0 && visits_before(mi, M[j * S])) {
+      T[(j + 1) * S] = T[j * S];
+      M[(j + 1) * S] = M[j * S];
+      if (TEX) UV[(j + 1) * S] = UV[j * S];
+      j--;
+    }
+    T[(j + 1) * S] = ti;
+    M[(j + 1) * S] = mi;
+    if (TEX) UV[(j + 1) * S] = ui;
+  }
+  double depth[3] = {0, 0, 0};
+  double eps = sc.epsilon;
+  for (int i = 0; i < n; i++) {
+    double t = T[i * S];
+    int2 mi = M[i * S];
+    int k = mi.y & 0x7fffffff;
+    bool front = (mi.y & 0x80000000) != 0;
+    bool dup = false;
+    for (int j = 0; j < i; j++)  // tracer.ts:572: hits already counted (marked by leaf -1)
+      if (M[j * S].x >= 0 && jm_fabs(T[j * S] - t) <= eps) dup = true;
+    if (dup) {
+      M[i * S] = make_int2(-1, mi.y);
+      continue;
+    }
+    const DevMaterial &m = sc.materials[__float_as_uint(sc.tri[(size_t)k * 3 + 2].z)];
+    double alpha = m.opacity, mcr = m.color[0], mcg = m.color[1], mcb = m.color[2];
+    if (TEX && (m.map >= 0 || m.alpha_map >= 0)) {
+      double2 hu = UV[i * S];
+      shadow_texel(sc, m, k, hu.x, hu.y, alpha, mcr, mcg, mcb);
+      if (m.alpha_test > 0 && alpha < m.alpha_test) {  // cut out: neither counted nor seen
+        M[i * S] = make_int2(-1, mi.y);
+        continue;
+      }
+    }
+    if (!m.transparent) alpha = 1;
+    if (m.thickness > 0) {
+      double sgn = front ? -t : t;
+      depth[0] = depth[0] + m.attenuation[0] * sgn;
+      depth[1] = depth[1] + m.attenuation[1] * sgn;
+      depth[2] = depth[2] + m.attenuation[2] * sgn;
+    }
+    double open = 1 - alpha, through = m.transmission;
+    c[0] = c[0] * (open + alpha * through * mcr);
+    c[1] = c[1] * (open + alpha * through * mcg);
+    c[2] = c[2] * (open + alpha * through * mcb);
+    if (c[0] + c[1] + c[2] < 1e-4) {
+      c[0] = c[1] = c[2] = 0;
+      break;
+    }
+  }
+  for (int k = 0; k < 3; k++)
+    if (depth[k] > 0) c[k] = c[k] * jm_exp(-depth[k]);
+}
+
 template <bool MIX, bool WIDE, bool TEX, class Stack>
 MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &w, Stack &st,
                             double *seen, int2 *seen_meta, double2 *seen_uv, DevCounters *cnt) {
@@ -1263,67 +1327,10 @@ MCR_INLINE void shadow_loop(const ViewDev *__restrict__ views, const Wavefront &
     // More see-through hits than the list keeps: k_shadow_deep traces the ray again in the
     // reference's order. An opaque hit settles the ray whatever else it crosses.
     deep = !blocked && col.overflow;
-    double depth[3] = {0, 0, 0};
     if (blocked) {
       c[0] = c[1] = c[2] = 0;
     } else if (col.n > 0 && !deep) {
-      // Replay the see-through hits in the reference's visit order (insertion sort, few hits).
-      for (int i = 1; i < col.n; i++) {
-        double ti = T[i * S];
-        int2 mi = M[i * S];
-        double2 ui = TEX ? UV[i * S] : make_double2(0, 0);
-        int j = i - 1;
-        while (j >= 0 && visits_before(mi, M[j * S])) {
-          T[(j + 1) * S] = T[j * S];
-          M[(j + 1) * S] = M[j * S];
-          if (TEX) UV[(j + 1) * S] = UV[j * S];
-          j--;
-        }
-        T[(j + 1) * S] = ti;
-        M[(j + 1) * S] = mi;
-        if (TEX) UV[(j + 1) * S] = ui;
-      }
-      double eps = sc.epsilon;
-      for (int i = 0; i < col.n; i++) {  // attenuateShadow for each hit, in order
-        double t = T[i * S];
-        int2 mi = M[i * S];
-        int k = mi.y & 0x7fffffff;
-        bool front = (mi.y & 0x80000000) != 0;
-        bool dup = false;
-        for (int j = 0; j < i; j++)  // tracer.ts:572: hits already counted (marked by leaf -1)
-          if (M[j * S].x >= 0 && jm_fabs(T[j * S] - t) <= eps) dup = true;
-        if (dup) {
-          M[i * S] = make_int2(-1, mi.y);
-          continue;
-        }
-        const DevMaterial &m = sc.materials[__float_as_uint(sc.tri[(size_t)k * 3 + 2].z)];
-        double alpha = m.opacity, mcr = m.color[0], mcg = m.color[1], mcb = m.color[2];
-        if (TEX && (m.map >= 0 || m.alpha_map >= 0)) {
-          double2 hu = UV[i * S];
-          shadow_texel(sc, m, k, hu.x, hu.y, alpha, mcr, mcg, mcb);
-          if (m.alpha_test > 0 && alpha < m.alpha_test) {  // cut out: neither counted nor seen
-            M[i * S] = make_int2(-1, mi.y);
-            continue;
-          }
-        }
-        if (!m.transparent) alpha = 1;
-        if (m.thickness > 0) {
-          double sgn = front ? -t : t;
-          depth[0] = depth[0] + m.attenuation[0] * sgn;
-          depth[1] = depth[1] + m.attenuation[1] * sgn;
-          depth[2] = depth[2] + m.attenuation[2] * sgn;
-        }
-        double open = 1 - alpha, through = m.transmission;
-        c[0] = c[0] * (open + alpha * through * mcr);
-        c[1] = c[1] * (open + alpha * through * mcg);
-        c[2] = c[2] * (open + alpha * through * mcb);
-        if (c[0] + c[1] + c[2] < 1e-4) {
-          c[0] = c[1] = c[2] = 0;
-          break;
-        }
-      }
-      for (int k = 0; k < 3; k++)
-        if (depth[k] > 0) c[k] = c[k] * jm_exp(-depth[k]);
+      shadow_replay<TEX>(sc, T, M, UV, S, col.n, c);
     }
     });
     queries++;
@@ -1449,7 +1456,6 @@ __global__ __launch_bounds__(256, 5) void k_shadow(const ViewDev *__restrict__ v
                                                 int32_t *spill, uint32_t spill_stride,
                                                 double *seen, int2 *seen_meta, double2 *seen_uv,
                                                 DevCounters *cnt) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   MCR_LANE_STACK(st, spill, spill_stride);
   shadow_loop<MIX, false, TEX>(views, w, st, seen, seen_meta, seen_uv, cnt);
 }

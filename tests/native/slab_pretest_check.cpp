@@ -23,7 +23,7 @@ static SlabF32 slab_setup(const float *rb, double ox, double oy, double oz, doub
   SlabF32 r; float B = 0; for (int k = 0; k < 6; k++) B = std::fmax(B, std::fabs(rb[k]));
   double ax = std::fabs(ix), ay = std::fabs(iy), az = std::fabs(iz);
   double mx = ((double)B + std::fabs(ox)) * ax, my = ((double)B + std::fabs(oy)) * ay, mz = ((double)B + std::fabs(oz)) * az;
-  double m = std::fmax(std::fmax(mx, my), mz) * 0x1p-19; m = std::fmax(m, 0x1p-100);
+  double m = std::fmax(std::fmax(mx, my), mz) * 0x1p-21; m = std::fmax(m, 0x1p-100);
   r.ok = ax <= 0x1p64 && ay <= 0x1p64 && az <= 0x1p64 && m <= 0x1p80;
   r.ix = (float)ix; r.iy = (float)iy; r.iz = (float)iz;
   r.oix = (float)(ox * ix); r.oiy = (float)(oy * iy); r.oiz = (float)(oz * iz);
@@ -84,6 +84,10 @@ int main(int argc, char **argv) {
         double p[3] = {ox + dx * t, oy + dy * t, oz + dz * t};
         bx[c][ax + 3 * side] = (float)p[ax];
         if (g() % 2) { int ax2 = (ax + 1) % 3; bx[c][ax2 + 3 * (g() % 2)] = (float)p[ax2]; }
+        if (g() % 2) {  // then move that face by a few times the FP32 error bound, either way
+          double org[3] = {ox, oy, oz};
+          bx[c][ax + 3 * side] = (float)(p[ax] + U(g) * 8 * 0x1p-23 * (scale + std::fabs(org[ax])));
+        }
         for (int k = 0; k < 3; k++) if (bx[c][k] > bx[c][k + 3]) std::swap(bx[c][k], bx[c][k + 3]);
       }
       if (g() % 8 == 0) { int ax = g() % 3; bx[c][ax + 3] = bx[c][ax]; }  // flat box
