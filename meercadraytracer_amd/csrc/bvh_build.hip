@@ -65,8 +65,10 @@ struct Decision {
 
 // Counter words in device memory.
 // C_SMALL_ALLOC: records set aside for the small subtrees; C_SMALL_USED: those they used.
-enum { C_NODES = 0, C_NEXT = 1, C_LEAVES = 2, C_BOUNDS = 3, C_ERROR = 4, C_CHUNKS = 5, C_SMALL = 6,
-       C_SMALL_ALLOC = 7, C_SMALL_USED = 8, C_COUNT = 12 };
+// C_NEXT_CHUNKS: chunks of the nodes the next level bins (so the host learns the next level's
+// sizes from the one read per level).
+enum { C_NODES = 0, C_NEXT = 1, C_LEAVES = 2, C_BOUNDS = 3, C_NEXT_CHUNKS = 4, C_ERROR = 5, C_CHUNKS = 6,
+       C_SMALL = 7, C_SMALL_ALLOC = 8, C_SMALL_USED = 9, C_COUNT = 12 };
 
 __device__ __host__ inline uint32_t fkey(float f) {
   uint32_t b = __builtin_bit_cast(uint32_t, f);
@@ -612,6 +614,11 @@ __global__ __launch_bounds__(256) void k_create(const int32_t *active, uint32_t 
   uint32_t nlv = split ? (uint32_t)leaf[0] + (uint32_t)leaf[1] : 0u;
   uint32_t nsm = split ? (uint32_t)small[0] + (uint32_t)small[1] : 0u;
   uint32_t nnx = split ? 2u - nlv - nsm : 0u;
+  uint32_t nch = 0;
+  if (split)
+    for (int side = 0; side < 2; side++)
+      if (!leaf[side] && !small[side]) nch += (uint32_t)((nodes[base + side].n + CHUNK - 1) / CHUNK);
+  wave_reserve(&counters[C_NEXT_CHUNKS], nch);
   uint32_t bpos = wave_reserve(&counters[C_BOUNDS], nb);
   uint32_t lpos = wave_reserve(&counters[C_LEAVES], nlv);
   uint32_t spos = wave_reserve(&counters[C_SMALL], nsm);
@@ -919,6 +926,24 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint32_t *tile_sums, uint32_
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
+// The whole scan in one block for n <= 16 * 1024: each thread sums its run of up to 16 inputs,
+// the block scans the run sums, and each thread writes its run's prefixes.
+const uint32_t SCAN_ONE_MAX = 16 * 1024;
+__global__ __launch_bounds__(1024) void k_scan_one(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *total) {
+  const uint32_t per = (n + 1023) / 1024, start = threadIdx.x * per;
+  uint32_t run = 0;
+  for (uint32_t k = 0; k < per; k++)
+    if (start + k < n) run += in[start + k];
+  uint32_t tot, acc = tile_scan_1024(run, tot);
+  for (uint32_t k = 0; k < per; k++)
+    if (start + k < n) {
+      uint32_t v = in[start + k];
+      out[start + k] = acc;
+      acc += v;
+    }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
 __global__ void k_scan_add(uint32_t *out, uint32_t n, const uint32_t *tile_sums) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] += tile_sums[i / 1024];
@@ -1140,6 +1165,10 @@ inline uint32_t nblocks(uint64_t n, uint32_t b = 256) { return (uint32_t)((n + b
 // Exclusive scan of in[0..n) into out[0..n); the sum goes to *total (device) when given.
 void launch_scan(hipStream_t sm, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *total,
                  uint32_t *tile_sums) {
+  if (n <= SCAN_ONE_MAX) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, sm, in, n, out, total);
+    return;
+  }
   uint32_t tiles = nblocks(n, 1024);
   hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(1024), 0, sm, in, n, out, tile_sums);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, sm, tile_sums, tiles, total);
@@ -1240,7 +1269,7 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
   hipLaunchKernelGGL(k_node_bounds_wide, dim3(std::min<uint32_t>(1024, nblocks(n))), dim3(256), 0, sm, L.nodes, 0,
                      L.P[0], 0u, n);
   std::vector<uint32_t> level_start = {0, 1};
-  uint32_t nact = 0, nsmall = 0;
+  uint32_t nact = 0, nsmall = 0, nchunks = 0;  // nchunks: chunks of the level's active nodes
   int cur = 0;
   // Host words copied to the device (the copies complete before the synchronisations below).
   static const int32_t zero = 0;
@@ -1254,6 +1283,7 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
   } else {
     BTRY(hipMemcpyAsync(L.active[0], &zero, sizeof(int32_t), hipMemcpyHostToDevice, sm));
     nact = 1;
+    nchunks = (n + CHUNK - 1) / CHUNK;
   }
   int act = 0;
   while (nact > 0) {
@@ -1265,10 +1295,7 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
     // chunks of the level's nodes
     hipLaunchKernelGGL(k_chunk_count, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_cnt);
     launch_scan(sm, L.chunk_cnt, nact, L.chunk_off_node, L.counters + C_CHUNKS, L.tile_sums);
-    BTRY(hipMemsetAsync(L.counters + C_NEXT, 0, sizeof(uint32_t) * 3, sm));
-    BTRY(hipMemcpyAsync(ws.host, L.counters, sizeof(uint32_t) * C_COUNT, hipMemcpyDeviceToHost, sm));
-    BTRY(hipStreamSynchronize(sm));
-    uint32_t nchunks = ws.host[C_CHUNKS];
+    BTRY(hipMemsetAsync(L.counters + C_NEXT, 0, sizeof(uint32_t) * 4, sm));  // C_NEXT .. C_NEXT_CHUNKS
     if (nchunks > L.max_chunks) {
       err = "builder: chunk list overflow";
       return MCR_E_OVERFLOW;
@@ -1299,6 +1326,7 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
     BTRY(hipMemcpyAsync(ws.host, L.counters, sizeof(uint32_t) * C_COUNT, hipMemcpyDeviceToHost, sm));
     BTRY(hipStreamSynchronize(sm));
     uint32_t nbounds = ws.host[C_BOUNDS], nleaves = ws.host[C_LEAVES], nnext = ws.host[C_NEXT];
+    nchunks = ws.host[C_NEXT_CHUNKS];
     nsmall = ws.host[C_SMALL];
     if (ws.host[C_ERROR]) {
       err = "builder: internal consistency check failed";
