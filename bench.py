@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = 'Mrays/s @1M tris 1080p 64spp vs CPU; p50/p99 time-to-first-preview in mix'
+BATCHED_BEST_SLOTS = 1 << 23  # the batched form's autotuner pick on its mix (2^20 .. 2^23 tried)
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES = 64, 48
 
@@ -294,31 +295,38 @@ def served(args, ctx, workloads):
     scenes = workloads.resident_scenes(ctx, 2, 32, 50_000, 4_000_000, log)
     try:
         tuned, trials = None, []
-        slots = args.served_slots
+        slots, cus = args.served_slots, 0
         if not slots:
             tuned, trials = workloads.autotune_served(ctx, scenes, log=log)
-            slots = tuned['slots']
+            slots, cus = tuned['slots'], tuned['preview_cus']
         sweep = []
         for rate in [float(r) for r in args.served_rates.split(',') if r]:
-            log(f'served mix at {rate}/s for {args.served_seconds} s, {slots} slots ...')
-            r = workloads.served_mix(ctx, rate, args.served_seconds, scenes=scenes, slots=slots,
+            log(f'served mix at {rate}/s for {args.served_seconds} s, {slots} pass slots ...')
+            r = workloads.served_mix(ctx, rate, args.served_seconds, scenes=scenes, slots=slots, preview_cus=cus,
                                      drain_seconds=10.0, log=log)
             specs = r.pop('completed_specs')
             log(json.dumps(r))
             sweep.append(r)
-        # The requests the last rate completed, as one batch on the same slots.
+        # The requests the last rate completed, as one batch: on the pass lane's slots, and on the
+        # slot count the batched form's own autotuner picks (its best).
         same = workloads.batched_equivalent(ctx, scenes, specs, slots=slots, log=log)
+        best = same if slots == BATCHED_BEST_SLOTS else workloads.batched_equivalent(
+            ctx, scenes, specs, slots=BATCHED_BEST_SLOTS, log=log)
     finally:
         for gpu, _, _ in scenes:
             gpu.release()
     top = sweep[-1]
     return {'workload': '32 resident plain_gpu scenes (50k-4M triangles, log-uniform), Poisson arrivals, '
                         'resolutions {320x240, 640x480, 1280x720, 1920x1080}, samples log-uniform [4, 256], '
-                        '30% camera-move restarts, 64-px tiles, 4x4 preview, HBM budget 200 GB',
+                        '30% camera-move restarts, 64-px tiles, 4x4 preview, HBM budget 200 GB; previews on a '
+                        'preview lane (2^18 slots, preview_cus CUs), passes on a pass lane (slots) on the rest',
             'triangles': int(sum(n for _, n, _ in scenes)),
             'ttfp_ms_p50': top['ttfp_ms_p50'], 'ttfp_ms_p99': top['ttfp_ms_p99'],
-            'saturated_mrays_s': top['mrays_s'], 'slots': slots, 'autotune': trials, 'batched_same_scenes': same,
-            'vs_batched_same_scenes': round(top['mrays_s'] / same['mrays_s'], 3), 'sweep': sweep}
+            'saturated_mrays_s': top['mrays_s'], 'mrays_s_during_arrivals': top['mrays_s_arrivals'],
+            'slots': slots, 'preview_cus': cus, 'autotune': trials,
+            'batched_same_scenes': same, 'vs_batched_same_scenes': round(top['mrays_s'] / same['mrays_s'], 3),
+            'batched_best_slots': best, 'vs_batched_best_slots': round(top['mrays_s'] / best['mrays_s'], 3),
+            'sweep': sweep}
 
 
 def main():

@@ -274,12 +274,18 @@ int mcr_render_batch(mcr_ctx *ctx, const mcr_view *views, uint32_t view_count,
 typedef struct mcr_server mcr_server;
 typedef struct mcr_server_config {
   uint64_t hbm_budget;      /* bytes; 0 = 200 GB */
-  uint32_t slots;           /* wavefront path slots; 0 = 2^21 */
+  uint32_t slots;           /* path slots of the lane that takes the passes; 0 = 2^21 with one
+                               lane, 2^22 beside a preview lane */
   uint32_t window;          /* unstarted samples kept queued ahead of the slots; 0 = slots */
   uint32_t max_requests;    /* admitted requests at once; 0 = 64 */
   int32_t use_graph;        /* replay captured bounce iterations (1) or launch directly (0) */
   uint32_t max_lights;      /* light slots per path: the most lights (plus 1 with environments) a
                                scene added to the server may have; 0 = 16, at most MCR_MAX_LIGHTS_PER_SCENE */
+  uint32_t lanes;           /* 1: one wavefront takes every work item; 2 (0 = default): a preview lane
+                               takes the previews on preview_cus CUs kept for it, beside a pass lane
+                               of `slots` slots on the other CUs, each on its own stream */
+  uint32_t preview_slots;   /* the preview lane's path slots; 0 = 2^18 */
+  uint32_t preview_cus;     /* CUs kept for the preview lane; 0 = 3/32 of the device's */
 } mcr_server_config;
 typedef struct mcr_request {
   mcr_scene *scene;         /* added with mcr_server_add_scene */

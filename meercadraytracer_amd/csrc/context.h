@@ -51,6 +51,8 @@ const uint64_t LIGHT_SLOT_BYTES = 109;
 const uint64_t LIGHT_SLOT_BUDGET = 24ull << 30;
 // A server holds all its slots from the start and its latency grows with them: 2^21 by default.
 const uint32_t SERVER_DEFAULT_SLOTS = 1u << 21;
+const uint32_t SERVER_DEFAULT_PREVIEW_SLOTS = 1u << 18;
+const uint32_t SERVER_DEFAULT_PASS_SLOTS = 1u << 22;
 
 // One wavefront engine: the path slots and everything its kernels read and write. A context owns
 // one for its synchronous calls; a server owns another for its request loop.

@@ -27,6 +27,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <set>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -59,13 +60,14 @@ struct SrvRequest {
   float *sum = nullptr;
   uint64_t bytes = 0;
   std::vector<SrvTile> tiles;
-  std::deque<Item> pending;         // work items not in the job window, in work-list order
+  // Work items not in a job window, in work-list order: previews, and the pass-major items.
+  std::deque<Item> pending_preview, pending_pass;
   uint32_t jobs_live = 0;           // jobs in the window or waiting to be encoded
   uint32_t covered = 0, finished_tiles = 0;
   bool cancel = false, release = false;
   double t_submit = -1, t_admit = -1, t_preview = -1, t_done = -1;
   uint64_t paths = 0;
-  uint64_t it_admit = 0, it_preview = 0;  // bounce iterations run by the server so far
+  uint64_t it_admit = 0, it_preview = 0;  // bounce iterations of the preview lane so far
 };
 
 struct SrvJob {
@@ -78,40 +80,87 @@ struct SrvJob {
   uint32_t done = 0;
 };
 
-// First-fit allocator over the result arena (offsets and sizes in floats, multiples of 4).
+// Best-fit allocator over the result arena (offsets and sizes in floats, multiples of 4): free
+// ranges indexed by offset (to merge neighbours on release) and by (length, offset) (to find the
+// smallest range that fits), so both operations cost O(log ranges) however fragmented it gets.
 struct Arena {
-  std::map<uint64_t, uint64_t> free_;  // offset -> length
-  void reset(uint64_t n) {
-    free_.clear();
-    free_[0] = n;
+  std::map<uint64_t, uint64_t> by_off;        // offset -> length
+  std::set<std::pair<uint64_t, uint64_t>> by_len;  // (length, offset)
+  void reset(uint64_t base, uint64_t n) {
+    by_off.clear();
+    by_len.clear();
+    put(base, n);
+  }
+  void put(uint64_t off, uint64_t len) {
+    by_off[off] = len;
+    by_len.emplace(len, off);
+  }
+  void take(std::map<uint64_t, uint64_t>::iterator it) {
+    by_len.erase({it->second, it->first});
+    by_off.erase(it);
   }
   uint64_t alloc(uint64_t n) {
     n = (n + 3) & ~3ull;
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < n) continue;
-      uint64_t off = it->first, len = it->second;
-      free_.erase(it);
-      if (len > n) free_[off + n] = len - n;
-      return off;
-    }
-    return UINT64_MAX;
+    auto it = by_len.lower_bound({n, 0});
+    if (it == by_len.end()) return UINT64_MAX;
+    uint64_t len = it->first, off = it->second;
+    by_len.erase(it);
+    by_off.erase(off);
+    if (len > n) put(off + n, len - n);
+    return off;
   }
   void release(uint64_t off, uint64_t n) {
     n = (n + 3) & ~3ull;
-    auto it = free_.emplace(off, n).first;
-    auto next = std::next(it);
-    if (next != free_.end() && it->first + it->second == next->first) {
-      it->second += next->second;
-      free_.erase(next);
+    auto next = by_off.lower_bound(off);
+    if (next != by_off.end() && off + n == next->first) {
+      n += next->second;
+      take(next);
     }
-    if (it != free_.begin()) {
-      auto prev = std::prev(it);
-      if (prev->first + prev->second == it->first) {
-        prev->second += it->second;
-        free_.erase(it);
+    auto after = by_off.lower_bound(off);
+    if (after != by_off.begin()) {
+      auto prev = std::prev(after);
+      if (prev->first + prev->second == off) {
+        off = prev->first;
+        n += prev->second;
+        take(prev);
       }
     }
+    put(off, n);
   }
+};
+
+// One wavefront with its own stream and job window. A server runs one lane that takes every work
+// item, or a preview lane (the previews, on a few CUs kept for it) beside a pass lane (the passes,
+// on the other CUs): a preview then waits for neither the pass lane's long bounce iterations nor
+// its slots, and the pass lane can run as many slots as throughput wants.
+struct Lane {
+  Engine eng;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;       // recorded after a launched step's count copies
+  bool previews = true, passes = true;  // the items the lane takes
+  uint32_t window = 0;             // unstarted samples queued ahead of the slots
+  std::vector<SrvJob> table;       // the lane's device job list, by sample_start
+  uint32_t total = 0;              // samples [0, total) may be started
+  int q = 0;                       // queue holding the live paths
+  bool inflight = false, views_dirty = true;
+  uint32_t launched = 0;           // jobs of the step in flight
+  uint64_t iterations = 0;
+  // Bounce iterations per step: a captured pair, or one direct launch; the pass lane of two lanes
+  // runs two of those per step (its steps are long; fewer host round trips between them).
+  uint32_t launches_per_step = 1;
+  uint32_t iterations_per_step() const { return launches_per_step * (eng.use_graph ? 2u : 1u); }
+  JobDev *jobs_host = nullptr;     // pinned staging of the job list
+  uint32_t *done_host = nullptr;   // pinned: finished samples per job
+  uint32_t host_cap = 0;
+  void *counters_host = nullptr;   // pinned: the device counter copies
+  DevCounters counters = {};
+  // The lane's tile results wait in its own range of the server's arena until encoded, and the
+  // lane encodes them on its own stream: a range freed after an encode is reused only by the
+  // lane's later steps, which follow the encode on that stream.
+  Arena arena;
+  EncodeItem *enc_host = nullptr, *enc_dev = nullptr;  // staging of the encode items
+  hipEvent_t encoded = nullptr;    // recorded after every encode launch of the lane
+  bool enc_more = false;           // tile results of the lane wait for their encodes
 };
 
 // makeTiles (progressive.ts:160-170): row-major tiles, stable-sorted by the distance of the tile
@@ -138,35 +187,25 @@ std::vector<mcr_tile> make_tiles(uint32_t width, uint32_t height, uint32_t size)
 
 struct mcr_server {
   mcr_ctx *ctx = nullptr;
-  hipStream_t stream = nullptr, copy_stream = nullptr;
-  hipEvent_t encoded = nullptr;    // recorded after every encode launch
-  Engine eng;
+  hipStream_t stream = nullptr, copy_stream = nullptr;  // stream: admissions and frees
+  hipEvent_t admitted = nullptr;   // recorded on `stream` after each admission's clears
+  hipEvent_t read_ready = nullptr; // recorded on `stream` by mcr_server_read
+  std::vector<std::unique_ptr<Lane>> lanes;  // lanes[0] takes the previews
   mcr_server_config cfg = {};
   float *arena_dev = nullptr;
   uint64_t arena_floats = 0;
-  Arena arena;
   std::vector<ViewDev> views;
   std::vector<int> view_free;
-  bool views_dirty = true;
   std::unordered_map<uint64_t, std::unique_ptr<SrvRequest>> reqs;
   std::deque<uint64_t> admit_queue;
   std::vector<uint64_t> running;   // admitted, in admission order
-  std::vector<SrvJob> table;       // the device job list, by sample_start
-  uint32_t total = 0;              // samples [0, total) may be started
-  int q = 0;                       // queue holding the live paths
   uint64_t bytes_used = 0;
   std::vector<mcr_scene *> scenes;  // held (scene_hold) until removed or the server is destroyed
   std::vector<uint64_t> scene_bytes;  // what each counts against the HBM budget
-  EncodeItem *enc_host = nullptr, *enc_dev = nullptr;
-  uint32_t enc_cap = 0;
-  JobDev *jobs_host = nullptr;     // pinned staging of the job list
-  uint32_t *done_host = nullptr;   // pinned
-  uint32_t host_cap = 0;
-  void *counters_host = nullptr;  // pinned: the device counter copies
+  uint32_t enc_cap = 0;             // encode items staged per launch
   // stats
-  uint64_t iterations = 0, paths = 0;
-  double device_ms = 0;
-  DevCounters counters = {};
+  uint64_t paths = 0;
+  double device_ms = 0;            // time with at least one lane's step in flight
   // thread
   std::thread thread;
   std::mutex mu;
@@ -178,9 +217,27 @@ struct mcr_server {
   std::string fatal_msg;
 
   double now() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+  uint64_t preview_iterations() const { return lanes[0]->iterations; }
+  DevCounters counters() const {
+    DevCounters c = {};
+    for (const auto &l : lanes) {
+      const unsigned long long *a = reinterpret_cast<const unsigned long long *>(&l->counters);
+      unsigned long long *o = reinterpret_cast<unsigned long long *>(&c);
+      for (size_t k = 0; k < sizeof(DevCounters) / sizeof(unsigned long long); k++) o[k] += a[k];
+    }
+    return c;
+  }
 };
 
 namespace {
+
+// The lane whose arena holds preview (or pass) results.
+Lane &result_lane(mcr_server *s, bool preview) { return *s->lanes[preview || s->lanes.size() == 1 ? 0 : 1]; }
+
+// Orders `stream` after every encode launched so far (request buffers are written by encodes).
+void after_encodes(mcr_server *s, hipStream_t stream) {
+  for (auto &l : s->lanes) (void)hipStreamWaitEvent(stream, l->encoded, 0);
+}
 
 // Renders beyond this many samples keep their running sum in FP64.
 const uint32_t WIDE_SUM_SAMPLES = 1024;
@@ -190,18 +247,21 @@ uint64_t sum_bytes(const mcr_request &r) {
 }
 uint64_t request_bytes(const mcr_request &r) { return (uint64_t)r.width * r.height * 4 + sum_bytes(r); }
 
-// Frees a terminal request's device memory and slot once nothing of it is in flight.
+// Frees a terminal request's device memory and slot once nothing of it is in flight. Request
+// buffers are written only by encodes (on the lanes' streams), so the frees on s->stream are
+// ordered after every encode launched so far.
 void retire(mcr_server *s, SrvRequest &r) {
   if (r.jobs_live || r.view < 0) return;
   if (r.state != MCR_REQ_DONE && r.state != MCR_REQ_CANCELLED && r.state != MCR_REQ_FAILED) return;
+  after_encodes(s, s->stream);
   if (r.state == MCR_REQ_CANCELLED || r.state == MCR_REQ_FAILED) {
-    hipFree(r.image);
-    hipFree(r.sum);
+    hipFreeAsync(r.image, s->stream);
+    hipFreeAsync(r.sum, s->stream);
     r.image = nullptr;
     r.sum = nullptr;
     s->bytes_used -= r.bytes;
   } else {
-    hipFree(r.sum);  // the image stays readable until released
+    hipFreeAsync(r.sum, s->stream);  // the image stays readable until released
     r.sum = nullptr;
     s->bytes_used -= sum_bytes(r.r);
     r.bytes = (uint64_t)r.r.width * r.r.height * 4;
@@ -213,58 +273,52 @@ void retire(mcr_server *s, SrvRequest &r) {
 
 int admit(mcr_server *s, SrvRequest &r) {
   uint64_t npx = (uint64_t)r.r.width * r.r.height;
-  HIP_TRY(hipMalloc(&r.image, npx * 4));
+  HIP_TRY(hipMallocAsync((void **)&r.image, npx * 4, s->stream));
   uint64_t sb = sum_bytes(r.r);
-  hipError_t e = hipMalloc(&r.sum, sb);
+  hipError_t e = hipMallocAsync((void **)&r.sum, sb, s->stream);
   if (e != hipSuccess) {
-    hipFree(r.image);
+    hipFreeAsync(r.image, s->stream);
     r.image = nullptr;
-    return mcr_fail(MCR_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return mcr_fail(MCR_E_DEVICE, std::string("hipMallocAsync: ") + hipGetErrorString(e));
   }
   HIP_TRY(hipMemsetAsync(r.image, 0, npx * 4, s->stream));
   HIP_TRY(hipMemsetAsync(r.sum, 0, sb, s->stream));
+  HIP_TRY(hipEventRecord(s->admitted, s->stream));  // encodes into the buffers wait for this
   r.view = s->view_free.back();
   s->view_free.pop_back();
   s->views[r.view] = make_view(r.r.scene, r.r.camera, r.r.settings);
-  s->views_dirty = true;
+  for (auto &l : s->lanes) l->views_dirty = true;
   s->bytes_used += r.bytes;
   auto tiles = make_tiles(r.r.width, r.r.height, r.r.tile_size);
   r.tiles.resize(tiles.size());
   for (size_t i = 0; i < tiles.size(); i++) r.tiles[i].t = tiles[i];
   // progressive.ts:207-210: preview items for every tile, then pass-major items.
   if (r.r.preview_block > 1)
-    for (uint32_t i = 0; i < tiles.size(); i++) r.pending.push_back(Item{i, 0, r.r.preview_block});
+    for (uint32_t i = 0; i < tiles.size(); i++) r.pending_preview.push_back(Item{i, 0, r.r.preview_block});
   for (uint32_t p = 0; p < r.r.samples; p++)
-    for (uint32_t i = 0; i < tiles.size(); i++) r.pending.push_back(Item{i, p, 1});
+    for (uint32_t i = 0; i < tiles.size(); i++) r.pending_pass.push_back(Item{i, p, 1});
   r.state = MCR_REQ_RUNNING;
   r.t_admit = s->now();
-  r.it_admit = s->iterations;
+  r.it_admit = s->preview_iterations();
   s->running.push_back(r.id);
   return MCR_OK;
 }
 
 // Admissions, cancellations and releases (under the lock).
 int housekeeping(mcr_server *s) {
-  // Encodes of the last step may still read or write what is freed below.
-  bool frees = false;
-  for (auto &kv : s->reqs) {
-    const SrvRequest &r = *kv.second;
-    frees |= r.cancel || r.release ||
-             (r.jobs_live == 0 && r.view >= 0 && r.state != MCR_REQ_RUNNING && r.state != MCR_REQ_QUEUED);
-  }
-  if (frees) HIP_TRY(hipStreamSynchronize(s->stream));
   for (auto it = s->reqs.begin(); it != s->reqs.end();) {
     SrvRequest &r = *it->second;
     if (r.cancel && (r.state == MCR_REQ_QUEUED || r.state == MCR_REQ_RUNNING)) {
       if (r.state == MCR_REQ_QUEUED)
         s->admit_queue.erase(std::remove(s->admit_queue.begin(), s->admit_queue.end(), r.id), s->admit_queue.end());
       r.state = MCR_REQ_CANCELLED;
-      r.pending.clear();
+      r.pending_preview.clear();
+      r.pending_pass.clear();
       for (SrvTile &t : r.tiles) {
         uint64_t floats = (uint64_t)t.t.width * t.t.height * 4;
-        for (auto &kv : t.ready) s->arena.release(kv.second, floats);
+        for (auto &kv : t.ready) result_lane(s, false).arena.release(kv.second, floats);
         t.ready.clear();
-        if (t.preview != UINT64_MAX) s->arena.release(t.preview, floats);
+        if (t.preview != UINT64_MAX) result_lane(s, true).arena.release(t.preview, floats);
         t.preview = UINT64_MAX;
       }
       r.t_done = s->now();
@@ -272,8 +326,9 @@ int housekeeping(mcr_server *s) {
     }
     retire(s, r);
     if (r.release && r.view < 0 && r.jobs_live == 0) {
-      hipFree(r.image);
-      hipFree(r.sum);
+      after_encodes(s, s->stream);
+      if (r.image) hipFreeAsync(r.image, s->stream);
+      if (r.sum) hipFreeAsync(r.sum, s->stream);
       s->bytes_used -= r.image ? r.bytes : 0;
       it = s->reqs.erase(it);
       continue;
@@ -289,40 +344,55 @@ int housekeeping(mcr_server *s) {
   return MCR_OK;
 }
 
-// Step 2: the job window. `next` is the first sample not yet started; `idle` slots are free now
-// and take their samples from the window first, so the window holds that many samples more.
-void plan_window(mcr_server *s, uint32_t next, uint32_t idle) {
+// The job window of lane L. `next` is the lane's first sample not yet started; `idle` slots are
+// free now and take their samples from the window first, so the window holds that many samples
+// more. A lane that takes both kinds of items returns its unstarted jobs to their requests and
+// chooses the window afresh, so a new request's preview overtakes queued passes. A lane of one
+// kind keeps its unstarted jobs where they are (renumbered to follow the started ones) and only
+// drops those of requests that have ended, so a step's planning costs what it adds.
+void plan_window(mcr_server *s, Lane &L, uint32_t next, uint32_t idle) {
   std::vector<SrvJob> keep;
   std::unordered_map<uint64_t, std::vector<Item>> returned;
+  const bool afresh = L.previews && L.passes;
   uint32_t base = next;
-  for (SrvJob &j : s->table) {
+  for (SrvJob &j : L.table)  // started jobs come first (the table is in sample order)
+    if (j.done != j.size && j.dev.sample_start < next) base = std::max(base, j.dev.sample_start + j.size);
+  uint32_t cursor = base;
+  keep.reserve(L.table.size());
+  for (SrvJob &j : L.table) {
     if (j.done == j.size) continue;  // finished: handled by deliver()
     if (j.dev.sample_start < next) {
       keep.push_back(j);
-      base = std::max(base, j.dev.sample_start + j.size);
-    } else {
-      returned[j.req].push_back(j.item);
-      s->arena.release(j.arena, j.floats);
-      SrvRequest &r = *s->reqs[j.req];
-      r.jobs_live--;
+      continue;
     }
+    SrvRequest &r = *s->reqs[j.req];
+    if (!afresh && r.state == MCR_REQ_RUNNING) {
+      j.dev.sample_start = cursor;
+      cursor += j.size;
+      keep.push_back(j);
+      continue;
+    }
+    if (r.state == MCR_REQ_RUNNING) returned[j.req].push_back(j.item);
+    L.arena.release(j.arena, j.floats);
+    r.jobs_live--;
   }
   for (auto &kv : returned) {
     SrvRequest &r = *s->reqs[kv.first];
-    if (r.state != MCR_REQ_RUNNING) continue;
-    for (auto it = kv.second.rbegin(); it != kv.second.rend(); ++it) r.pending.push_front(*it);
+    for (auto it = kv.second.rbegin(); it != kv.second.rend(); ++it)
+      (it->block > 1 ? r.pending_preview : r.pending_pass).push_front(*it);
   }
-  s->table.swap(keep);
-  // A replayed pair runs two iterations before the next step, so it needs two windows.
-  uint64_t target = (uint64_t)s->cfg.window * (s->eng.use_graph ? 2 : 1) + idle, queued = 0;
+  L.table.swap(keep);
+  base = cursor;
+  // A step runs L.iterations_per_step bounce iterations, each of which may start a window's worth.
+  uint64_t target = (uint64_t)L.window * L.iterations_per_step() + idle, queued = cursor - next;
   uint32_t start = base;
-  auto place = [&](SrvRequest &r) -> bool {
-    Item it = r.pending.front();
+  auto place = [&](SrvRequest &r, std::deque<Item> &pending) -> bool {
+    Item it = pending.front();
     const mcr_tile &t = r.tiles[it.tile].t;
     // Sample indices stay below 2^31; past that the window drains and numbering restarts.
     if ((uint64_t)start + (uint64_t)t.width * t.height > (1ull << 31)) return false;
     uint64_t floats = (uint64_t)t.width * t.height * 4;
-    uint64_t off = s->arena.alloc(floats);
+    uint64_t off = L.arena.alloc(floats);
     if (off == UINT64_MAX) return false;
     mcr_tile_job tj{t, r.r.width, r.r.height, it.pass, it.block, off};
     SrvJob j;
@@ -333,8 +403,8 @@ void plan_window(mcr_server *s, uint32_t next, uint32_t idle) {
     j.floats = floats;
     j.size = (uint32_t)n;
     j.done = 0;
-    s->table.push_back(j);
-    r.pending.pop_front();
+    L.table.push_back(j);
+    pending.pop_front();
     r.jobs_live++;
     start += (uint32_t)n;
     queued += (uint64_t)n;
@@ -342,26 +412,30 @@ void plan_window(mcr_server *s, uint32_t next, uint32_t idle) {
   };
   bool room = true;
   // Previews of requests whose preview has not been queued completely go first.
-  for (uint64_t id : s->running) {
-    SrvRequest &r = *s->reqs[id];
-    while (room && r.state == MCR_REQ_RUNNING && !r.pending.empty() && r.pending.front().block > 1)
-      room = place(r);
-  }
+  if (L.previews)
+    for (uint64_t id : s->running) {
+      SrvRequest &r = *s->reqs[id];
+      while (room && r.state == MCR_REQ_RUNNING && !r.pending_preview.empty()) room = place(r, r.pending_preview);
+    }
   // Then the passes of the oldest request first: the wavefront keeps working on few scenes at a
   // time (their BVHs stay in cache), and requests finish in arrival order.
-  for (uint64_t id : s->running) {
-    SrvRequest &r = *s->reqs[id];
-    while (room && queued < target && r.state == MCR_REQ_RUNNING && !r.pending.empty()) room = place(r);
-    if (!room || queued >= target) break;
-  }
-  s->total = start;
+  if (L.passes)
+    for (uint64_t id : s->running) {
+      SrvRequest &r = *s->reqs[id];
+      // One lane: a request's passes follow its preview (progressive.ts:207-210).
+      if (L.previews && !r.pending_preview.empty()) continue;
+      while (room && queued < target && r.state == MCR_REQ_RUNNING && !r.pending_pass.empty())
+        room = place(r, r.pending_pass);
+      if (!room || queued >= target) break;
+    }
+  L.total = start;
 }
 
-// Step 4: finished jobs go to their requests; tiles are written in pass order. Returns whether
-// anything was written.
-int deliver(mcr_server *s, bool &progress) {
+// Finished jobs of lane L go to their requests; L's tile results are written in pass order by
+// encodes on L's stream (after L's step has been seen complete). Sets progress if anything moved.
+int deliver(mcr_server *s, Lane &L, bool &progress) {
   std::vector<SrvJob> still;
-  for (SrvJob &j : s->table) {
+  for (SrvJob &j : L.table) {
     if (j.done != j.size) {
       still.push_back(j);
       continue;
@@ -372,15 +446,20 @@ int deliver(mcr_server *s, bool &progress) {
     s->paths += j.size;
     r.jobs_live--;
     SrvTile *t = r.state == MCR_REQ_RUNNING ? &r.tiles[j.item.tile] : nullptr;
-    if (!t) s->arena.release(j.arena, j.floats);
+    if (!t) L.arena.release(j.arena, j.floats);
     else if (j.item.block > 1) t->preview = j.arena;
     else t->ready[j.item.pass] = j.arena;
+    L.enc_more |= t != nullptr;
   }
-  s->table.swap(still);
+  L.table.swap(still);
+  if (!L.enc_more) return MCR_OK;  // no tile result of the lane waits for its encode
+  L.enc_more = false;
   // Rounds of one launch each; a tile is staged at most once per round, so no two items of a
   // launch write the same pixel.
   uint32_t nenc = 0, launched = 0;
   std::vector<std::pair<uint64_t, uint64_t>> to_free;  // arena ranges, freed after the launches
+  // The staging of the lane's last encodes may still be in its copy to the device.
+  HIP_TRY(hipEventSynchronize(L.encoded));
   for (;;) {
     uint32_t first = nenc;
     for (uint64_t id : s->running) {
@@ -391,14 +470,14 @@ int deliver(mcr_server *s, bool &progress) {
         uint64_t floats = (uint64_t)t.t.width * t.t.height * 4;
         uint64_t data;
         bool preview = false;
-        if (t.preview != UINT64_MAX) {
+        if (L.previews && t.preview != UINT64_MAX) {
           // progressive.ts:257-258: a preview is written only over a tile with no samples yet.
           data = t.preview;
           t.preview = UINT64_MAX;
           to_free.emplace_back(data, floats);
           if (t.samples != 0 || t.ready.count(0)) continue;
           preview = true;
-        } else {
+        } else if (L.passes) {
           auto it = t.ready.find(t.samples);
           if (it == t.ready.end()) continue;
           data = it->second;
@@ -406,13 +485,15 @@ int deliver(mcr_server *s, bool &progress) {
           to_free.emplace_back(data, floats);
           t.samples++;
           if (t.samples == r.r.samples) r.finished_tiles++;
+        } else {
+          continue;
         }
         bool wide = r.r.samples > WIDE_SUM_SAMPLES;
-        s->enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
-                                         preview || !wide ? nullptr : reinterpret_cast<double *>(r.sum),
-                                         s->arena_dev + data, r.r.width,
-                                         t.t.x, t.t.y, t.t.width, t.t.height, preview ? 1u : t.samples,
-                                         r.r.srgb, 0};
+        L.enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
+                                        preview || !wide ? nullptr : reinterpret_cast<double *>(r.sum),
+                                        s->arena_dev + data, r.r.width,
+                                        t.t.x, t.t.y, t.t.width, t.t.height, preview ? 1u : t.samples,
+                                        r.r.srgb, 0};
         if (!t.covered) {
           t.covered = true;
           r.covered++;
@@ -422,20 +503,29 @@ int deliver(mcr_server *s, bool &progress) {
     if (nenc == first) break;
     uint32_t blocks = 0;
     for (uint32_t k = first; k < nenc; k++) {
-      s->enc_host[k].block_start = blocks;
-      blocks += (uint32_t)(((uint64_t)s->enc_host[k].tw * s->enc_host[k].th + 255) / 256);
+      L.enc_host[k].block_start = blocks;
+      blocks += (uint32_t)(((uint64_t)L.enc_host[k].tw * L.enc_host[k].th + 255) / 256);
     }
-    HIP_TRY(hipMemcpyAsync(s->enc_dev + first, s->enc_host + first, (nenc - first) * sizeof(EncodeItem),
-                           hipMemcpyHostToDevice, s->stream));
-    launch_encode_batch(s->stream, s->enc_dev + first, nenc - first, blocks);
+    if (first == 0) {
+      // After the buffers' clears at admission; a pass lane also after every preview encode
+      // launched so far (a preview is written only before a tile's first pass).
+      HIP_TRY(hipStreamWaitEvent(L.stream, s->admitted, 0));
+      if (!L.previews) HIP_TRY(hipStreamWaitEvent(L.stream, s->lanes[0]->encoded, 0));
+    }
+    HIP_TRY(hipMemcpyAsync(L.enc_dev + first, L.enc_host + first, (nenc - first) * sizeof(EncodeItem),
+                           hipMemcpyHostToDevice, L.stream));
+    launch_encode_batch(L.stream, L.enc_dev + first, nenc - first, blocks);
     HIP_TRY(hipGetLastError());
     launched = nenc;
     progress = true;
-    if (nenc == s->enc_cap) break;  // the rest waits for the next step
+    if (nenc == s->enc_cap) {  // the rest waits for the next call
+      L.enc_more = true;
+      break;
+    }
   }
-  if (launched) HIP_TRY(hipEventRecord(s->encoded, s->stream));
-  // Ranges are reused only by later kernels on this stream, after the encodes that read them.
-  for (auto &f : to_free) s->arena.release(f.first, f.second);
+  if (launched) HIP_TRY(hipEventRecord(L.encoded, L.stream));
+  // The lane's later steps follow these encodes on its stream, so the ranges can be reused now.
+  for (auto &f : to_free) L.arena.release(f.first, f.second);
   // Status: first complete preview, completion (stamped once the writes have finished).
   bool need_sync = false;
   for (uint64_t id : s->running) {
@@ -443,7 +533,8 @@ int deliver(mcr_server *s, bool &progress) {
     if (r.state != MCR_REQ_RUNNING) continue;
     if ((r.t_preview < 0 && r.covered == r.tiles.size()) || r.finished_tiles == r.tiles.size()) need_sync = true;
   }
-  if (need_sync) HIP_TRY(hipStreamSynchronize(s->stream));
+  if (need_sync)
+    for (auto &l : s->lanes) HIP_TRY(hipEventSynchronize(l->encoded));
   double now = s->now();
   std::vector<uint64_t> ids(s->running);
   for (uint64_t id : ids) {
@@ -451,10 +542,10 @@ int deliver(mcr_server *s, bool &progress) {
     if (r.state != MCR_REQ_RUNNING) continue;
     if (r.t_preview < 0 && r.covered == r.tiles.size()) {
       r.t_preview = now;
-      r.it_preview = s->iterations;
+      r.it_preview = s->preview_iterations();
       s->changed.notify_all();
     }
-    if (r.finished_tiles == r.tiles.size() && r.pending.empty()) {
+    if (r.finished_tiles == r.tiles.size() && r.pending_pass.empty() && r.pending_preview.empty()) {
       r.state = MCR_REQ_DONE;
       r.t_done = now;
       retire(s, r);
@@ -464,111 +555,118 @@ int deliver(mcr_server *s, bool &progress) {
   return MCR_OK;
 }
 
-int ensure_host(mcr_server *s, uint32_t n) {
-  if (s->host_cap >= n) return MCR_OK;
-  uint32_t want = std::max<uint32_t>(n, s->host_cap * 2);
-  if (s->jobs_host) hipHostFree(s->jobs_host);
-  if (s->done_host) hipHostFree(s->done_host);
-  s->jobs_host = nullptr;
-  s->done_host = nullptr;
-  s->host_cap = 0;
-  HIP_TRY(hipHostMalloc(&s->jobs_host, want * sizeof(JobDev), hipHostMallocDefault));
-  HIP_TRY(hipHostMalloc(&s->done_host, want * sizeof(uint32_t), hipHostMallocDefault));
-  s->host_cap = want;
+int ensure_host(Lane &L, uint32_t n) {
+  if (L.host_cap >= n) return MCR_OK;
+  uint32_t want = std::max<uint32_t>(n, L.host_cap * 2);
+  if (L.jobs_host) hipHostFree(L.jobs_host);
+  if (L.done_host) hipHostFree(L.done_host);
+  L.jobs_host = nullptr;
+  L.done_host = nullptr;
+  L.host_cap = 0;
+  HIP_TRY(hipHostMalloc(&L.jobs_host, want * sizeof(JobDev), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(&L.done_host, want * sizeof(uint32_t), hipHostMallocDefault));
+  L.host_cap = want;
   return MCR_OK;
 }
 
-// One host step with the lock held across planning and delivery but not across device waits.
-int step(mcr_server *s, std::unique_lock<std::mutex> &lk, bool &progress) {
-  Engine &e = s->eng;
-  MCR_TRY(housekeeping(s));
+// Lane L's step in flight: if it has finished, takes its counts (no wait otherwise).
+int lane_poll(mcr_server *s, Lane &L, bool &progress) {
+  if (!L.inflight) return MCR_OK;
+  hipError_t q = hipEventQuery(L.done);
+  if (q == hipErrorNotReady) return MCR_OK;
+  if (q != hipSuccess) return mcr_fail(MCR_E_DEVICE, std::string("lane step: ") + hipGetErrorString(q));
+  L.inflight = false;
+  progress = true;
+  MCR_TRY(engine_check(L.eng));
+  for (uint32_t i = 0; i < L.launched; i++) L.table[i].done = L.done_host[i];
+  L.counters = counters_sum(L.counters_host);
+  return MCR_OK;
+}
+
+// Plans lane L's window and launches its next step (a refill, bounce iterations and the copies of
+// its counts), recording L.done after them. Waits for nothing.
+int lane_launch(mcr_server *s, Lane &L, bool &progress) {
+  Engine &e = L.eng;
   // With no path live every started sample has finished and no job in the table has started,
   // so sample numbering starts again from 0.
-  uint32_t next = e.counts_host[cw(s->q)] == 0 ? 0 : std::min(e.counts_host[cw(C_NEXT)], s->total);
-  plan_window(s, next, e.counts_host[cw(C_FREE)]);
-  uint32_t n = (uint32_t)s->table.size();
-  if (n == 0 && e.counts_host[cw(s->q)] == 0) return deliver(s, progress);
+  uint32_t next = e.counts_host[cw(L.q)] == 0 ? 0 : std::min(e.counts_host[cw(C_NEXT)], L.total);
+  plan_window(s, L, next, e.counts_host[cw(C_FREE)]);
+  uint32_t n = (uint32_t)L.table.size();
+  if (n == 0 && e.counts_host[cw(L.q)] == 0) return MCR_OK;
   progress = true;
   MCR_TRY(engine_reserve(e, e.capacity_limit, (uint32_t)s->views.size(), n, true, s->cfg.max_lights));
-  MCR_TRY(ensure_host(s, std::max<uint32_t>(n, 1)));
-  bool mixed = false;
-  for (const SrvJob &j : s->table) mixed |= j.dev.view != 0;
-  // Samples of other views may still be in flight with no job left in the table.
-  mixed |= s->running.size() > 1;
+  MCR_TRY(ensure_host(L, std::max<uint32_t>(n, 1)));
+  bool mixed = s->running.size() > 1;  // samples of other views may be in flight with no job left
+  for (const SrvJob &j : L.table) mixed |= j.dev.view != 0;
   e.mix = mixed;
-  // The previous step waited for its copies, so the pinned staging is free.
+  // The lane's last step has completed, so its pinned staging is free.
   for (uint32_t i = 0; i < n; i++) {
-    s->jobs_host[i] = s->table[i].dev;
-    s->done_host[i] = s->table[i].done;
+    L.jobs_host[i] = L.table[i].dev;
+    L.done_host[i] = L.table[i].done;
   }
-  if (s->views_dirty) {
+  if (L.views_dirty) {
     MCR_TRY(engine_upload_views(e, s->views.data(), (uint32_t)s->views.size()));
-    s->views_dirty = false;
+    L.views_dirty = false;
   }
   if (n) {
-    HIP_TRY(hipMemcpyAsync(e.jobs, s->jobs_host, n * sizeof(JobDev), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(e.job_done, s->done_host, n * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(e.jobs, L.jobs_host, n * sizeof(JobDev), hipMemcpyHostToDevice, L.stream));
+    HIP_TRY(hipMemcpyAsync(e.job_done, L.done_host, n * sizeof(uint32_t), hipMemcpyHostToDevice, L.stream));
   }
-  MCR_TRY(engine_set_control(e, s->arena_dev, n, s->total, true));
-  uint32_t free_slots = e.counts_host[cw(C_FREE)], live = e.counts_host[cw(s->q)];
-  uint32_t k = std::min<uint32_t>(free_slots, s->total - next);
-  MCR_TRY(engine_refill(e, s->q, k, free_slots, next, live));
+  MCR_TRY(engine_set_control(e, s->arena_dev, n, L.total, true));
+  uint32_t free_slots = e.counts_host[cw(C_FREE)], live = e.counts_host[cw(L.q)];
+  uint32_t k = std::min<uint32_t>(free_slots, L.total - next);
+  MCR_TRY(engine_refill(e, L.q, k, free_slots, next, live));
   live += k;
   // Refilled slots come from the free list in the order they went idle (after a drain, every
   // slot does), and in-place regeneration would keep them scattered for the rest of the run;
   // re-sorting the queue after a refill puts every wave back on neighbouring slots (coalesced
   // path-state accesses). Steps without a refill keep the queue's own order: waves requeue in
   // the order they finish, which groups paths of similar cost.
-  if (k) MCR_TRY(engine_sort_queue(e, s->q, live));
+  if (k) MCR_TRY(engine_sort_queue(e, L.q, live));
   uint32_t max_lights = 0;
   e.tex = false;
   for (const ViewDev &v : s->views) {
     max_lights = std::max(max_lights, v.shadow_slots);
     e.tex |= v.shadow_tex != 0;
   }
-  lk.unlock();
-  auto t_dev = Clock::now();
-  int rc = MCR_OK;
-  if (live) {
+  for (uint32_t k = 0; live && k < L.launches_per_step; k++) {
     if (e.use_graph) {
-      if (s->q != 0) {  // the captured pair starts on queue 0
-        rc = engine_iteration(e, s->q, live, max_lights, nullptr, nullptr, nullptr);
-        s->q ^= 1;
-        s->iterations++;
+      if (L.q != 0) {  // the captured pair starts on queue 0
+        MCR_TRY(engine_iteration(e, L.q, live, max_lights, nullptr, nullptr, nullptr));
+        L.q ^= 1;
+        L.iterations++;
       }
-      if (!rc) rc = engine_iteration_pair(e, max_lights);
-      s->iterations += 2;
+      MCR_TRY(engine_iteration_pair(e, max_lights));
+      L.iterations += 2;
     } else {
-      rc = engine_iteration(e, s->q, live, max_lights, nullptr, nullptr, nullptr);
-      s->q ^= 1;
-      s->iterations++;
+      MCR_TRY(engine_iteration(e, L.q, live, max_lights, nullptr, nullptr, nullptr));
+      L.q ^= 1;
+      L.iterations++;
     }
   }
-  // One wait per step: counter words, finished samples per job and ray counters together.
-  if (!rc) rc = engine_fetch_counts(e);
-  if (!rc) {
-    hipError_t err = hipSuccess;
-    if (n) err = hipMemcpyAsync(s->done_host, e.job_done, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
-    if (err == hipSuccess) err = hipMemcpyAsync(s->counters_host, e.counters, COUNTER_BYTES, hipMemcpyDeviceToHost, s->stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(s->stream);
-    if (err != hipSuccess) rc = mcr_fail(MCR_E_DEVICE, std::string("job counts: ") + hipGetErrorString(err));
-  }
-  if (!rc) rc = engine_check(e);
-  if (!rc) s->counters = counters_sum(s->counters_host);
-  lk.lock();
-  s->device_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_dev).count();
-  if (rc) return rc;
-  for (uint32_t i = 0; i < n; i++) s->table[i].done = s->done_host[i];
-  return deliver(s, progress);
+  // Counter words, finished samples per job and ray counters, behind one event.
+  MCR_TRY(engine_fetch_counts(e));
+  if (n) HIP_TRY(hipMemcpyAsync(L.done_host, e.job_done, n * sizeof(uint32_t), hipMemcpyDeviceToHost, L.stream));
+  HIP_TRY(hipMemcpyAsync(L.counters_host, e.counters, COUNTER_BYTES, hipMemcpyDeviceToHost, L.stream));
+  HIP_TRY(hipEventRecord(L.done, L.stream));
+  L.launched = n;
+  L.inflight = true;
+  return MCR_OK;
+}
+
+bool lanes_busy(const mcr_server *s) {
+  for (const auto &l : s->lanes)
+    if (l->inflight || !l->table.empty() || l->eng.counts_host[cw(l->q)] != 0) return true;
+  return false;
 }
 
 void run(mcr_server *s) {
   hipSetDevice(s->ctx->device);
   std::unique_lock<std::mutex> lk(s->mu);
+  Clock::time_point busy_since;
+  bool was_busy = false;
   while (!s->stop) {
-    Engine &e = s->eng;
-    bool busy = !s->running.empty() || !s->admit_queue.empty() || !s->table.empty() ||
-                e.counts_host[cw(s->q)] != 0;
+    bool busy = !s->running.empty() || !s->admit_queue.empty() || lanes_busy(s);
     bool chores = false;
     for (auto &kv : s->reqs) chores |= kv.second->cancel || kv.second->release;
     if (!busy && !chores) {
@@ -576,9 +674,20 @@ void run(mcr_server *s) {
       continue;
     }
     bool progress = false;
-    int rc = step(s, lk, progress);
-    // Nothing to trace (e.g. every queued request waits for memory): sleep until woken.
-    if (!rc && !progress) s->wake.wait_for(lk, std::chrono::milliseconds(2));
+    int rc = housekeeping(s);
+    // Each lane: take a finished step, deliver its tiles, launch its next step. The lanes run
+    // side by side on their own streams; this thread only ever waits for short encodes.
+    for (auto &l : s->lanes) {
+      if (!rc) rc = lane_poll(s, *l, progress);
+      if (!rc && !l->inflight) rc = deliver(s, *l, progress);
+      if (!rc && !l->inflight) rc = lane_launch(s, *l, progress);
+    }
+    bool inflight = false;
+    for (auto &l : s->lanes) inflight |= l->inflight;
+    auto now = Clock::now();
+    if (inflight && !was_busy) busy_since = now;
+    if (!inflight && was_busy) s->device_ms += std::chrono::duration<double, std::milli>(now - busy_since).count();
+    was_busy = inflight;
     if (rc) {
       // A device failure ends every request; the server keeps answering status calls.
       s->fatal = rc;
@@ -592,16 +701,100 @@ void run(mcr_server *s) {
       }
       s->admit_queue.clear();
       s->running.clear();
-      s->table.clear();
+      for (auto &l : s->lanes) l->table.clear();
       s->changed.notify_all();
       s->wake.wait(lk, [&] { return s->stop; });
+      continue;
+    }
+    if (progress) continue;
+    if (inflight) {
+      // Steps in flight: look again shortly (API calls take the lock meanwhile).
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(40));
+      lk.lock();
+    } else {
+      // Nothing to trace (e.g. every queued request waits for memory): sleep until woken.
+      s->wake.wait_for(lk, std::chrono::milliseconds(2));
     }
   }
+  if (was_busy) s->device_ms += std::chrono::duration<double, std::milli>(Clock::now() - busy_since).count();
 }
 
 }  // namespace
 
 extern "C" {
+
+static void server_free(mcr_server *s) {
+  hipSetDevice(s->ctx->device);
+  for (auto &l : s->lanes)
+    if (l->stream) hipStreamSynchronize(l->stream);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (auto &kv : s->reqs) {
+    if (kv.second->image) hipFreeAsync(kv.second->image, s->stream);
+    if (kv.second->sum) hipFreeAsync(kv.second->sum, s->stream);
+  }
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (auto &l : s->lanes) {
+    engine_free(l->eng);
+    hipFree(l->enc_dev);
+    if (l->enc_host) hipHostFree(l->enc_host);
+    if (l->encoded) hipEventDestroy(l->encoded);
+    if (l->jobs_host) hipHostFree(l->jobs_host);
+    if (l->done_host) hipHostFree(l->done_host);
+    if (l->counters_host) hipHostFree(l->counters_host);
+    if (l->done) hipEventDestroy(l->done);
+    if (l->stream) hipStreamDestroy(l->stream);
+  }
+  s->lanes.clear();
+  for (mcr_scene *scene : s->scenes) scene_drop(scene);
+  s->scenes.clear();
+  hipFree(s->arena_dev);
+  if (s->admitted) hipEventDestroy(s->admitted);
+  if (s->read_ready) hipEventDestroy(s->read_ready);
+  if (s->copy_stream) hipStreamDestroy(s->copy_stream);
+  if (s->stream) hipStreamDestroy(s->stream);
+}
+
+const uint32_t LANE_JOBS = 1u << 16;  // job-list entries a lane starts with
+
+// A lane on CUs [cu_first, cu_first + cu_count) of the device (all of them when cu_count is 0 or
+// a CU-masked stream cannot be made), with `slots` path slots.
+static int lane_init(mcr_server *s, Lane &L, uint32_t slots, uint32_t window, bool previews, bool passes,
+                     int cu_first, int cu_count, int cus) {
+  L.previews = previews;
+  L.passes = passes;
+  L.window = window;
+  if (cu_count > 0 && cu_count < cus) {
+    uint32_t words = (uint32_t)(cus + 31) / 32;
+    std::vector<uint32_t> mask(words, 0u);
+    for (int k = cu_first; k < cu_first + cu_count && k < cus; k++) mask[(size_t)k / 32] |= 1u << (k % 32);
+    if (hipExtStreamCreateWithCUMask(&L.stream, words, mask.data()) != hipSuccess) {
+      (void)hipGetLastError();
+      L.stream = nullptr;
+    }
+  }
+  if (!L.stream) HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&L.encoded, hipEventDisableTiming));
+  HIP_TRY(hipHostMalloc(&L.enc_host, s->enc_cap * sizeof(EncodeItem), hipHostMallocDefault));
+  HIP_TRY(hipMalloc(&L.enc_dev, s->enc_cap * sizeof(EncodeItem)));
+  MCR_TRY(engine_init(L.eng, s->ctx->device, L.stream));
+  L.eng.capacity_limit = slots;
+  L.eng.use_graph = s->cfg.use_graph != 0;
+  HIP_TRY(hipHostMalloc(&L.counters_host, COUNTER_BYTES, hipHostMallocDefault));
+  memset(L.counters_host, 0, COUNTER_BYTES);
+  // Job lists sized up front: growing one later frees device memory, which waits for every lane.
+  MCR_TRY(engine_reserve(L.eng, slots, s->cfg.max_requests, LANE_JOBS, true, s->cfg.max_lights));
+  MCR_TRY(ensure_host(L, LANE_JOBS));
+  if (engine_capacity_for(L.eng, s->cfg.max_lights) < slots)
+    return mcr_fail(MCR_E_INVALID, "slots x max_lights exceeds the light-slot memory budget; lower one of them");
+  MCR_TRY(engine_reset(L.eng));
+  HIP_TRY(counters_clear(L.eng.counters, L.stream));
+  HIP_TRY(hipStreamSynchronize(L.stream));
+  memset(L.eng.counts_host, 0, sizeof(uint32_t) * COUNT_WORDS_ALLOC);
+  L.eng.counts_host[cw(C_FREE)] = L.eng.capacity;
+  return MCR_OK;
+}
 
 int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **out) {
   if (!c || !out) return mcr_fail(MCR_E_INVALID, "null argument");
@@ -611,41 +804,68 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   if (config) s->cfg = *config;
   else s->cfg.use_graph = 1;
   if (!s->cfg.hbm_budget) s->cfg.hbm_budget = 200ull << 30;
-  if (!s->cfg.slots) s->cfg.slots = SERVER_DEFAULT_SLOTS;
+  if (!s->cfg.lanes) s->cfg.lanes = 2;
+  if (s->cfg.lanes > 2) return mcr_fail(MCR_E_INVALID, "lanes must be 1 or 2");
+  // One lane bounds its slots for short previews; a pass lane can take more for throughput.
+  if (!s->cfg.slots) s->cfg.slots = s->cfg.lanes == 1 ? SERVER_DEFAULT_SLOTS : SERVER_DEFAULT_PASS_SLOTS;
   if (s->cfg.slots < 256 || s->cfg.slots > (1u << 26)) return mcr_fail(MCR_E_INVALID, "slots must be in [256, 2^26]");
   if (!s->cfg.window) s->cfg.window = s->cfg.slots;
   if (!s->cfg.max_requests) s->cfg.max_requests = 64;
   if (!s->cfg.max_lights) s->cfg.max_lights = 16;
   if (s->cfg.max_lights > MCR_MAX_LIGHTS) return mcr_fail(MCR_E_INVALID, "max_lights beyond MCR_MAX_LIGHTS_PER_SCENE");
-  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreateWithFlags(&s->encoded, hipEventDisableTiming));
-  MCR_TRY(engine_init(s->eng, c->device, s->stream));
-  s->eng.capacity_limit = s->cfg.slots;
-  s->eng.use_graph = s->cfg.use_graph != 0;
+  if (!s->cfg.preview_slots) s->cfg.preview_slots = SERVER_DEFAULT_PREVIEW_SLOTS;
+  if (s->cfg.preview_slots < 256 || s->cfg.preview_slots > (1u << 26))
+    return mcr_fail(MCR_E_INVALID, "preview_slots must be in [256, 2^26]");
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  if (!s->cfg.preview_cus) s->cfg.preview_cus = (uint32_t)std::max(1, cus * 3 / 32);
+  if (s->cfg.lanes == 2 && (int)s->cfg.preview_cus >= cus)
+    return mcr_fail(MCR_E_INVALID, "preview_cus must leave CUs for the pass lane");
+  int rc = MCR_OK;
+  auto fail_free = [&](int code) {
+    server_free(s.get());
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->admitted, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->read_ready, hipEventDisableTiming) != hipSuccess)
+    return fail_free(mcr_fail(MCR_E_DEVICE, "server streams"));
+  s->enc_cap = 1u << 16;
   s->views.assign(s->cfg.max_requests, ViewDev());
   memset(s->views.data(), 0, s->views.size() * sizeof(ViewDev));
   for (int i = (int)s->cfg.max_requests - 1; i >= 0; i--) s->view_free.push_back(i);
-  // Tile results wait here until encoded: the window plus tiles waiting for an earlier pass.
-  s->arena_floats = std::max<uint64_t>((uint64_t)s->cfg.window * 4 * 16, 1ull << 26);
-  HIP_TRY(hipMalloc(&s->arena_dev, s->arena_floats * sizeof(float)));
-  s->arena.reset(s->arena_floats);
-  HIP_TRY(hipHostMalloc(&s->counters_host, COUNTER_BYTES, hipHostMallocDefault));
-  memset(s->counters_host, 0, COUNTER_BYTES);
-  s->enc_cap = 1u << 16;
-  HIP_TRY(hipHostMalloc(&s->enc_host, s->enc_cap * sizeof(EncodeItem), hipHostMallocDefault));
-  HIP_TRY(hipMalloc(&s->enc_dev, s->enc_cap * sizeof(EncodeItem)));
-  MCR_TRY(engine_reserve(s->eng, s->cfg.slots, s->cfg.max_requests, 1024, true, s->cfg.max_lights));
-  if (engine_capacity_for(s->eng, s->cfg.max_lights) < s->cfg.slots)
-    return mcr_fail(MCR_E_INVALID, "slots x max_lights exceeds the light-slot memory budget; lower one of them");
-  MCR_TRY(engine_reset(s->eng));
-  HIP_TRY(counters_clear(s->eng.counters, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  memset(s->eng.counts_host, 0, sizeof(uint32_t) * COUNT_WORDS_ALLOC);
-  s->eng.counts_host[cw(C_FREE)] = s->eng.capacity;
+  if (s->cfg.lanes == 1) {
+    s->lanes.emplace_back(new Lane());
+    rc = lane_init(s.get(), *s->lanes[0], s->cfg.slots, s->cfg.window, true, true, 0, 0, cus);
+  } else {
+    // The preview lane on the last preview_cus CUs, the pass lane on the others.
+    int pc = (int)s->cfg.preview_cus;
+    s->lanes.emplace_back(new Lane());
+    s->lanes.emplace_back(new Lane());
+    rc = lane_init(s.get(), *s->lanes[0], s->cfg.preview_slots, s->cfg.preview_slots, true, false, cus - pc, pc, cus);
+    if (!rc) rc = lane_init(s.get(), *s->lanes[1], s->cfg.slots, s->cfg.window, false, true, 0, cus - pc, cus);
+    s->lanes[1]->launches_per_step = 2;
+  }
+  if (rc) return fail_free(rc);
+  // Tile results wait in the arena until encoded: each lane's windows plus tiles waiting for an
+  // earlier pass, in one range per lane.
+  std::vector<uint64_t> share;
+  for (auto &l : s->lanes)
+    share.push_back(std::max<uint64_t>((uint64_t)l->window * (l->iterations_per_step() + 2) * 4 * 2, 1ull << 24));
+  s->arena_floats = 0;
+  for (uint64_t n : share) s->arena_floats += n;
+  if (hipMalloc(&s->arena_dev, s->arena_floats * sizeof(float)) != hipSuccess)
+    return fail_free(mcr_fail(MCR_E_DEVICE, "server arena"));
+  uint64_t at = 0;
+  for (size_t i = 0; i < s->lanes.size(); i++) {
+    s->lanes[i]->arena.reset(at, share[i]);
+    at += share[i];
+  }
   uint64_t fixed = (uint64_t)s->arena_floats * 4;
   s->bytes_used = fixed;
-  if (s->bytes_used > s->cfg.hbm_budget) return mcr_fail(MCR_E_OVERFLOW, "the HBM budget does not cover the server's own buffers");
+  if (s->bytes_used > s->cfg.hbm_budget)
+    return fail_free(mcr_fail(MCR_E_OVERFLOW, "the HBM budget does not cover the server's own buffers"));
   s->t0 = Clock::now();
   mcr_server *raw = s.release();
   raw->thread = std::thread(run, raw);
@@ -661,23 +881,7 @@ void mcr_server_destroy(mcr_server *s) {
   }
   s->wake.notify_all();
   if (s->thread.joinable()) s->thread.join();
-  hipSetDevice(s->ctx->device);
-  hipStreamSynchronize(s->stream);
-  for (auto &kv : s->reqs) {
-    hipFree(kv.second->image);
-    hipFree(kv.second->sum);
-  }
-  engine_free(s->eng);
-  for (mcr_scene *scene : s->scenes) scene_drop(scene);
-  hipFree(s->arena_dev);
-  hipFree(s->enc_dev);
-  if (s->enc_host) hipHostFree(s->enc_host);
-  if (s->jobs_host) hipHostFree(s->jobs_host);
-  if (s->done_host) hipHostFree(s->done_host);
-  if (s->counters_host) hipHostFree(s->counters_host);
-  if (s->encoded) hipEventDestroy(s->encoded);
-  if (s->copy_stream) hipStreamDestroy(s->copy_stream);
-  if (s->stream) hipStreamDestroy(s->stream);
+  server_free(s);
   delete s;
 }
 
@@ -817,8 +1021,11 @@ int mcr_server_read(mcr_server *s, uint64_t id, uint8_t *rgba) {
     return MCR_OK;
   }
   HIP_TRY(hipSetDevice(s->ctx->device));
-  // After every encode launched so far; the image stays allocated while the lock is held.
-  HIP_TRY(hipStreamWaitEvent(s->copy_stream, s->encoded, 0));
+  // After everything queued on the server stream so far (the image's allocation and clearing)
+  // and every encode launched; the image stays allocated while the lock is held.
+  HIP_TRY(hipEventRecord(s->read_ready, s->stream));
+  HIP_TRY(hipStreamWaitEvent(s->copy_stream, s->read_ready, 0));
+  after_encodes(s, s->copy_stream);
   HIP_TRY(hipMemcpyAsync(rgba, r.image, (size_t)r.r.width * r.r.height * 4, hipMemcpyDeviceToHost, s->copy_stream));
   HIP_TRY(hipStreamSynchronize(s->copy_stream));
   return MCR_OK;
@@ -839,12 +1046,14 @@ int mcr_server_release(mcr_server *s, uint64_t id) {
 int mcr_server_stats_get(mcr_server *s, mcr_server_stats *out) {
   if (!s || !out) return mcr_fail(MCR_E_INVALID, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
-  out->closest_queries = s->counters.closest_queries;
-  out->any_queries = s->counters.any_queries;
+  DevCounters c = s->counters();
+  out->closest_queries = c.closest_queries;
+  out->any_queries = c.any_queries;
   out->paths = s->paths;
-  out->iterations = s->iterations;
-  out->shade_waves = s->counters.shade_waves;
-  out->mixed_view_waves = s->counters.mixed_view_waves;
+  out->iterations = 0;
+  for (const auto &l : s->lanes) out->iterations += l->iterations;
+  out->shade_waves = c.shade_waves;
+  out->mixed_view_waves = c.mixed_view_waves;
   out->bytes_in_use = s->bytes_used;
   out->requests_live = s->running.size();
   out->requests_queued = s->admit_queue.size();

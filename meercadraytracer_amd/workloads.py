@@ -14,7 +14,9 @@
                  upload costs of a scene version (geometry, shading with texels, shading only).
 Log-uniform draws are exp(U[ln a, ln b]) rounded to an integer.
 """
+import json
 import math
+import os
 import time
 
 import numpy as np
@@ -300,36 +302,51 @@ def batched_equivalent(ctx, scenes, specs, slots=1 << 21, log=print):
 
 
 def autotune_served(ctx, scenes, target_ms=50.0, rate=8.0, seconds=3.0, margin=0.9,
-                    candidates=(1 << 21, 3 << 19, 1 << 20, 3 << 18, 1 << 19, 1 << 18), log=print):
-    """Path slots for the served form: a short run of the mix at each candidate slot count, from the
-    most slots down; the first whose p99 time-to-first-preview stays under margin * `target_ms`
-    wins (more slots give more throughput, fewer give shorter bounce iterations and so earlier
-    previews; the margin covers the spread between a short trial and a long run)."""
+                    candidates=(1 << 23, 1 << 22, 1 << 21), preview_cus=(24, 32), log=print):
+    """Pass-lane path slots and preview-lane CUs for the served form: a short run of the mix for
+    each pair; the fastest (Mrays/s) whose p99 time-to-first-preview stays under margin * target_ms
+    wins (the preview lane keeps previews off the pass lane's iterations, so more pass slots buy
+    throughput; more preview CUs shorten previews and take CUs from the passes; the margin covers
+    the spread between a short trial and a long run)."""
     trials = []
     for slots in candidates:
-        r = served_mix(ctx, rate, seconds, seed=7, scenes=scenes, slots=slots, drain_seconds=2.0,
-                       log=lambda *_: None)
-        trials.append({'slots': slots, 'ttfp_ms_p99': r['ttfp_ms_p99'], 'mrays_s': r['mrays_s'],
-                       'ms_per_iteration': r['ms_per_iteration']})
-        log(f'served autotune: {trials[-1]}')
-        if r['ttfp_ms_p99'] is not None and r['ttfp_ms_p99'] <= margin * target_ms:
-            return trials[-1], trials
-    return trials[-1], trials
+        for cus in preview_cus:
+            r = served_mix(ctx, rate, seconds, seed=7, scenes=scenes, slots=slots, preview_cus=cus,
+                           drain_seconds=2.0, log=lambda *_: None)
+            trials.append({'slots': slots, 'preview_cus': cus, 'ttfp_ms_p99': r['ttfp_ms_p99'],
+                           'mrays_s': r['mrays_s']})
+            log(f'served autotune: {trials[-1]}')
+    ok = [t for t in trials if t['ttfp_ms_p99'] is not None and t['ttfp_ms_p99'] <= margin * target_ms]
+    best = max(ok or trials, key=lambda t: t['mrays_s'])
+    return best, trials
 
 
 # ---- served form ---------------------------------------------------------------------------------
 def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_frac=0.3,
-               budget=200 << 30, slots=0, window=0, max_requests=1024, drain_seconds=60.0, log=print):
+               budget=200 << 30, slots=0, window=0, max_requests=1024, drain_seconds=60.0, lanes=0,
+               preview_slots=0, preview_cus=0, warmup=True, log=print):
     """Runs the arrival process for `seconds`, waits up to `drain_seconds` for the admitted
     requests, cancels what is left, and returns the latency and throughput figures."""
     rng = np.random.Generator(np.random.PCG64(seed))
     own = scenes is None
     if own:
         scenes = resident_scenes(ctx, seed, n_scenes, 50_000, 4_000_000, log)
-    srv = native.Server(ctx, hbm_budget=budget, slots=slots, window=window, max_requests=max_requests)
+    srv = native.Server(ctx, hbm_budget=budget, slots=slots, window=window, max_requests=max_requests,
+                        lanes=lanes, preview_slots=preview_slots, preview_cus=preview_cus)
     try:
         for gpu, _, _ in scenes:
             srv.add_scene(gpu)
+        # A first request warms the new server (graph captures, first use of each lane and of the
+        # allocation pool); its time to first preview is reported on its own, not in the mix.
+        warm_ttfp = None
+        if warmup:
+            gpu, _, options = scenes[0]
+            rid = srv.submit(gpu, compile_camera(orbit_camera(1920, 1080, 0.0)), trace_settings(options),
+                             1920, 1080, 2, 64, 4)
+            srv.wait(rid, 60000)
+            st = srv.status(rid)
+            warm_ttfp = round(st['first_preview_ms'] - st['submit_ms'], 2)
+            srv.release(rid)
         live = []          # (rid, scene index, w, h, spp, angle)
         record = {}        # rid -> kind
         t_next = 0.0
@@ -364,6 +381,7 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
             record[rid] = (kind, (k, w, h, spp, angle))
             submitted += 1
         t_arrivals = time.perf_counter() - t0
+        stats_mid = srv.stats()  # the device has been kept busy by arrivals until here
         deadline = time.perf_counter() + drain_seconds
         for rid in record:
             left = (deadline - time.perf_counter()) * 1e3
@@ -383,6 +401,8 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
         ttfp, ttt, cancelled_before_preview, unfinished, wait, iters = [], [], 0, 0, [], []
         for rid in record:
             st = srv.status(rid)
+            if os.environ.get('MCR_SERVED_TRACE'):
+                log(json.dumps({'rid': rid, 'kind': record[rid][0], 'spec': record[rid][1], **st}))
             if st['first_preview_ms'] >= 0:
                 ttfp.append(st['first_preview_ms'] - st['submit_ms'])
                 wait.append(st['admit_ms'] - st['submit_ms'])
@@ -394,13 +414,17 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
             elif st['state'] != abi.REQ_CANCELLED:
                 unfinished += 1
         rays = (stats1['closest_queries'] + stats1['any_queries']) - (stats0['closest_queries'] + stats0['any_queries'])
+        rays_mid = ((stats_mid['closest_queries'] + stats_mid['any_queries']) -
+                    (stats0['closest_queries'] + stats0['any_queries']))
+        wall_mid = max(stats_mid['uptime_ms'] - stats0['uptime_ms'], 1e-9)
         wall = max(stats1['uptime_ms'] - stats0['uptime_ms'], 1e-9)
         device = max(stats1['device_ms'] - stats0['device_ms'], 1e-9)
 
         def pct(a, q):
             return round(float(np.percentile(a, q)), 2) if a else None
         completed_specs = [record[rid][1] for rid in record if srv.status(rid)['state'] == abi.REQ_DONE]
-        return {'completed_specs': completed_specs, 'rate_per_s': rate, 'arrival_seconds': round(t_arrivals, 2), 'submitted': submitted,
+        return {'completed_specs': completed_specs, 'first_request_ttfp_ms': warm_ttfp, 'rate_per_s': rate,
+                'arrival_seconds': round(t_arrivals, 2), 'submitted': submitted,
                 'restarts': restarts, 'previews': len(ttfp), 'completed': len(ttt),
                 'cancelled_before_preview': cancelled_before_preview, 'unfinished_at_drain': unfinished,
                 'ttfp_ms_p50': pct(ttfp, 50), 'ttfp_ms_p99': pct(ttfp, 99),
@@ -412,6 +436,7 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
                                               max(1, stats1['shade_waves'] - stats0['shade_waves']), 4),
                 'ms_per_iteration': round(device / max(1, stats1['iterations'] - stats0['iterations']), 3),
                 'mrays_s': round(rays / (wall / 1e3) / 1e6, 2),
+                'mrays_s_arrivals': round(rays_mid / (wall_mid / 1e3) / 1e6, 2),
                 'mrays_s_device_busy': round(rays / (device / 1e3) / 1e6, 2), 'wall_seconds': round(t_end, 2),
                 'paths': int(stats1['paths'] - stats0['paths'])}
     finally:

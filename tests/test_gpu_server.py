@@ -68,12 +68,12 @@ def _server(world, **kw):
     return srv
 
 
-@pytest.mark.parametrize('graph', [True, False])
-def test_alone_equals_raytrace(world, graph):
-    srv = _server(world, use_graph=graph)
+@pytest.mark.parametrize('graph,lanes', [(True, 2), (False, 2), (True, 1), (False, 1)])
+def test_alone_equals_raytrace(world, graph, lanes):
+    srv = _server(world, use_graph=graph, lanes=lanes)
     try:
         for spec in REQUESTS[:2]:
-            print(f'request {spec[:6]} graph {graph}', flush=True)
+            print(f'request {spec[:6]} graph {graph} lanes {lanes}', flush=True)
             rid = _submit(srv, world, spec)
             assert srv.wait(rid, 60000) == abi.REQ_DONE
             st = srv.status(rid)
@@ -85,10 +85,12 @@ def test_alone_equals_raytrace(world, graph):
         srv.close()
 
 
-def test_mix_equals_alone(world):
+@pytest.mark.parametrize('lanes', [2, 1])
+def test_mix_equals_alone(world, lanes):
     """Five requests at once, then three more while they run; slots small enough that every
-    wave mixes requests."""
-    srv = _server(world, slots=512, window=1024)
+    wave mixes requests. Two lanes: the previews run on their own lane and CUs beside the
+    passes."""
+    srv = _server(world, slots=512, window=1024, lanes=lanes, preview_slots=256)
     try:
         ids = [(_submit(srv, world, spec), spec) for spec in REQUESTS]
         time.sleep(0.01)
