@@ -656,7 +656,7 @@ int64_t layout_job(const mcr_tile_job &j, uint32_t index, uint32_t view, uint64_
   }
   uint32_t cols = (j.tile.width + block - 1) / block, rows = (j.tile.height + block - 1) / block;
   out = JobDev{j.tile.x, j.tile.y, j.tile.width, j.tile.height, j.width, j.height, j.pass, block,
-               j.out_offset, (uint32_t)start, cols, view, 0};
+               j.out_offset, (uint32_t)start, cols, view, rows};
   return (int64_t)cols * rows;
 }
 

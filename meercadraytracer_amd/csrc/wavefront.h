@@ -178,8 +178,8 @@ struct JobDev {  // one tile job, device side
   int32_t tx, ty, tw, th;
   uint32_t width, height, pass, block;
   uint64_t out_offset;
-  uint32_t sample_start, cols;
-  uint32_t view, pad;
+  uint32_t sample_start, cols;  // samples [sample_start, + cols * rows): cols x rows blocks
+  uint32_t view, rows;
 };
 
 // One tile result to accumulate and encode (writeTile): sum may be null (a preview tile).
