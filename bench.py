@@ -341,7 +341,8 @@ def main():
     ap.add_argument('--height', type=int, default=1080)
     ap.add_argument('--no-batched', action='store_true', help='skip the batched-form measurement')
     ap.add_argument('--no-served', action='store_true', help='skip the served-mix measurement')
-    ap.add_argument('--served-rates', default='2,8', help='request arrival rates (per second) to sweep')
+    ap.add_argument('--served-rates', default='2,8,16',
+                    help='request arrival rates (per second) to sweep; the last one should saturate the device')
     ap.add_argument('--served-seconds', type=float, default=12.0, help='arrival window per rate')
     ap.add_argument('--served-slots', type=int, default=0, help='server path slots (0 = default)')
     ap.add_argument('--no-precision', action='store_true', help='skip the precision-form probe measurement')

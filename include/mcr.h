@@ -313,8 +313,11 @@ typedef struct mcr_server_stats {
   uint64_t closest_queries, any_queries, paths, iterations;
   uint64_t shade_waves, mixed_view_waves;  /* shade waves, and those holding several requests */
   uint64_t bytes_in_use, requests_live, requests_queued;
-  double device_ms;                /* time spent waiting for the device */
+  double device_ms;                /* time with at least one lane's step in flight */
   double uptime_ms;
+  double host_ms;                  /* the server thread's own work: admission, planning, encodes, launches */
+  double plan_ms, deliver_ms;      /* of which job-window planning, and finished-tile delivery and encodes */
+  double pass_busy_ms;             /* time the lane taking the passes has a step in flight */
 } mcr_server_stats;
 int mcr_server_create(mcr_ctx *ctx, const mcr_server_config *config, mcr_server **out);
 void mcr_server_destroy(mcr_server *server);

@@ -221,7 +221,9 @@ class ServerStats(C.Structure):
     _fields_ = [('closest_queries', C.c_uint64), ('any_queries', C.c_uint64), ('paths', C.c_uint64),
                 ('iterations', C.c_uint64), ('shade_waves', C.c_uint64), ('mixed_view_waves', C.c_uint64),
                 ('bytes_in_use', C.c_uint64), ('requests_live', C.c_uint64),
-                ('requests_queued', C.c_uint64), ('device_ms', C.c_double), ('uptime_ms', C.c_double)]
+                ('requests_queued', C.c_uint64), ('device_ms', C.c_double), ('uptime_ms', C.c_double),
+                ('host_ms', C.c_double), ('plan_ms', C.c_double), ('deliver_ms', C.c_double),
+                ('pass_busy_ms', C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

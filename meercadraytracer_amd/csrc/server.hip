@@ -49,6 +49,12 @@ struct SrvTile {
   bool covered = false;             // preview or a pass written
   std::map<uint32_t, uint64_t> ready;  // finished passes waiting for their turn: pass -> arena offset
   uint64_t preview = UINT64_MAX;    // arena offset of a finished preview not yet written
+  uint8_t listed = 0;               // lanes (by bit) whose encode list holds this tile
+};
+
+struct TileRef {
+  uint64_t req;
+  uint32_t tile;
 };
 
 struct SrvRequest {
@@ -135,7 +141,7 @@ struct Arena {
 // its slots, and the pass lane can run as many slots as throughput wants.
 struct Lane {
   Engine eng;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;    // the lane's steps, on the lane's CUs
   hipEvent_t done = nullptr;       // recorded after a launched step's count copies
   bool previews = true, passes = true;  // the items the lane takes
   uint32_t window = 0;             // unstarted samples queued ahead of the slots
@@ -145,6 +151,8 @@ struct Lane {
   bool inflight = false, views_dirty = true;
   uint32_t launched = 0;           // jobs of the step in flight
   uint64_t iterations = 0;
+  Clock::time_point launched_at;   // when the step in flight was launched
+  double busy_ms = 0;              // launch to seen-complete, summed over the lane's steps
   // Bounce iterations per step: a captured pair, or one direct launch; the pass lane of two lanes
   // runs two of those per step (its steps are long; fewer host round trips between them).
   uint32_t launches_per_step = 1;
@@ -155,12 +163,14 @@ struct Lane {
   void *counters_host = nullptr;   // pinned: the device counter copies
   DevCounters counters = {};
   // The lane's tile results wait in its own range of the server's arena until encoded, and the
-  // lane encodes them on its own stream: a range freed after an encode is reused only by the
-  // lane's later steps, which follow the encode on that stream.
+  // lane encodes them on its own encode stream: a range freed after an encode is reused only by
+  // the lane's later steps, which wait for the encode.
   Arena arena;
   EncodeItem *enc_host = nullptr, *enc_dev = nullptr;  // staging of the encode items
+  hipStream_t enc_stream = nullptr;  // the lane's encodes
   hipEvent_t encoded = nullptr;    // recorded after every encode launch of the lane
-  bool enc_more = false;           // tile results of the lane wait for their encodes
+  std::vector<TileRef> dirty;      // tiles holding a result of this lane that waits for its encode
+  uint8_t bit = 1;                 // this lane's bit in SrvTile::listed
 };
 
 // makeTiles (progressive.ts:160-170): row-major tiles, stable-sorted by the distance of the tile
@@ -206,6 +216,7 @@ struct mcr_server {
   // stats
   uint64_t paths = 0;
   double device_ms = 0;            // time with at least one lane's step in flight
+  double host_ms = 0, plan_ms = 0, deliver_ms = 0;  // the server thread's own work
   // thread
   std::thread thread;
   std::mutex mu;
@@ -360,7 +371,7 @@ void plan_window(mcr_server *s, Lane &L, uint32_t next, uint32_t idle) {
   uint32_t cursor = base;
   keep.reserve(L.table.size());
   for (SrvJob &j : L.table) {
-    if (j.done == j.size) continue;  // finished: handled by deliver()
+    if (j.done == j.size) continue;  // finished: taken out by collect() before this runs
     if (j.dev.sample_start < next) {
       keep.push_back(j);
       continue;
@@ -431,9 +442,9 @@ void plan_window(mcr_server *s, Lane &L, uint32_t next, uint32_t idle) {
   L.total = start;
 }
 
-// Finished jobs of lane L go to their requests; L's tile results are written in pass order by
-// encodes on L's stream (after L's step has been seen complete). Sets progress if anything moved.
-int deliver(mcr_server *s, Lane &L, bool &progress) {
+// Finished jobs of lane L (its step has been seen complete) go to their requests: tile results
+// wait in the arena, listed for the lane's encodes. Sets progress if anything moved.
+void collect(mcr_server *s, Lane &L, bool &progress) {
   std::vector<SrvJob> still;
   for (SrvJob &j : L.table) {
     if (j.done != j.size) {
@@ -446,85 +457,118 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
     s->paths += j.size;
     r.jobs_live--;
     SrvTile *t = r.state == MCR_REQ_RUNNING ? &r.tiles[j.item.tile] : nullptr;
-    if (!t) L.arena.release(j.arena, j.floats);
-    else if (j.item.block > 1) t->preview = j.arena;
+    if (!t) {
+      L.arena.release(j.arena, j.floats);
+      continue;
+    }
+    if (j.item.block > 1) t->preview = j.arena;
     else t->ready[j.item.pass] = j.arena;
-    L.enc_more |= t != nullptr;
+    if (!(t->listed & L.bit)) {
+      t->listed |= L.bit;
+      L.dirty.push_back(TileRef{r.id, j.item.tile});
+    }
   }
   L.table.swap(still);
-  if (!L.enc_more) return MCR_OK;  // no tile result of the lane waits for its encode
-  L.enc_more = false;
-  // Rounds of one launch each; a tile is staged at most once per round, so no two items of a
-  // launch write the same pixel.
+}
+
+// L's tile results are written in pass order by encodes on the lane's encode stream, which run
+// beside the lane's next step (launched first, so the device does not wait for this host work).
+// The step whose results these are has been seen complete by the host, so the encodes need no
+// wait for it; the lane's later steps wait for the encodes before they reuse the arena ranges
+// freed here. Sets progress if anything moved.
+int deliver(mcr_server *s, Lane &L, bool &progress) {
+  if (L.dirty.empty()) return MCR_OK;  // no tile result of the lane waits for its encode
+  // Rounds of one launch each over the tiles holding a result of this lane; a tile is staged at
+  // most once per round, so no two items of a launch write the same pixel. A tile stays listed
+  // while it may hold another item for a later round.
   uint32_t nenc = 0, launched = 0;
   std::vector<std::pair<uint64_t, uint64_t>> to_free;  // arena ranges, freed after the launches
   // The staging of the lane's last encodes may still be in its copy to the device.
   HIP_TRY(hipEventSynchronize(L.encoded));
+  std::vector<TileRef> cur, again;
+  cur.swap(L.dirty);
   for (;;) {
     uint32_t first = nenc;
-    for (uint64_t id : s->running) {
-      SrvRequest &r = *s->reqs[id];
-      if (r.state != MCR_REQ_RUNNING) continue;
-      for (uint32_t i = 0; i < r.tiles.size() && nenc < s->enc_cap; i++) {
-        SrvTile &t = r.tiles[i];
-        uint64_t floats = (uint64_t)t.t.width * t.t.height * 4;
-        uint64_t data;
-        bool preview = false;
-        if (L.previews && t.preview != UINT64_MAX) {
-          // progressive.ts:257-258: a preview is written only over a tile with no samples yet.
-          data = t.preview;
-          t.preview = UINT64_MAX;
-          to_free.emplace_back(data, floats);
-          if (t.samples != 0 || t.ready.count(0)) continue;
-          preview = true;
-        } else if (L.passes) {
-          auto it = t.ready.find(t.samples);
-          if (it == t.ready.end()) continue;
-          data = it->second;
-          t.ready.erase(it);
-          to_free.emplace_back(data, floats);
-          t.samples++;
-          if (t.samples == r.r.samples) r.finished_tiles++;
-        } else {
+    again.clear();
+    for (const TileRef &d : cur) {
+      auto f = s->reqs.find(d.req);
+      if (f == s->reqs.end()) continue;
+      SrvRequest &r = *f->second;
+      SrvTile &t = r.tiles[d.tile];
+      if (r.state != MCR_REQ_RUNNING) {
+        t.listed &= (uint8_t)~L.bit;
+        continue;
+      }
+      if (nenc == s->enc_cap) {  // waits for the next call
+        L.dirty.push_back(d);
+        continue;
+      }
+      uint64_t floats = (uint64_t)t.t.width * t.t.height * 4;
+      uint64_t data;
+      bool preview = false;
+      if (L.previews && t.preview != UINT64_MAX) {
+        // progressive.ts:257-258: a preview is written only over a tile with no samples yet.
+        data = t.preview;
+        t.preview = UINT64_MAX;
+        to_free.emplace_back(data, floats);
+        again.push_back(d);  // its passes, in a later round
+        if (t.samples != 0 || t.ready.count(0)) continue;
+        preview = true;
+      } else if (L.passes) {
+        auto it = t.ready.find(t.samples);
+        if (it == t.ready.end()) {
+          t.listed &= (uint8_t)~L.bit;
           continue;
         }
-        bool wide = r.r.samples > WIDE_SUM_SAMPLES;
-        L.enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
-                                        preview || !wide ? nullptr : reinterpret_cast<double *>(r.sum),
-                                        s->arena_dev + data, r.r.width,
-                                        t.t.x, t.t.y, t.t.width, t.t.height, preview ? 1u : t.samples,
-                                        r.r.srgb, 0};
-        if (!t.covered) {
-          t.covered = true;
-          r.covered++;
-        }
+        data = it->second;
+        t.ready.erase(it);
+        to_free.emplace_back(data, floats);
+        t.samples++;
+        if (t.samples == r.r.samples) r.finished_tiles++;
+        again.push_back(d);
+      } else {
+        t.listed &= (uint8_t)~L.bit;
+        continue;
+      }
+      bool wide = r.r.samples > WIDE_SUM_SAMPLES;
+      L.enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
+                                      preview || !wide ? nullptr : reinterpret_cast<double *>(r.sum),
+                                      s->arena_dev + data, r.r.width,
+                                      t.t.x, t.t.y, t.t.width, t.t.height, preview ? 1u : t.samples,
+                                      r.r.srgb, 0};
+      if (!t.covered) {
+        t.covered = true;
+        r.covered++;
       }
     }
-    if (nenc == first) break;
-    uint32_t blocks = 0;
-    for (uint32_t k = first; k < nenc; k++) {
-      L.enc_host[k].block_start = blocks;
-      blocks += (uint32_t)(((uint64_t)L.enc_host[k].tw * L.enc_host[k].th + 255) / 256);
+    if (nenc > first) {
+      uint32_t blocks = 0;
+      for (uint32_t k = first; k < nenc; k++) {
+        L.enc_host[k].block_start = blocks;
+        blocks += (uint32_t)(((uint64_t)L.enc_host[k].tw * L.enc_host[k].th + 255) / 256);
+      }
+      if (first == 0) {
+        // After the buffers' clears at admission; a pass lane also after every preview encode
+        // launched so far (a preview is written only before a tile's first pass).
+        HIP_TRY(hipStreamWaitEvent(L.enc_stream, s->admitted, 0));
+        if (!L.previews) HIP_TRY(hipStreamWaitEvent(L.enc_stream, s->lanes[0]->encoded, 0));
+      }
+      HIP_TRY(hipMemcpyAsync(L.enc_dev + first, L.enc_host + first, (nenc - first) * sizeof(EncodeItem),
+                             hipMemcpyHostToDevice, L.enc_stream));
+      launch_encode_batch(L.enc_stream, L.enc_dev + first, nenc - first, blocks);
+      HIP_TRY(hipGetLastError());
+      launched = nenc;
+      progress = true;
     }
-    if (first == 0) {
-      // After the buffers' clears at admission; a pass lane also after every preview encode
-      // launched so far (a preview is written only before a tile's first pass).
-      HIP_TRY(hipStreamWaitEvent(L.stream, s->admitted, 0));
-      if (!L.previews) HIP_TRY(hipStreamWaitEvent(L.stream, s->lanes[0]->encoded, 0));
-    }
-    HIP_TRY(hipMemcpyAsync(L.enc_dev + first, L.enc_host + first, (nenc - first) * sizeof(EncodeItem),
-                           hipMemcpyHostToDevice, L.stream));
-    launch_encode_batch(L.stream, L.enc_dev + first, nenc - first, blocks);
-    HIP_TRY(hipGetLastError());
-    launched = nenc;
-    progress = true;
+    if (again.empty()) break;
     if (nenc == s->enc_cap) {  // the rest waits for the next call
-      L.enc_more = true;
+      L.dirty.insert(L.dirty.end(), again.begin(), again.end());
       break;
     }
+    cur.swap(again);
   }
-  if (launched) HIP_TRY(hipEventRecord(L.encoded, L.stream));
-  // The lane's later steps follow these encodes on its stream, so the ranges can be reused now.
+  if (launched) HIP_TRY(hipEventRecord(L.encoded, L.enc_stream));
+  // The lane's steps launched from now on wait for these encodes, so the ranges can be reused.
   for (auto &f : to_free) L.arena.release(f.first, f.second);
   // Status: first complete preview, completion (stamped once the writes have finished).
   bool need_sync = false;
@@ -576,6 +620,7 @@ int lane_poll(mcr_server *s, Lane &L, bool &progress) {
   if (q == hipErrorNotReady) return MCR_OK;
   if (q != hipSuccess) return mcr_fail(MCR_E_DEVICE, std::string("lane step: ") + hipGetErrorString(q));
   L.inflight = false;
+  L.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - L.launched_at).count();
   progress = true;
   MCR_TRY(engine_check(L.eng));
   for (uint32_t i = 0; i < L.launched; i++) L.table[i].done = L.done_host[i];
@@ -590,10 +635,14 @@ int lane_launch(mcr_server *s, Lane &L, bool &progress) {
   // With no path live every started sample has finished and no job in the table has started,
   // so sample numbering starts again from 0.
   uint32_t next = e.counts_host[cw(L.q)] == 0 ? 0 : std::min(e.counts_host[cw(C_NEXT)], L.total);
+  Clock::time_point t_plan = Clock::now();
   plan_window(s, L, next, e.counts_host[cw(C_FREE)]);
+  s->plan_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_plan).count();
   uint32_t n = (uint32_t)L.table.size();
   if (n == 0 && e.counts_host[cw(L.q)] == 0) return MCR_OK;
   progress = true;
+  // Arena ranges this step's jobs may reuse were freed after encodes launched so far.
+  HIP_TRY(hipStreamWaitEvent(L.stream, L.encoded, 0));
   MCR_TRY(engine_reserve(e, e.capacity_limit, (uint32_t)s->views.size(), n, true, s->cfg.max_lights));
   MCR_TRY(ensure_host(L, std::max<uint32_t>(n, 1)));
   bool mixed = s->running.size() > 1;  // samples of other views may be in flight with no job left
@@ -651,6 +700,7 @@ int lane_launch(mcr_server *s, Lane &L, bool &progress) {
   HIP_TRY(hipEventRecord(L.done, L.stream));
   L.launched = n;
   L.inflight = true;
+  L.launched_at = Clock::now();
   return MCR_OK;
 }
 
@@ -674,14 +724,20 @@ void run(mcr_server *s) {
       continue;
     }
     bool progress = false;
+    Clock::time_point t_work = Clock::now();
     int rc = housekeeping(s);
     // Each lane: take a finished step, deliver its tiles, launch its next step. The lanes run
     // side by side on their own streams; this thread only ever waits for short encodes.
     for (auto &l : s->lanes) {
       if (!rc) rc = lane_poll(s, *l, progress);
-      if (!rc && !l->inflight) rc = deliver(s, *l, progress);
-      if (!rc && !l->inflight) rc = lane_launch(s, *l, progress);
+      if (rc || l->inflight) continue;
+      collect(s, *l, progress);
+      rc = lane_launch(s, *l, progress);
+      Clock::time_point t_deliver = Clock::now();
+      if (!rc) rc = deliver(s, *l, progress);
+      s->deliver_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_deliver).count();
     }
+    s->host_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_work).count();
     bool inflight = false;
     for (auto &l : s->lanes) inflight |= l->inflight;
     auto now = Clock::now();
@@ -726,8 +782,10 @@ extern "C" {
 
 static void server_free(mcr_server *s) {
   hipSetDevice(s->ctx->device);
-  for (auto &l : s->lanes)
+  for (auto &l : s->lanes) {
     if (l->stream) hipStreamSynchronize(l->stream);
+    if (l->enc_stream) hipStreamSynchronize(l->enc_stream);
+  }
   if (s->stream) hipStreamSynchronize(s->stream);
   for (auto &kv : s->reqs) {
     if (kv.second->image) hipFreeAsync(kv.second->image, s->stream);
@@ -743,6 +801,7 @@ static void server_free(mcr_server *s) {
     if (l->done_host) hipHostFree(l->done_host);
     if (l->counters_host) hipHostFree(l->counters_host);
     if (l->done) hipEventDestroy(l->done);
+    if (l->enc_stream) hipStreamDestroy(l->enc_stream);
     if (l->stream) hipStreamDestroy(l->stream);
   }
   s->lanes.clear();
@@ -774,6 +833,7 @@ static int lane_init(mcr_server *s, Lane &L, uint32_t slots, uint32_t window, bo
     }
   }
   if (!L.stream) HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&L.enc_stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&L.encoded, hipEventDisableTiming));
   HIP_TRY(hipHostMalloc(&L.enc_host, s->enc_cap * sizeof(EncodeItem), hipHostMallocDefault));
@@ -848,6 +908,7 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
     s->lanes[1]->launches_per_step = 2;
   }
   if (rc) return fail_free(rc);
+  for (size_t i = 0; i < s->lanes.size(); i++) s->lanes[i]->bit = (uint8_t)(1u << i);
   // Tile results wait in the arena until encoded: each lane's windows plus tiles waiting for an
   // earlier pass, in one range per lane.
   std::vector<uint64_t> share;
@@ -1059,6 +1120,10 @@ int mcr_server_stats_get(mcr_server *s, mcr_server_stats *out) {
   out->requests_queued = s->admit_queue.size();
   out->device_ms = s->device_ms;
   out->uptime_ms = s->now();
+  out->host_ms = s->host_ms;
+  out->plan_ms = s->plan_ms;
+  out->deliver_ms = s->deliver_ms;
+  out->pass_busy_ms = s->lanes.back()->busy_ms;
   return MCR_OK;
 }
 

@@ -301,13 +301,14 @@ def batched_equivalent(ctx, scenes, specs, slots=1 << 21, log=print):
             'mixed_view_wave_frac': round(cnt.mixed_view_waves / max(1, cnt.shade_waves), 4)}
 
 
-def autotune_served(ctx, scenes, target_ms=50.0, rate=8.0, seconds=3.0, margin=0.9,
-                    candidates=(1 << 23, 1 << 22, 1 << 21), preview_cus=(24, 32), log=print):
+def autotune_served(ctx, scenes, target_ms=50.0, rate=16.0, seconds=3.0, margin=0.9,
+                    candidates=(1 << 23, 1 << 22), preview_cus=(8, 16, 24), log=print):
     """Pass-lane path slots and preview-lane CUs for the served form: a short run of the mix for
-    each pair; the fastest (Mrays/s) whose p99 time-to-first-preview stays under margin * target_ms
-    wins (the preview lane keeps previews off the pass lane's iterations, so more pass slots buy
-    throughput; more preview CUs shorten previews and take CUs from the passes; the margin covers
-    the spread between a short trial and a long run)."""
+    each pair, at an arrival rate that keeps the pass lane saturated; the fastest (Mrays/s) whose
+    p99 time-to-first-preview stays under margin * target_ms wins (the preview lane keeps previews
+    off the pass lane's iterations, so more pass slots buy throughput; more preview CUs shorten
+    previews and take CUs from the passes; the margin covers the spread between a short trial and
+    a long run)."""
     trials = []
     for slots in candidates:
         for cus in preview_cus:
@@ -438,7 +439,11 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
                 'mrays_s': round(rays / (wall / 1e3) / 1e6, 2),
                 'mrays_s_arrivals': round(rays_mid / (wall_mid / 1e3) / 1e6, 2),
                 'mrays_s_device_busy': round(rays / (device / 1e3) / 1e6, 2), 'wall_seconds': round(t_end, 2),
-                'paths': int(stats1['paths'] - stats0['paths'])}
+                'paths': int(stats1['paths'] - stats0['paths']),
+                # the server thread's own work and the pass lane's time with a step in flight, as
+                # fractions of the wall time
+                'server_thread': {k: round((stats1[k] - stats0[k]) / wall, 4)
+                                  for k in ('host_ms', 'plan_ms', 'deliver_ms', 'pass_busy_ms')}}
     finally:
         srv.close()
         if own:
