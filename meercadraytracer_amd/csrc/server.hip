@@ -74,6 +74,10 @@ struct SrvRequest {
   double t_submit = -1, t_admit = -1, t_preview = -1, t_done = -1;
   uint64_t paths = 0;
   uint64_t it_admit = 0, it_preview = 0;  // bounce iterations of the preview lane so far
+  // The first complete preview and the completion are stamped once the encodes that wrote them have
+  // finished: the encode launches (by lane) they wait for, set when the host staged the last write.
+  bool preview_staged = false, done_staged = false;
+  uint64_t preview_fence[2] = {0, 0}, done_fence[2] = {0, 0};
 };
 
 struct SrvJob {
@@ -167,8 +171,17 @@ struct Lane {
   // the lane's later steps, which wait for the encode.
   Arena arena;
   EncodeItem *enc_host = nullptr, *enc_dev = nullptr;  // staging of the encode items
-  hipStream_t enc_stream = nullptr;  // the lane's encodes
+  hipStream_t enc_stream = nullptr;  // the lane's encodes, on the lane's CUs
   hipEvent_t encoded = nullptr;    // recorded after every encode launch of the lane
+  // One fence per deliver call that launches encodes, polled (never waited for) to learn which
+  // writes have finished: fences [1, enc_done] have completed, `fences` holds the rest in order.
+  uint64_t enc_seq = 0, enc_done = 0;
+  std::deque<std::pair<uint64_t, hipEvent_t>> fences;
+  std::vector<hipEvent_t> spare_fences;
+  // The encode items are staged in two halves used by turns; a half is rewritten once the fence of
+  // its last use has completed (two deliver calls back, so normally without a wait).
+  uint32_t half = 0;
+  uint64_t half_fence[2] = {0, 0};
   std::vector<TileRef> dirty;      // tiles holding a result of this lane that waits for its encode
   uint8_t bit = 1;                 // this lane's bit in SrvTile::listed
 };
@@ -471,6 +484,28 @@ void collect(mcr_server *s, Lane &L, bool &progress) {
   L.table.swap(still);
 }
 
+// Takes the lane's completed encode fences (no wait).
+int fences_poll(Lane &L) {
+  while (!L.fences.empty()) {
+    hipError_t q = hipEventQuery(L.fences.front().second);
+    if (q == hipErrorNotReady) break;
+    if (q != hipSuccess) return mcr_fail(MCR_E_DEVICE, std::string("encode: ") + hipGetErrorString(q));
+    L.enc_done = L.fences.front().first;
+    L.spare_fences.push_back(L.fences.front().second);
+    L.fences.pop_front();
+  }
+  return MCR_OK;
+}
+
+// Waits until the lane's fence `seq` (0: none) has completed.
+int fence_wait(Lane &L, uint64_t seq) {
+  MCR_TRY(fences_poll(L));
+  if (seq <= L.enc_done) return MCR_OK;
+  for (auto &f : L.fences)
+    if (f.first == seq) HIP_TRY(hipEventSynchronize(f.second));
+  return fences_poll(L);
+}
+
 // L's tile results are written in pass order by encodes on the lane's encode stream, which run
 // beside the lane's next step (launched first, so the device does not wait for this host work).
 // The step whose results these are has been seen complete by the host, so the encodes need no
@@ -483,8 +518,10 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
   // while it may hold another item for a later round.
   uint32_t nenc = 0, launched = 0;
   std::vector<std::pair<uint64_t, uint64_t>> to_free;  // arena ranges, freed after the launches
-  // The staging of the lane's last encodes may still be in its copy to the device.
-  HIP_TRY(hipEventSynchronize(L.encoded));
+  // This call stages into half L.half, whose last encodes (and their copy) must have finished.
+  MCR_TRY(fence_wait(L, L.half_fence[L.half]));
+  const uint32_t cap = s->enc_cap / 2;
+  EncodeItem *enc_host = L.enc_host + (size_t)L.half * cap, *enc_dev = L.enc_dev + (size_t)L.half * cap;
   std::vector<TileRef> cur, again;
   cur.swap(L.dirty);
   for (;;) {
@@ -499,7 +536,7 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
         t.listed &= (uint8_t)~L.bit;
         continue;
       }
-      if (nenc == s->enc_cap) {  // waits for the next call
+      if (nenc == cap) {  // waits for the next call
         L.dirty.push_back(d);
         continue;
       }
@@ -531,7 +568,7 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
         continue;
       }
       bool wide = r.r.samples > WIDE_SUM_SAMPLES;
-      L.enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
+      enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
                                       preview || !wide ? nullptr : reinterpret_cast<double *>(r.sum),
                                       s->arena_dev + data, r.r.width,
                                       t.t.x, t.t.y, t.t.width, t.t.height, preview ? 1u : t.samples,
@@ -544,8 +581,8 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
     if (nenc > first) {
       uint32_t blocks = 0;
       for (uint32_t k = first; k < nenc; k++) {
-        L.enc_host[k].block_start = blocks;
-        blocks += (uint32_t)(((uint64_t)L.enc_host[k].tw * L.enc_host[k].th + 255) / 256);
+        enc_host[k].block_start = blocks;
+        blocks += (uint32_t)(((uint64_t)enc_host[k].tw * enc_host[k].th + 255) / 256);
       }
       if (first == 0) {
         // After the buffers' clears at admission; a pass lane also after every preview encode
@@ -553,50 +590,80 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
         HIP_TRY(hipStreamWaitEvent(L.enc_stream, s->admitted, 0));
         if (!L.previews) HIP_TRY(hipStreamWaitEvent(L.enc_stream, s->lanes[0]->encoded, 0));
       }
-      HIP_TRY(hipMemcpyAsync(L.enc_dev + first, L.enc_host + first, (nenc - first) * sizeof(EncodeItem),
+      HIP_TRY(hipMemcpyAsync(enc_dev + first, enc_host + first, (nenc - first) * sizeof(EncodeItem),
                              hipMemcpyHostToDevice, L.enc_stream));
-      launch_encode_batch(L.enc_stream, L.enc_dev + first, nenc - first, blocks);
+      launch_encode_batch(L.enc_stream, enc_dev + first, nenc - first, blocks);
       HIP_TRY(hipGetLastError());
       launched = nenc;
       progress = true;
     }
     if (again.empty()) break;
-    if (nenc == s->enc_cap) {  // the rest waits for the next call
+    if (nenc == cap) {  // the rest waits for the next call
       L.dirty.insert(L.dirty.end(), again.begin(), again.end());
       break;
     }
     cur.swap(again);
   }
-  if (launched) HIP_TRY(hipEventRecord(L.encoded, L.enc_stream));
+  if (launched) {
+    HIP_TRY(hipEventRecord(L.encoded, L.enc_stream));
+    hipEvent_t ev = nullptr;
+    if (!L.spare_fences.empty()) {
+      ev = L.spare_fences.back();
+      L.spare_fences.pop_back();
+    } else {
+      HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(ev, L.enc_stream));
+    L.fences.emplace_back(++L.enc_seq, ev);
+    L.half_fence[L.half] = L.enc_seq;
+    L.half ^= 1;
+  }
   // The lane's steps launched from now on wait for these encodes, so the ranges can be reused.
   for (auto &f : to_free) L.arena.release(f.first, f.second);
-  // Status: first complete preview, completion (stamped once the writes have finished).
-  bool need_sync = false;
+  // A request whose last preview tile or last pass has been staged waits for the encodes launched
+  // so far on every lane; stamp() marks it once they have finished.
   for (uint64_t id : s->running) {
     SrvRequest &r = *s->reqs[id];
     if (r.state != MCR_REQ_RUNNING) continue;
-    if ((r.t_preview < 0 && r.covered == r.tiles.size()) || r.finished_tiles == r.tiles.size()) need_sync = true;
+    if (!r.preview_staged && r.covered == r.tiles.size()) {
+      r.preview_staged = true;
+      for (size_t k = 0; k < s->lanes.size(); k++) r.preview_fence[k] = s->lanes[k]->enc_seq;
+    }
+    if (!r.done_staged && r.finished_tiles == r.tiles.size() && r.pending_pass.empty() && r.pending_preview.empty()) {
+      r.done_staged = true;
+      for (size_t k = 0; k < s->lanes.size(); k++) r.done_fence[k] = s->lanes[k]->enc_seq;
+    }
   }
-  if (need_sync)
-    for (auto &l : s->lanes) HIP_TRY(hipEventSynchronize(l->encoded));
+  return MCR_OK;
+}
+
+// Stamps the first complete preview and the completion of requests whose encodes have finished.
+// Sets progress if anything changed.
+void stamp(mcr_server *s, bool &progress) {
+  auto finished = [&](const uint64_t *fence) {
+    for (size_t k = 0; k < s->lanes.size(); k++)
+      if (s->lanes[k]->enc_done < fence[k]) return false;
+    return true;
+  };
   double now = s->now();
   std::vector<uint64_t> ids(s->running);
   for (uint64_t id : ids) {
     SrvRequest &r = *s->reqs[id];
     if (r.state != MCR_REQ_RUNNING) continue;
-    if (r.t_preview < 0 && r.covered == r.tiles.size()) {
+    if (r.t_preview < 0 && r.preview_staged && finished(r.preview_fence)) {
       r.t_preview = now;
       r.it_preview = s->preview_iterations();
       s->changed.notify_all();
+      progress = true;
     }
-    if (r.finished_tiles == r.tiles.size() && r.pending_pass.empty() && r.pending_preview.empty()) {
+    if (r.done_staged && finished(r.done_fence)) {
       r.state = MCR_REQ_DONE;
       r.t_done = now;
       retire(s, r);
       s->changed.notify_all();
+      progress = true;
     }
   }
-  return MCR_OK;
 }
 
 int ensure_host(Lane &L, uint32_t n) {
@@ -726,8 +793,8 @@ void run(mcr_server *s) {
     bool progress = false;
     Clock::time_point t_work = Clock::now();
     int rc = housekeeping(s);
-    // Each lane: take a finished step, deliver its tiles, launch its next step. The lanes run
-    // side by side on their own streams; this thread only ever waits for short encodes.
+    // Each lane: take a finished step, launch its next step, encode the finished tiles. The lanes
+    // run side by side on their own streams; this thread polls and never waits for a step.
     for (auto &l : s->lanes) {
       if (!rc) rc = lane_poll(s, *l, progress);
       if (rc || l->inflight) continue;
@@ -737,6 +804,9 @@ void run(mcr_server *s) {
       if (!rc) rc = deliver(s, *l, progress);
       s->deliver_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_deliver).count();
     }
+    for (auto &l : s->lanes)
+      if (!rc) rc = fences_poll(*l);
+    if (!rc) stamp(s, progress);
     s->host_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_work).count();
     bool inflight = false;
     for (auto &l : s->lanes) inflight |= l->inflight;
@@ -763,8 +833,10 @@ void run(mcr_server *s) {
       continue;
     }
     if (progress) continue;
-    if (inflight) {
-      // Steps in flight: look again shortly (API calls take the lock meanwhile).
+    bool encoding = false;
+    for (auto &l : s->lanes) encoding |= !l->fences.empty();
+    if (inflight || encoding) {
+      // Steps or encodes in flight: look again shortly (API calls take the lock meanwhile).
       lk.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(40));
       lk.lock();
@@ -802,6 +874,8 @@ static void server_free(mcr_server *s) {
     if (l->counters_host) hipHostFree(l->counters_host);
     if (l->done) hipEventDestroy(l->done);
     if (l->enc_stream) hipStreamDestroy(l->enc_stream);
+    for (auto &f : l->fences) hipEventDestroy(f.second);
+    for (hipEvent_t ev : l->spare_fences) hipEventDestroy(ev);
     if (l->stream) hipStreamDestroy(l->stream);
   }
   s->lanes.clear();
@@ -827,13 +901,16 @@ static int lane_init(mcr_server *s, Lane &L, uint32_t slots, uint32_t window, bo
     uint32_t words = (uint32_t)(cus + 31) / 32;
     std::vector<uint32_t> mask(words, 0u);
     for (int k = cu_first; k < cu_first + cu_count && k < cus; k++) mask[(size_t)k / 32] |= 1u << (k % 32);
-    if (hipExtStreamCreateWithCUMask(&L.stream, words, mask.data()) != hipSuccess) {
+    if (hipExtStreamCreateWithCUMask(&L.stream, words, mask.data()) != hipSuccess ||
+        hipExtStreamCreateWithCUMask(&L.enc_stream, words, mask.data()) != hipSuccess) {
       (void)hipGetLastError();
-      L.stream = nullptr;
+      if (L.stream) hipStreamDestroy(L.stream);
+      if (L.enc_stream) hipStreamDestroy(L.enc_stream);
+      L.stream = L.enc_stream = nullptr;
     }
   }
   if (!L.stream) HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
-  HIP_TRY(hipStreamCreateWithFlags(&L.enc_stream, hipStreamNonBlocking));
+  if (!L.enc_stream) HIP_TRY(hipStreamCreateWithFlags(&L.enc_stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&L.encoded, hipEventDisableTiming));
   HIP_TRY(hipHostMalloc(&L.enc_host, s->enc_cap * sizeof(EncodeItem), hipHostMallocDefault));

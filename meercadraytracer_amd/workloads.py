@@ -301,7 +301,7 @@ def batched_equivalent(ctx, scenes, specs, slots=1 << 21, log=print):
             'mixed_view_wave_frac': round(cnt.mixed_view_waves / max(1, cnt.shade_waves), 4)}
 
 
-def autotune_served(ctx, scenes, target_ms=50.0, rate=16.0, seconds=3.0, margin=0.9,
+def autotune_served(ctx, scenes, target_ms=50.0, rate=16.0, seconds=5.0, margin=0.9,
                     candidates=(1 << 23, 1 << 22), preview_cus=(8, 16, 24), log=print):
     """Pass-lane path slots and preview-lane CUs for the served form: a short run of the mix for
     each pair, at an arrival rate that keeps the pass lane saturated; the fastest (Mrays/s) whose
