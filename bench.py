@@ -295,15 +295,15 @@ def served(args, ctx, workloads):
     scenes = workloads.resident_scenes(ctx, 2, 32, 50_000, 4_000_000, log)
     try:
         tuned, trials = None, []
-        slots, cus = args.served_slots, 0
+        slots, cus, pp = args.served_slots, 0, False
         if not slots:
             tuned, trials = workloads.autotune_served(ctx, scenes, log=log)
-            slots, cus = tuned['slots'], tuned['preview_cus']
+            slots, cus, pp = tuned['slots'], tuned['preview_cus'], tuned['preview_passes']
         sweep = []
         for rate in [float(r) for r in args.served_rates.split(',') if r]:
             log(f'served mix at {rate}/s for {args.served_seconds} s, {slots} pass slots ...')
             r = workloads.served_mix(ctx, rate, args.served_seconds, scenes=scenes, slots=slots, preview_cus=cus,
-                                     drain_seconds=10.0, log=log)
+                                     preview_passes=pp, drain_seconds=10.0, log=log)
             specs = r.pop('completed_specs')
             log(json.dumps(r))
             sweep.append(r)
@@ -323,7 +323,7 @@ def served(args, ctx, workloads):
             'triangles': int(sum(n for _, n, _ in scenes)),
             'ttfp_ms_p50': top['ttfp_ms_p50'], 'ttfp_ms_p99': top['ttfp_ms_p99'],
             'saturated_mrays_s': top['mrays_s'], 'mrays_s_during_arrivals': top['mrays_s_arrivals'],
-            'slots': slots, 'preview_cus': cus, 'autotune': trials,
+            'slots': slots, 'preview_cus': cus, 'preview_passes': pp, 'autotune': trials,
             'batched_same_scenes': same, 'vs_batched_same_scenes': round(top['mrays_s'] / same['mrays_s'], 3),
             'batched_best_slots': best, 'vs_batched_best_slots': round(top['mrays_s'] / best['mrays_s'], 3),
             'sweep': sweep}

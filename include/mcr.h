@@ -286,6 +286,8 @@ typedef struct mcr_server_config {
                                of `slots` slots on the other CUs, each on its own stream */
   uint32_t preview_slots;   /* the preview lane's path slots; 0 = 2^18 */
   uint32_t preview_cus;     /* CUs kept for the preview lane; 0 = 3/32 of the device's */
+  int32_t preview_passes;   /* beside a pass lane: 1 = the preview lane also runs passes (at most an eighth
+                               of its window) while no preview waits; 0 = previews only */
 } mcr_server_config;
 typedef struct mcr_request {
   mcr_scene *scene;         /* added with mcr_server_add_scene */

@@ -191,7 +191,8 @@ class WideInfo(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [('hbm_budget', C.c_uint64), ('slots', C.c_uint32), ('window', C.c_uint32),
                 ('max_requests', C.c_uint32), ('use_graph', C.c_int32), ('max_lights', C.c_uint32),
-                ('lanes', C.c_uint32), ('preview_slots', C.c_uint32), ('preview_cus', C.c_uint32)]
+                ('lanes', C.c_uint32), ('preview_slots', C.c_uint32), ('preview_cus', C.c_uint32),
+                ('preview_passes', C.c_int32)]
 
 
 class Request(C.Structure):

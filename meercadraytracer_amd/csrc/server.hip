@@ -397,7 +397,7 @@ void plan_window(mcr_server *s, Lane &L, uint32_t next, uint32_t idle) {
       continue;
     }
     if (r.state == MCR_REQ_RUNNING) returned[j.req].push_back(j.item);
-    L.arena.release(j.arena, j.floats);
+    result_lane(s, j.item.block > 1).arena.release(j.arena, j.floats);
     r.jobs_live--;
   }
   for (auto &kv : returned) {
@@ -416,7 +416,7 @@ void plan_window(mcr_server *s, Lane &L, uint32_t next, uint32_t idle) {
     // Sample indices stay below 2^31; past that the window drains and numbering restarts.
     if ((uint64_t)start + (uint64_t)t.width * t.height > (1ull << 31)) return false;
     uint64_t floats = (uint64_t)t.width * t.height * 4;
-    uint64_t off = L.arena.alloc(floats);
+    uint64_t off = result_lane(s, it.block > 1).arena.alloc(floats);
     if (off == UINT64_MAX) return false;
     mcr_tile_job tj{t, r.r.width, r.r.height, it.pass, it.block, off};
     SrvJob j;
@@ -443,14 +443,20 @@ void plan_window(mcr_server *s, Lane &L, uint32_t next, uint32_t idle) {
     }
   // Then the passes of the oldest request first: the wavefront keeps working on few scenes at a
   // time (their BVHs stay in cache), and requests finish in arrival order.
+  // A preview lane beside a pass lane fills only an eighth of its window with passes: a preview's
+  // latency grows with the paths its lane carries, so passes there only take up CUs that would
+  // otherwise idle between previews.
+  uint64_t pass_target = target;
+  if (L.previews && s->lanes.size() > 1)
+    pass_target = std::min<uint64_t>(target, (uint64_t)(L.window / 8) * L.iterations_per_step());
   if (L.passes)
     for (uint64_t id : s->running) {
       SrvRequest &r = *s->reqs[id];
       // One lane: a request's passes follow its preview (progressive.ts:207-210).
       if (L.previews && !r.pending_preview.empty()) continue;
-      while (room && queued < target && r.state == MCR_REQ_RUNNING && !r.pending_pass.empty())
+      while (room && queued < pass_target && r.state == MCR_REQ_RUNNING && !r.pending_pass.empty())
         room = place(r, r.pending_pass);
-      if (!room || queued >= target) break;
+      if (!room || queued >= pass_target) break;
     }
   L.total = start;
 }
@@ -469,16 +475,17 @@ void collect(mcr_server *s, Lane &L, bool &progress) {
     r.paths += j.size;
     s->paths += j.size;
     r.jobs_live--;
+    Lane &E = result_lane(s, j.item.block > 1);  // holds the result and encodes it
     SrvTile *t = r.state == MCR_REQ_RUNNING ? &r.tiles[j.item.tile] : nullptr;
     if (!t) {
-      L.arena.release(j.arena, j.floats);
+      E.arena.release(j.arena, j.floats);
       continue;
     }
     if (j.item.block > 1) t->preview = j.arena;
     else t->ready[j.item.pass] = j.arena;
-    if (!(t->listed & L.bit)) {
-      t->listed |= L.bit;
-      L.dirty.push_back(TileRef{r.id, j.item.tile});
+    if (!(t->listed & E.bit)) {
+      t->listed |= E.bit;
+      E.dirty.push_back(TileRef{r.id, j.item.tile});
     }
   }
   L.table.swap(still);
@@ -551,7 +558,7 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
         again.push_back(d);  // its passes, in a later round
         if (t.samples != 0 || t.ready.count(0)) continue;
         preview = true;
-      } else if (L.passes) {
+      } else if (&L == &result_lane(s, false)) {
         auto it = t.ready.find(t.samples);
         if (it == t.ready.end()) {
           t.listed &= (uint8_t)~L.bit;
@@ -708,8 +715,9 @@ int lane_launch(mcr_server *s, Lane &L, bool &progress) {
   uint32_t n = (uint32_t)L.table.size();
   if (n == 0 && e.counts_host[cw(L.q)] == 0) return MCR_OK;
   progress = true;
-  // Arena ranges this step's jobs may reuse were freed after encodes launched so far.
-  HIP_TRY(hipStreamWaitEvent(L.stream, L.encoded, 0));
+  // Arena ranges this step's jobs may reuse were freed after encodes launched so far (by any
+  // lane: the preview lane's pass results live in the pass lane's arena).
+  for (auto &l : s->lanes) HIP_TRY(hipStreamWaitEvent(L.stream, l->encoded, 0));
   MCR_TRY(engine_reserve(e, e.capacity_limit, (uint32_t)s->views.size(), n, true, s->cfg.max_lights));
   MCR_TRY(ensure_host(L, std::max<uint32_t>(n, 1)));
   bool mixed = s->running.size() > 1;  // samples of other views may be in flight with no job left
@@ -980,7 +988,10 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
     int pc = (int)s->cfg.preview_cus;
     s->lanes.emplace_back(new Lane());
     s->lanes.emplace_back(new Lane());
-    rc = lane_init(s.get(), *s->lanes[0], s->cfg.preview_slots, s->cfg.preview_slots, true, false, cus - pc, pc, cus);
+    // With preview_passes the preview lane also runs passes while no preview waits (its window
+    // puts previews first), so its CUs do not idle between previews.
+    rc = lane_init(s.get(), *s->lanes[0], s->cfg.preview_slots, s->cfg.preview_slots, true,
+                   s->cfg.preview_passes > 0, cus - pc, pc, cus);
     if (!rc) rc = lane_init(s.get(), *s->lanes[1], s->cfg.slots, s->cfg.window, false, true, 0, cus - pc, cus);
     s->lanes[1]->launches_per_step = 2;
   }

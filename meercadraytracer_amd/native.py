@@ -435,10 +435,11 @@ class Server:
     one progressive render (camera, settings, size, samples, tile size, preview block)."""
 
     def __init__(self, ctx=None, hbm_budget=0, slots=0, window=0, max_requests=0, use_graph=True, max_lights=0,
-                 lanes=0, preview_slots=0, preview_cus=0):
+                 lanes=0, preview_slots=0, preview_cus=0, preview_passes=False):
         self.ctx = ctx or default_context()
         cfg = abi.ServerConfig(int(hbm_budget), int(slots), int(window), int(max_requests), int(bool(use_graph)),
-                               int(max_lights), int(lanes), int(preview_slots), int(preview_cus))
+                               int(max_lights), int(lanes), int(preview_slots), int(preview_cus),
+                               1 if preview_passes else 0)
         h = C.c_void_p()
         check(lib().mcr_server_create(self.ctx.handle, C.byref(cfg), C.byref(h)))
         self.handle = h.value

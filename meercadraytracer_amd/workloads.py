@@ -301,21 +301,22 @@ def batched_equivalent(ctx, scenes, specs, slots=1 << 21, log=print):
             'mixed_view_wave_frac': round(cnt.mixed_view_waves / max(1, cnt.shade_waves), 4)}
 
 
-def autotune_served(ctx, scenes, target_ms=50.0, rate=16.0, seconds=5.0, margin=0.9,
-                    candidates=(1 << 23, 1 << 22), preview_cus=(8, 16, 24), log=print):
-    """Pass-lane path slots and preview-lane CUs for the served form: a short run of the mix for
-    each pair, at an arrival rate that keeps the pass lane saturated; the fastest (Mrays/s) whose
-    p99 time-to-first-preview stays under margin * target_ms wins (the preview lane keeps previews
-    off the pass lane's iterations, so more pass slots buy throughput; more preview CUs shorten
-    previews and take CUs from the passes; the margin covers the spread between a short trial and
-    a long run)."""
+def autotune_served(ctx, scenes, target_ms=50.0, rate=16.0, seconds=5.0, margin=0.9, slots=1 << 23,
+                    preview_cus=(8, 16, 24), preview_passes=(False, True), log=print):
+    """Preview-lane CUs, and whether the preview lane also runs passes, for the served form: a
+    short run of the mix for each pair at an arrival rate that keeps the pass lane saturated (its
+    `slots`: more pass slots only buy throughput, since previews run on their own lane); the
+    fastest (Mrays/s) whose p99 time-to-first-preview stays under margin * target_ms wins. More
+    preview CUs shorten previews and take CUs from the passes; passes on the preview lane use those
+    CUs between previews and lengthen previews. The margin covers the spread between a short trial
+    and a long run."""
     trials = []
-    for slots in candidates:
-        for cus in preview_cus:
+    for cus in preview_cus:
+        for pp in preview_passes:
             r = served_mix(ctx, rate, seconds, seed=7, scenes=scenes, slots=slots, preview_cus=cus,
-                           drain_seconds=2.0, log=lambda *_: None)
-            trials.append({'slots': slots, 'preview_cus': cus, 'ttfp_ms_p99': r['ttfp_ms_p99'],
-                           'mrays_s': r['mrays_s']})
+                           preview_passes=pp, drain_seconds=2.0, log=lambda *_: None)
+            trials.append({'slots': slots, 'preview_cus': cus, 'preview_passes': pp,
+                           'ttfp_ms_p99': r['ttfp_ms_p99'], 'mrays_s': r['mrays_s']})
             log(f'served autotune: {trials[-1]}')
     ok = [t for t in trials if t['ttfp_ms_p99'] is not None and t['ttfp_ms_p99'] <= margin * target_ms]
     best = max(ok or trials, key=lambda t: t['mrays_s'])
@@ -325,7 +326,7 @@ def autotune_served(ctx, scenes, target_ms=50.0, rate=16.0, seconds=5.0, margin=
 # ---- served form ---------------------------------------------------------------------------------
 def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_frac=0.3,
                budget=200 << 30, slots=0, window=0, max_requests=1024, drain_seconds=60.0, lanes=0,
-               preview_slots=0, preview_cus=0, warmup=True, log=print):
+               preview_slots=0, preview_cus=0, preview_passes=False, warmup=True, log=print):
     """Runs the arrival process for `seconds`, waits up to `drain_seconds` for the admitted
     requests, cancels what is left, and returns the latency and throughput figures."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -333,7 +334,8 @@ def served_mix(ctx, rate, seconds, seed=2, n_scenes=32, scenes=None, restart_fra
     if own:
         scenes = resident_scenes(ctx, seed, n_scenes, 50_000, 4_000_000, log)
     srv = native.Server(ctx, hbm_budget=budget, slots=slots, window=window, max_requests=max_requests,
-                        lanes=lanes, preview_slots=preview_slots, preview_cus=preview_cus)
+                        lanes=lanes, preview_slots=preview_slots, preview_cus=preview_cus,
+                        preview_passes=preview_passes)
     try:
         for gpu, _, _ in scenes:
             srv.add_scene(gpu)

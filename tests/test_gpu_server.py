@@ -85,12 +85,13 @@ def test_alone_equals_raytrace(world, graph, lanes):
         srv.close()
 
 
-@pytest.mark.parametrize('lanes', [2, 1])
-def test_mix_equals_alone(world, lanes):
+@pytest.mark.parametrize('lanes,preview_passes', [(2, False), (2, True), (1, False)])
+def test_mix_equals_alone(world, lanes, preview_passes):
     """Five requests at once, then three more while they run; slots small enough that every
     wave mixes requests. Two lanes: the previews run on their own lane and CUs beside the
-    passes."""
-    srv = _server(world, slots=512, window=1024, lanes=lanes, preview_slots=256)
+    passes; with preview_passes the preview lane also runs passes, whose results the pass lane
+    encodes in pass order with its own."""
+    srv = _server(world, slots=512, window=1024, lanes=lanes, preview_slots=256, preview_passes=preview_passes)
     try:
         ids = [(_submit(srv, world, spec), spec) for spec in REQUESTS]
         time.sleep(0.01)
