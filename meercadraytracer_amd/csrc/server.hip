@@ -167,8 +167,8 @@ struct Lane {
   void *counters_host = nullptr;   // pinned: the device counter copies
   DevCounters counters = {};
   // The lane's tile results wait in its own range of the server's arena until encoded, and the
-  // lane encodes them on its own encode stream: a range freed after an encode is reused only by
-  // the lane's later steps, which wait for the encode.
+  // lane encodes them on its own encode stream: a range read by an encode returns to the allocator
+  // once the host has seen that encode's fence complete, so no step waits for encodes.
   Arena arena;
   EncodeItem *enc_host = nullptr, *enc_dev = nullptr;  // staging of the encode items
   hipStream_t enc_stream = nullptr;  // the lane's encodes, on the lane's CUs
@@ -182,6 +182,11 @@ struct Lane {
   // its last use has completed (two deliver calls back, so normally without a wait).
   uint32_t half = 0;
   uint64_t half_fence[2] = {0, 0};
+  // Arena ranges read by encodes of this lane, freed once the fence `seq` has completed.
+  struct Deferred {
+    uint64_t seq, off, len;
+  };
+  std::deque<Deferred> deferred;
   std::vector<TileRef> dirty;      // tiles holding a result of this lane that waits for its encode
   uint8_t bit = 1;                 // this lane's bit in SrvTile::listed
 };
@@ -501,6 +506,11 @@ int fences_poll(Lane &L) {
     L.spare_fences.push_back(L.fences.front().second);
     L.fences.pop_front();
   }
+  // Arena ranges whose encodes have finished are free on the device: back to the allocator.
+  while (!L.deferred.empty() && L.deferred.front().seq <= L.enc_done) {
+    L.arena.release(L.deferred.front().off, L.deferred.front().len);
+    L.deferred.pop_front();
+  }
   return MCR_OK;
 }
 
@@ -516,8 +526,8 @@ int fence_wait(Lane &L, uint64_t seq) {
 // L's tile results are written in pass order by encodes on the lane's encode stream, which run
 // beside the lane's next step (launched first, so the device does not wait for this host work).
 // The step whose results these are has been seen complete by the host, so the encodes need no
-// wait for it; the lane's later steps wait for the encodes before they reuse the arena ranges
-// freed here. Sets progress if anything moved.
+// wait for it; the arena ranges they read return to the allocator once their fence has
+// completed. Sets progress if anything moved.
 int deliver(mcr_server *s, Lane &L, bool &progress) {
   if (L.dirty.empty()) return MCR_OK;  // no tile result of the lane waits for its encode
   // Rounds of one launch each over the tiles holding a result of this lane; a tile is staged at
@@ -625,8 +635,9 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
     L.half_fence[L.half] = L.enc_seq;
     L.half ^= 1;
   }
-  // The lane's steps launched from now on wait for these encodes, so the ranges can be reused.
-  for (auto &f : to_free) L.arena.release(f.first, f.second);
+  // The ranges go back to the allocator once this call's fence has completed (fences_poll): a step
+  // of any lane may reuse them then without waiting for the encodes on the device.
+  for (auto &f : to_free) L.deferred.push_back(Lane::Deferred{L.enc_seq, f.first, f.second});
   // A request whose last preview tile or last pass has been staged waits for the encodes launched
   // so far on every lane; stamp() marks it once they have finished.
   for (uint64_t id : s->running) {
@@ -715,9 +726,6 @@ int lane_launch(mcr_server *s, Lane &L, bool &progress) {
   uint32_t n = (uint32_t)L.table.size();
   if (n == 0 && e.counts_host[cw(L.q)] == 0) return MCR_OK;
   progress = true;
-  // Arena ranges this step's jobs may reuse were freed after encodes launched so far (by any
-  // lane: the preview lane's pass results live in the pass lane's arena).
-  for (auto &l : s->lanes) HIP_TRY(hipStreamWaitEvent(L.stream, l->encoded, 0));
   MCR_TRY(engine_reserve(e, e.capacity_limit, (uint32_t)s->views.size(), n, true, s->cfg.max_lights));
   MCR_TRY(ensure_host(L, std::max<uint32_t>(n, 1)));
   bool mixed = s->running.size() > 1;  // samples of other views may be in flight with no job left
