@@ -174,13 +174,6 @@ __global__ __launch_bounds__(256) void k_node_bounds_wide(BNode *nodes, int32_t 
   }
 }
 
-__global__ void k_bins_init(uint32_t *bins, uint32_t words) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= words) return;
-  int f = (i % NB);
-  bins[i] = f == 0 ? 0u : ((f - 1) / 3 % 2 == 0 ? KEY_POS_INF : KEY_NEG_INF);
-}
-
 struct NodeGeom {  // per-node binning parameters (bvh.ts:90-92)
   double lo[3], scale[3];
   bool valid[3];
@@ -946,6 +939,30 @@ __global__ __launch_bounds__(1024) void k_scan_one(const uint32_t *in, uint32_t 
   if (threadIdx.x == 0 && total) *total = tot;
 }
 
+// k_chunk_count, the scan of its counts and the clearing of `zero_n` counter words, in one block
+// for levels of at most SCAN_ONE_MAX active nodes (k_scan_one's arithmetic).
+__global__ __launch_bounds__(1024) void k_chunk_scan(const int32_t *active, uint32_t n, const BNode *nodes,
+                                                     uint32_t *cnt, uint32_t *out, uint32_t *total,
+                                                     uint32_t *zero, uint32_t zero_n) {
+  if (threadIdx.x < zero_n) zero[threadIdx.x] = 0;
+  const uint32_t per = (n + 1023) / 1024, start = threadIdx.x * per;
+  uint32_t run = 0;
+  for (uint32_t k = 0; k < per; k++)
+    if (start + k < n) {
+      uint32_t c = (uint32_t)((nodes[active[start + k]].n + CHUNK - 1) / CHUNK);
+      cnt[start + k] = c;
+      run += c;
+    }
+  uint32_t tot, acc = tile_scan_1024(run, tot);
+  for (uint32_t k = 0; k < per; k++)
+    if (start + k < n) {
+      uint32_t v = cnt[start + k];
+      out[start + k] = acc;
+      acc += v;
+    }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
 __global__ void k_scan_add(uint32_t *out, uint32_t n, const uint32_t *tile_sums) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] += tile_sums[i / 1024];
@@ -953,8 +970,12 @@ __global__ void k_scan_add(uint32_t *out, uint32_t n, const uint32_t *tile_sums)
 
 // Chunk c of the level: its node is the last one whose first chunk is <= c.
 __global__ void k_chunk_emit(const int32_t *active, uint32_t nact, BNode *nodes, const uint32_t *cnt,
-                             const uint32_t *off, uint32_t nchunks, int4 *chunks) {
+                             const uint32_t *off, uint32_t nchunks, int4 *chunks, uint32_t *bins, uint32_t words) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < words) {  // the level's bins: zero counts, empty bounds
+    int f = (c % NB);
+    bins[c] = f == 0 ? 0u : ((f - 1) / 3 % 2 == 0 ? KEY_POS_INF : KEY_NEG_INF);
+  }
   if (c >= nchunks) return;
   int lo = 0, hi = (int)nact - 1;
   while (lo < hi) {
@@ -1294,22 +1315,26 @@ int builder_run(BuilderWorkspace &ws, hipStream_t sm, const float *tris, uint32_
       return MCR_E_OVERFLOW;
     }
     int32_t *active = L.active[act], *next = L.active[act ^ 1];
-    // chunks of the level's nodes
-    hipLaunchKernelGGL(k_chunk_count, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_cnt);
-    launch_scan(sm, L.chunk_cnt, nact, L.chunk_off_node, L.counters + C_CHUNKS, L.tile_sums);
-    BTRY(hipMemsetAsync(L.counters + C_NEXT, 0, sizeof(uint32_t) * 4, sm));  // C_NEXT .. C_NEXT_CHUNKS
+    // chunks of the level's nodes; C_NEXT .. C_NEXT_CHUNKS cleared
+    if (nact <= SCAN_ONE_MAX) {
+      hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, sm, active, nact, L.nodes, L.chunk_cnt,
+                         L.chunk_off_node, L.counters + C_CHUNKS, L.counters + C_NEXT, 4u);
+    } else {
+      hipLaunchKernelGGL(k_chunk_count, dim3(nblocks(nact)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_cnt);
+      launch_scan(sm, L.chunk_cnt, nact, L.chunk_off_node, L.counters + C_CHUNKS, L.tile_sums);
+      BTRY(hipMemsetAsync(L.counters + C_NEXT, 0, sizeof(uint32_t) * 4, sm));
+    }
     if (nchunks > L.max_chunks) {
       err = "builder: chunk list overflow";
       return MCR_E_OVERFLOW;
     }
-    hipLaunchKernelGGL(k_chunk_emit, dim3(nblocks(nchunks)), dim3(256), 0, sm, active, nact, L.nodes, L.chunk_cnt,
-                       L.chunk_off_node, nchunks, L.chunks);
     // Copies of each node's bins: about one per 4 of its chunks, at most 16 (the bins buffer holds
-    // max_active sets).
+    // max_active sets). The chunk records and the bins' initial values in one launch.
     uint32_t copies = (uint32_t)std::min<size_t>(
         {16, std::max<size_t>(1, L.max_active / nact), std::max<size_t>(1, nchunks / (4 * (size_t)nact))});
     uint32_t words = (uint32_t)BIN_WORDS * nact * copies;
-    hipLaunchKernelGGL(k_bins_init, dim3(nblocks(words)), dim3(256), 0, sm, L.bins, words);
+    hipLaunchKernelGGL(k_chunk_emit, dim3(nblocks(std::max(nchunks, words))), dim3(256), 0, sm, active, nact, L.nodes,
+                       L.chunk_cnt, L.chunk_off_node, nchunks, L.chunks, L.bins, words);
     hipLaunchKernelGGL(k_bin, dim3(nchunks), dim3(256), 0, sm, L.chunks, nchunks, active, L.nodes, L.P[cur], L.bins,
                        copies);
     hipLaunchKernelGGL(k_decide, dim3(nblocks(nact, 4)), dim3(256), 0, sm, active, nact, L.nodes, L.bins, copies,
