@@ -277,7 +277,7 @@ typedef struct mcr_server_config {
   uint32_t slots;           /* path slots of the lane that takes the passes; 0 = 2^21 with one
                                lane, 2^22 beside a preview lane */
   uint32_t window;          /* unstarted samples kept queued ahead of the slots; 0 = slots */
-  uint32_t max_requests;    /* admitted requests at once; 0 = 64 */
+  uint32_t max_requests;    /* admitted requests at once; 0 = 64, at most 1024 */
   int32_t use_graph;        /* replay captured bounce iterations (1) or launch directly (0) */
   uint32_t max_lights;      /* light slots per path: the most lights (plus 1 with environments) a
                                scene added to the server may have; 0 = 16, at most MCR_MAX_LIGHTS_PER_SCENE */
@@ -334,7 +334,9 @@ int mcr_server_submit(mcr_server *server, const mcr_request *request, uint64_t *
 int mcr_server_cancel(mcr_server *server, uint64_t id);
 int mcr_server_status(mcr_server *server, uint64_t id, mcr_request_status *out);
 /* Waits until the request is done, cancelled or failed; MCR_E_TIMEOUT after timeout_ms (< 0:
- * no limit). */
+ * no limit). A failed request returns its own error: MCR_E_OVERFLOW when one of its shadow rays
+ * crossed more counted see-through surfaces than the device keeps (that request alone fails;
+ * the server and its other requests go on), or the device error that ended every request. */
 int mcr_server_wait(mcr_server *server, uint64_t id, double timeout_ms);
 /* The request's current frame, width*height*4 bytes, straight alpha (the frame image). */
 int mcr_server_read(mcr_server *server, uint64_t id, uint8_t *rgba_out);

@@ -61,6 +61,8 @@ struct SrvRequest {
   uint64_t id = 0;
   mcr_request r = {};
   int state = MCR_REQ_QUEUED;
+  int error = MCR_OK;      // a failed request's own error (MCR_OK: the server's failure ended it)
+  std::string error_msg;
   int view = -1;
   uint8_t *image = nullptr;
   float *sum = nullptr;
@@ -333,26 +335,31 @@ int admit(mcr_server *s, SrvRequest &r) {
   return MCR_OK;
 }
 
+// Ends a queued or running request as cancelled or failed: its unstarted items are dropped and
+// the tile results not yet encoded return to their arenas. Its samples in flight finish and are
+// discarded (collect); retire() frees its buffers once none is left.
+void end_request(mcr_server *s, SrvRequest &r, int state) {
+  if (r.state == MCR_REQ_QUEUED)
+    s->admit_queue.erase(std::remove(s->admit_queue.begin(), s->admit_queue.end(), r.id), s->admit_queue.end());
+  r.state = state;
+  r.pending_preview.clear();
+  r.pending_pass.clear();
+  for (SrvTile &t : r.tiles) {
+    uint64_t floats = (uint64_t)t.t.width * t.t.height * 4;
+    for (auto &kv : t.ready) result_lane(s, false).arena.release(kv.second, floats);
+    t.ready.clear();
+    if (t.preview != UINT64_MAX) result_lane(s, true).arena.release(t.preview, floats);
+    t.preview = UINT64_MAX;
+  }
+  r.t_done = s->now();
+  s->changed.notify_all();
+}
+
 // Admissions, cancellations and releases (under the lock).
 int housekeeping(mcr_server *s) {
   for (auto it = s->reqs.begin(); it != s->reqs.end();) {
     SrvRequest &r = *it->second;
-    if (r.cancel && (r.state == MCR_REQ_QUEUED || r.state == MCR_REQ_RUNNING)) {
-      if (r.state == MCR_REQ_QUEUED)
-        s->admit_queue.erase(std::remove(s->admit_queue.begin(), s->admit_queue.end(), r.id), s->admit_queue.end());
-      r.state = MCR_REQ_CANCELLED;
-      r.pending_preview.clear();
-      r.pending_pass.clear();
-      for (SrvTile &t : r.tiles) {
-        uint64_t floats = (uint64_t)t.t.width * t.t.height * 4;
-        for (auto &kv : t.ready) result_lane(s, false).arena.release(kv.second, floats);
-        t.ready.clear();
-        if (t.preview != UINT64_MAX) result_lane(s, true).arena.release(t.preview, floats);
-        t.preview = UINT64_MAX;
-      }
-      r.t_done = s->now();
-      s->changed.notify_all();
-    }
+    if (r.cancel && (r.state == MCR_REQ_QUEUED || r.state == MCR_REQ_RUNNING)) end_request(s, r, MCR_REQ_CANCELLED);
     retire(s, r);
     if (r.release && r.view < 0 && r.jobs_live == 0) {
       after_encodes(s, s->stream);
@@ -698,6 +705,29 @@ int ensure_host(Lane &L, uint32_t n) {
   return MCR_OK;
 }
 
+// A shadow ray crossed more counted see-through surfaces than the reference-order pass keeps
+// (the reference's list has no bound, so the image would differ from its raytrace()): the step's
+// counts name the views of such rays, and only the requests holding those views fail. Every
+// request's samples of the step have finished, so a flagged view still belongs to the request
+// whose samples set it (views return to the pool only once a request has no job left).
+int fail_overflowed(mcr_server *s, Lane &L) {
+  uint32_t *c = L.eng.counts_host;
+  for (uint64_t id : s->running) {
+    SrvRequest &r = *s->reqs[id];
+    if (r.view < 0 || r.state != MCR_REQ_RUNNING) continue;
+    if (!((c[C_FAIL_BITS + r.view / 32] >> (r.view % 32)) & 1u)) continue;
+    r.error = MCR_E_OVERFLOW;
+    r.error_msg = "a shadow ray crossed more than " + std::to_string(MCR_DEEP_HITS) + " counted see-through surfaces";
+    end_request(s, r, MCR_REQ_FAILED);
+  }
+  // The lane's next step is ordered after these clears on its stream (not a captured graph).
+  HIP_TRY(hipMemsetAsync(L.eng.w.counts + cw(C_OVERFLOW), 0, sizeof(uint32_t), L.stream));
+  HIP_TRY(hipMemsetAsync(L.eng.w.counts + C_FAIL_BITS, 0, sizeof(uint32_t) * (MCR_FAIL_VIEWS / 32), L.stream));
+  c[cw(C_OVERFLOW)] = 0;
+  memset(c + C_FAIL_BITS, 0, sizeof(uint32_t) * (MCR_FAIL_VIEWS / 32));
+  return MCR_OK;
+}
+
 // Lane L's step in flight: if it has finished, takes its counts (no wait otherwise).
 int lane_poll(mcr_server *s, Lane &L, bool &progress) {
   if (!L.inflight) return MCR_OK;
@@ -707,6 +737,7 @@ int lane_poll(mcr_server *s, Lane &L, bool &progress) {
   L.inflight = false;
   L.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - L.launched_at).count();
   progress = true;
+  if (L.eng.counts_host[cw(C_OVERFLOW)] == OVERFLOW_SEEN) MCR_TRY(fail_overflowed(s, L));
   MCR_TRY(engine_check(L.eng));
   for (uint32_t i = 0; i < L.launched; i++) L.table[i].done = L.done_host[i];
   L.counters = counters_sum(L.counters_host);
@@ -964,6 +995,8 @@ int mcr_server_create(mcr_ctx *c, const mcr_server_config *config, mcr_server **
   if (s->cfg.slots < 256 || s->cfg.slots > (1u << 26)) return mcr_fail(MCR_E_INVALID, "slots must be in [256, 2^26]");
   if (!s->cfg.window) s->cfg.window = s->cfg.slots;
   if (!s->cfg.max_requests) s->cfg.max_requests = 64;
+  if (s->cfg.max_requests > MCR_FAIL_VIEWS)
+    return mcr_fail(MCR_E_INVALID, "max_requests must be at most " + std::to_string(MCR_FAIL_VIEWS));
   if (!s->cfg.max_lights) s->cfg.max_lights = 16;
   if (s->cfg.max_lights > MCR_MAX_LIGHTS) return mcr_fail(MCR_E_INVALID, "max_lights beyond MCR_MAX_LIGHTS_PER_SCENE");
   if (!s->cfg.preview_slots) s->cfg.preview_slots = SERVER_DEFAULT_PREVIEW_SLOTS;
@@ -1162,7 +1195,9 @@ int mcr_server_wait(mcr_server *s, uint64_t id, double timeout_ms) {
     return mcr_fail(MCR_E_TIMEOUT, "the request is still running");
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return mcr_fail(MCR_E_INVALID, "no such request");
-  if (it->second->state == MCR_REQ_FAILED) return mcr_fail(s->fatal ? s->fatal : MCR_E_DEVICE, s->fatal_msg);
+  const SrvRequest &r = *it->second;
+  if (r.state == MCR_REQ_FAILED)
+    return r.error ? mcr_fail(r.error, r.error_msg) : mcr_fail(s->fatal ? s->fatal : MCR_E_DEVICE, s->fatal_msg);
   if (it->second->state == MCR_REQ_CANCELLED) return mcr_fail(MCR_E_CANCELLED, "the request was cancelled");
   return MCR_OK;
 }

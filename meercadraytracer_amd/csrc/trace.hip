@@ -1455,14 +1455,20 @@ __global__ __launch_bounds__(256) void k_shadow_deep(const ViewDev *__restrict__
     double dx = w.sray[3 * RS + idx], dy = w.sray[4 * RS + idx], dz = w.sray[5 * RS + idx];
     double maxT = w.sray[6 * RS + idx];
     double c[3];
-    with_view<MIX>(views, (uint32_t)w.i[I_VIEW * P + slot], [&](const ViewDev &V) {
+    uint32_t view = MIX ? (uint32_t)w.i[I_VIEW * P + slot] : 0;
+    bool over = false;
+    with_view<MIX>(views, view, [&](const ViewDev &V) {
       const DevScene &sc = V.sc;
       DeepShadow<TEX> vis{&sc, V.st.shadows == MCR_SHADOWS_CAST && li != V.light_slots, w.deep_seen, gid, 0, false,
                           {1, 1, 1}, {0, 0, 0}};
       intersect_any(sc, ox, oy, oz, dx, dy, dz, 0, maxT, st, tc, vis);
-      overflow |= vis.overflow;
+      over = vis.overflow;
       for (int k = 0; k < 3; k++) c[k] = vis.depth[k] > 0 ? vis.c[k] * jm_exp(-vis.depth[k]) : vis.c[k];
     });
+    if (over) {
+      overflow = true;
+      if (view < MCR_FAIL_VIEWS) atomicOr(&w.counts[C_FAIL_BITS + view / 32], 1u << (view % 32));
+    }
     bool visible = c[0] + c[1] + c[2] > 0;
     w.shadow[0 * RS + idx] = c[0];
     w.shadow[1 * RS + idx] = c[1];
@@ -1661,6 +1667,7 @@ __global__ __launch_bounds__(256) void k_reset_slots(Wavefront w) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0)
     for (int k = 0; k < C_WORDS; k++) w.counts[cw(k)] = k == C_FREE ? w.capacity : 0;
+  if (i < MCR_FAIL_VIEWS / 32) w.counts[C_FAIL_BITS + i] = 0;
   if (i < w.capacity) w.free_list[i] = (int32_t)(w.capacity - 1 - i);
 }
 

@@ -97,7 +97,12 @@ __host__ __device__ constexpr inline uint32_t sref_light(uint32_t e) { return e 
 // lengths. Wavefront::counts holds C_WORDS such lines; word x is counts[cw(x)].
 #define MCR_COUNT_STRIDE 32
 __host__ __device__ constexpr inline uint32_t cw(uint32_t word) { return word * MCR_COUNT_STRIDE; }
-const uint32_t COUNT_WORDS_ALLOC = C_WORDS * MCR_COUNT_STRIDE;
+// After the counter lines, one bit per view (packed, view v at bit v % 32 of word C_FAIL_BITS +
+// v / 32): a shadow ray of that view overflowed the reference-order pass. A server fails only the
+// requests whose bits are set; views from MCR_FAIL_VIEWS on set OVERFLOW_SEEN alone.
+#define MCR_FAIL_VIEWS 1024
+const uint32_t C_FAIL_BITS = C_WORDS * MCR_COUNT_STRIDE;
+const uint32_t COUNT_WORDS_ALLOC = C_FAIL_BITS + MCR_FAIL_VIEWS / 32;
 
 // Work distribution of the traversal kernels over a queue of n items in 64-item chunks. One
 // counter would serialise every wave's atomic on one cache line; MCR_FETCH_COUNTERS counters

@@ -270,3 +270,73 @@ def test_release_of_a_held_scene_waits_for_the_server():
         s.handle = None
     finally:
         srv.close()
+
+
+def _stack_scene(layers):
+    """A floor under `layers` stacked see-through planes that barely dim light (opacity 1e-4 each,
+    0.9 of the light left after 1,100): a shadow ray from the floor to the sun crosses every layer
+    before its colour falls below the reference's 1e-4 cut-off (tracer.ts:572-587)."""
+    import math
+    from meercadraytracer_amd import three as T
+    s = T.Scene()
+    floor = T.Mesh(T.PlaneGeometry(6, 6), T.MeshStandardMaterial(color=0xdddddd, roughness=0.8))
+    floor.rotation = [-math.pi / 2, 0.0, 0.0]
+    s.add(floor)
+    mat = T.MeshStandardMaterial(color=0xffffff, transparent=True, opacity=1e-4)
+    for i in range(layers):
+        layer = T.Mesh(T.PlaneGeometry(5, 5), mat)
+        layer.rotation = [-math.pi / 2, 0.0, 0.0]
+        layer.position = [0.0, 1.0 + 0.004 * i, 0.0]
+        layer.castShadow = True
+        s.add(layer)
+    sun = T.DirectionalLight(0xffffff, 2.0)
+    sun.position = [0.3, 5.0, 0.2]
+    sun.castShadow = True
+    s.add(sun)
+    s.updateMatrixWorld()
+    cam = T.PerspectiveCamera(60, 1.5, 0.05, 100)  # below the stack, looking at the floor under it
+    cam.position = [0.0, 0.6, 2.0]
+    cam.lookAt(0, 0, 0)
+    cam.updateMatrixWorld()
+    return s, cam
+
+
+def test_shadow_overflow_fails_only_its_request(world):
+    """A shadow ray through 1,100 counted see-through layers: more than the device keeps for its
+    reference-order replay, while the reference's list has no bound (tracer.ts:48, 544-594), so
+    the image could not equal the reference's. A direct render fails with MCR_E_OVERFLOW; on a
+    server only that request fails, and the requests beside it and after it finish exact."""
+    from meercadraytracer_amd.compile import compile_camera, compile_scene
+    from meercadraytracer_amd.types import Tile
+    scene, camera = _stack_scene(1100)
+    sc, cam = compile_scene(scene, oracle.build_bvh), compile_camera(camera)
+    st = trace_settings({})
+    W, H = 24, 16
+    s = native.Scene()
+    s.set_geometry(sc.geometry, 1)
+    s.set_shading(sc.shading, 1)
+    srv = _server(world, slots=4096)
+    try:
+        print('direct render of the 1,100-layer stack', flush=True)
+        with pytest.raises(native.NativeError) as e:
+            s.render_tile(cam, st, Tile(0, 0, W, H), W, H, 0, 1)
+        assert e.value.code == abi.MCR_E_OVERFLOW and 'see-through' in e.value.message
+        srv.add_scene(s)
+        print('served: the stack beside a plain request', flush=True)
+        bad = srv.submit(s, cam, st, W, H, 2, 8, 4)
+        good = _submit(srv, world, REQUESTS[0])
+        with pytest.raises(native.NativeError) as e:
+            srv.wait(bad, 60000)
+        assert e.value.code == abi.MCR_E_OVERFLOW and 'see-through' in e.value.message
+        assert srv.status(bad)['state'] == abi.REQ_FAILED
+        assert srv.wait(good, 60000) == abi.REQ_DONE
+        assert np.array_equal(srv.read(good), _expected(world, REQUESTS[0]))
+        srv.release(bad)
+        srv.release(good)
+        print('served: a request after the failure', flush=True)
+        later = _submit(srv, world, REQUESTS[5])
+        assert srv.wait(later, 60000) == abi.REQ_DONE
+        assert np.array_equal(srv.read(later), _expected(world, REQUESTS[5]))
+    finally:
+        srv.close()
+        s.release()
