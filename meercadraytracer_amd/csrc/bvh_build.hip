@@ -90,11 +90,6 @@ struct Prims {  // structure of arrays, `stride` entries per field
   __device__ void set(int field, uint32_t p, float v) { f[(size_t)field * stride + p] = v; }
 };
 
-__device__ inline void copy_prim(const Prims &a, uint32_t p, Prims &b, uint32_t q) {
-  for (int k = 0; k < 9; k++) b.set(k, q, a.get(k, p));
-  b.id[q] = a.id[p];
-}
-
 // bvh.ts:22-33
 __global__ void k_prim_init(const float *tris, uint32_t n, Prims P) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1031,6 +1026,13 @@ __global__ __launch_bounds__(256) void k_part_classify(const int4 *chunks, const
                                                        const BNode *nodes, Prims P, uint8_t *cls,
                                                        uint16_t *lpre, uint32_t *chunk_l) {
   int4 ch = chunks[blockIdx.x];
+  uint32_t p = (uint32_t)(ch.y + threadIdx.x);
+  bool in = (int)threadIdx.x < ch.z;
+  // The centroid's three coordinates go out with the node's read (the split axis is the node's):
+  // one dependent round trip less per block.
+  float cen[3] = {0, 0, 0};
+  if (in)
+    for (int a = 0; a < 3; a++) cen[a] = P.get(6 + a, p);
   const BNode &nd = nodes[ch.w];
   if (!(nd.flags & NF_SPLIT)) {  // uniform over the block
     if (threadIdx.x == 0) chunk_l[blockIdx.x] = 0;
@@ -1038,10 +1040,9 @@ __global__ __launch_bounds__(256) void k_part_classify(const int4 *chunks, const
   }
   NodeGeom g = node_geom(nd.ckey);
   uint32_t isL = 0;
-  uint32_t p = (uint32_t)(ch.y + threadIdx.x);
-  bool in = (int)threadIdx.x < ch.z;
   if (in) {
-    int b = bin_of((double)P.get(6 + nd.axis, p), g.lo[nd.axis], g.scale[nd.axis]);
+    const float c = nd.axis == 0 ? cen[0] : nd.axis == 1 ? cen[1] : cen[2];
+    int b = bin_of((double)c, g.lo[nd.axis], g.scale[nd.axis]);
     isL = b <= nd.split ? 1u : 0u;
     cls[p] = isL ? 0 : 1;
   }
@@ -1072,14 +1073,17 @@ __global__ __launch_bounds__(256) void k_part_fb(const int4 *chunks, const int32
                                                  const uint16_t *lpre, const uint32_t *chunk_off,
                                                  int32_t *fr, int32_t *bl, uint32_t *counters) {
   int4 ch = chunks[blockIdx.x];
-  const BNode &nd = nodes[ch.w];
-  if (!(nd.flags & NF_SPLIT) || (int)threadIdx.x >= ch.z) return;
+  if ((int)threadIdx.x >= ch.z) return;
   int p = ch.y + (int)threadIdx.x;
-  int L = (int)(chunk_off[blockIdx.x] - chunk_off[nd.chunk_begin] + lpre[p]);  // lefts in [first, p)
+  // The primitive's class and prefix go out with the node's read.
+  const uint32_t cp = cls[p], lp = lpre[p], off = chunk_off[blockIdx.x];
+  const BNode &nd = nodes[ch.w];
+  if (!(nd.flags & NF_SPLIT)) return;
+  int L = (int)(off - chunk_off[nd.chunk_begin] + lp);  // lefts in [first, p)
   int last = nd.first + nd.n - 1;
-  int j = p < nd.first + nd.F ? (cls[p] ? (p - nd.first) - L : -1) : (cls[p] ? -1 : nd.nl - 1 - L);
+  int j = p < nd.first + nd.F ? (cp ? (p - nd.first) - L : -1) : (cp ? -1 : nd.nl - 1 - L);
   if (j < 0) {
-    if (p < nd.first + nd.F ? cls[p] : !cls[p]) atomicOr(&counters[C_ERROR], 8u);
+    if (p < nd.first + nd.F ? cp : !cp) atomicOr(&counters[C_ERROR], 8u);
     return;
   }
   if (j >= nd.n) {
@@ -1098,22 +1102,31 @@ __global__ __launch_bounds__(256) void k_scatter(const int4 *chunks, const int32
                                                  const uint32_t *chunk_off, const int32_t *fr,
                                                  const int32_t *bl, uint32_t *order) {
   int4 ch = chunks[blockIdx.x];
-  const BNode &nd = nodes[ch.w];
   if ((int)threadIdx.x >= ch.z) return;
   int p = ch.y + (int)threadIdx.x;
+  // The primitive, its class and prefix go out with the node's read: every case below moves the
+  // primitive (or writes its id), so its loads need not wait for the node.
+  float pf[9];
+  for (int k = 0; k < 9; k++) pf[k] = A.get(k, (uint32_t)p);
+  const uint32_t id = A.id[p], cp = cls[p], lp = lpre[p], off = chunk_off[blockIdx.x];
+  const BNode &nd = nodes[ch.w];
+  auto put = [&](int q) {
+    for (int k = 0; k < 9; k++) B.set(k, (uint32_t)q, pf[k]);
+    B.id[q] = id;
+  };
   if (nd.flags & NF_SAHLEAF) {
-    order[p] = A.id[p];
+    order[p] = id;
     return;
   }
   if (nd.flags & NF_HALF) {
-    copy_prim(A, (uint32_t)p, B, (uint32_t)p);
+    put(p);
     return;
   }
-  int L = (int)(chunk_off[blockIdx.x] - chunk_off[nd.chunk_begin] + lpre[p]);
+  int L = (int)(off - chunk_off[nd.chunk_begin] + lp);
   int first = nd.first, last = nd.first + nd.n - 1;
   int np;
   if (p < first + nd.F) {
-    if (!cls[p]) {
+    if (!cp) {
       np = p;
     } else {
       int k = (p - first) - L;
@@ -1121,12 +1134,12 @@ __global__ __launch_bounds__(256) void k_scatter(const int4 *chunks, const int32
     }
   } else {
     int j = nd.nl - 1 - L;
-    if (cls[p]) np = p - 1;
+    if (cp) np = p - 1;
     else np = j < 0 || j >= nd.n ? p : first + fr[first + j];
   }
   // A corrupt partition (reported through the error word) must still write inside the node.
   if (np < first || np > last) np = p;
-  copy_prim(A, (uint32_t)p, B, (uint32_t)np);
+  put(np);
 }
 
 __global__ void k_flush_leaves(const int32_t *list, uint32_t n, const BNode *nodes, Prims P, uint32_t *order) {
