@@ -964,55 +964,82 @@ __device__ inline uint32_t shade_class(const DevScene &sc, int k) {
 // after another, so all but at most MCR_SHADE_CLASSES - 1 of its waves hold one class. Every
 // sample keeps its own random stream, so the order shade visits slots in changes no result.
 template <bool MIX>
+// Queue entries per thread of k_shade_bin: a wave takes SB_RUNS runs of 64 consecutive entries, so
+// a block reserves its bins' space for 1,024 entries with one returning atomic per class (the
+// blocks' same-address atomics on the class counters bound the kernel at 256 entries per block).
+#define SB_RUNS 4
 __global__ __launch_bounds__(256) void k_shade_bin(const ViewDev *__restrict__ views, Wavefront w, int q,
                                                    DevCounters *cnt) {
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = w.counts[cw(q)];
-  if (blockIdx.x * blockDim.x >= n) return;  // every thread of a block that goes on reaches the append
-  bool live = gid < n;
+  if (blockIdx.x * blockDim.x * SB_RUNS >= n) return;  // every thread of a block that goes on reaches the append
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t first = (blockIdx.x * 4 + wv) * 64 * SB_RUNS + lane;  // run r: first + 64 r
   size_t P = w.capacity;
-  int slot = live ? w.queue[q][gid] : 0;
-  uint32_t cls = 0;
-  if (live) {
-    int k = w.i[I_HTRI * P + slot];
-    uint32_t view = MIX ? (uint32_t)w.i[I_VIEW * P + slot] : 0;
-    with_view<MIX>(views, view, [&](const ViewDev &V) { cls = shade_class(V.sc, k); });
+  int slot[SB_RUNS];
+  uint32_t cls[SB_RUNS];
+#pragma unroll
+  for (int r = 0; r < SB_RUNS; r++) slot[r] = first + 64 * r < n ? w.queue[q][first + 64 * r] : -1;
+#pragma unroll
+  for (int r = 0; r < SB_RUNS; r++) {
+    cls[r] = MCR_SHADE_CLASSES;  // none
+    if (slot[r] >= 0) {
+      int k = w.i[I_HTRI * P + slot[r]];
+      uint32_t view = MIX ? (uint32_t)w.i[I_VIEW * P + slot[r]] : 0;
+      with_view<MIX>(views, view, [&](const ViewDev &V) { cls[r] = shade_class(V.sc, k); });
+    }
   }
+  // Positions in queue order within each class: a class's entries of run r follow those of runs
+  // before it in this wave, lanes in order within a run.
+  const uint64_t below = (1ull << lane) - 1ull;
   uint32_t *ctr[MCR_SHADE_CLASSES];
-  bool want[MCR_SHADE_CLASSES];
-  uint32_t pos[MCR_SHADE_CLASSES];
+  uint32_t total[MCR_SHADE_CLASSES], pos[MCR_SHADE_CLASSES], at = 0;
+  uint32_t kinds_mixed = 0;  // runs (64 consecutive entries, shade's waves) holding more than one class
 #pragma unroll
   for (int c = 0; c < MCR_SHADE_CLASSES; c++) {
     ctr[c] = &w.counts[cw(C_BIN + c)];
-    want[c] = live && cls == (uint32_t)c;
+    total[c] = 0;
+  }
+  uint32_t off[SB_RUNS];
+#pragma unroll
+  for (int r = 0; r < SB_RUNS; r++) {
+    uint32_t kinds = 0;
+    off[r] = 0;
+#pragma unroll
+    for (int c = 0; c < MCR_SHADE_CLASSES; c++) {
+      uint64_t m = __ballot(cls[r] == (uint32_t)c);
+      kinds += m != 0;
+      if (cls[r] == (uint32_t)c) off[r] = total[c] + (uint32_t)__popcll(m & below);
+      total[c] += (uint32_t)__popcll(m);
+    }
+    kinds_mixed += kinds > 1;
   }
   __shared__ uint32_t lds[4 * MCR_SHADE_CLASSES];
-  block_append<MCR_SHADE_CLASSES>(ctr, want, lds, pos);
-  // The class mix shade would see in queue order (a wave's lanes in more than one class), and the
-  // lanes per class: block totals from the wave ballots, one atomic per class per block.
+  block_reserve<MCR_SHADE_CLASSES>(ctr, total, lds, pos);
+  // Statistics: lanes per class and mixed runs, one atomic per class per block.
   {
-    uint32_t kinds = 0;
-#pragma unroll
-    for (int c = 0; c < MCR_SHADE_CLASSES; c++) kinds += __ballot(want[c]) != 0;
-    __shared__ uint32_t mixed_waves, lanes[MCR_SHADE_CLASSES];
+    __shared__ uint32_t mixed_runs, lanes[MCR_SHADE_CLASSES];
     if (threadIdx.x < MCR_SHADE_CLASSES) lanes[threadIdx.x] = 0;
-    if (threadIdx.x == 0) mixed_waves = 0;
+    if (threadIdx.x == 0) mixed_runs = 0;
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
-      if (kinds > 1) atomicAdd(&mixed_waves, 1u);
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < MCR_SHADE_CLASSES; c++)
+        if (total[c]) atomicAdd(&lanes[c], total[c]);
+      if (kinds_mixed) atomicAdd(&mixed_runs, kinds_mixed);
     }
-    if (live) atomicAdd(&lanes[cls], 1u);
     __syncthreads();
     DevCounters *cc = counter_copy(cnt);
     if (threadIdx.x < MCR_SHADE_CLASSES && lanes[threadIdx.x])
       atomicAdd(&cc->shade_class_lanes[threadIdx.x], (unsigned long long)lanes[threadIdx.x]);
-    if (threadIdx.x == 0 && mixed_waves) atomicAdd(&cc->class_mixed_waves, (unsigned long long)mixed_waves);
+    if (threadIdx.x == 0 && mixed_runs) atomicAdd(&cc->class_mixed_waves, (unsigned long long)mixed_runs);
   }
-  uint32_t at = 0;
 #pragma unroll
-  for (int c = 0; c < MCR_SHADE_CLASSES; c++)
-    if (cls == (uint32_t)c) at = pos[c];
-  if (live) w.bins[(size_t)cls * P + at] = slot;
+  for (int r = 0; r < SB_RUNS; r++) {
+#pragma unroll
+    for (int c = 0; c < MCR_SHADE_CLASSES; c++)
+      if (cls[r] == (uint32_t)c) at = pos[c];
+    if (slot[r] >= 0) w.bins[(size_t)cls[r] * P + at + off[r]] = slot[r];
+  }
 }
 
 // Entry i of the binned queue: the regions of w.bins one after another.
@@ -1834,7 +1861,7 @@ void launch_extend(hipStream_t s, bool mix, bool wide, const ViewDev *views, con
 }
 void launch_shade_bin(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                       uint32_t max_live, DevCounters *cnt) {
-  MCR_LAUNCH_MIX(k_shade_bin, blocks_for(max_live ? max_live : 1), s, views, w, q, cnt);
+  MCR_LAUNCH_MIX(k_shade_bin, blocks_for(max_live ? (max_live + SB_RUNS - 1) / SB_RUNS : 1), s, views, w, q, cnt);
 }
 void launch_shade(hipStream_t s, bool mix, bool binned, const ViewDev *views, const Wavefront &w, int q,
                   uint32_t max_live, DevCounters *cnt) {
