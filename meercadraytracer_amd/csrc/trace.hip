@@ -27,39 +27,41 @@ __device__ inline double nt_ld(const double *p) { return __builtin_nontemporal_l
 
 
 // Block-wide queue appends. A same-address atomic per wave serialises thousands of waves on the
-// counter's cache line (the resolve pass spent most of its time there); one atomic per block of
-// 256 threads (four waves) per counter cuts that four times. Every thread of the block must call
-// these; `lds` is shared memory of 4 words per counter.
+// counter's cache line (the resolve pass spent most of its time there); one atomic per block per
+// counter cuts that by the block's wave count (WAVES: 4 for 256 threads, 16 for 1,024). Every
+// thread of the block must call these; `lds` is shared memory of WAVES words per counter.
 //
 // block_reserve: the base of this wave's `total` entries (wave-uniform) on each of K counters.
-template <int K>
+template <int K, int WAVES = 4>
 __device__ inline void block_reserve(uint32_t *const (&counter)[K], const uint32_t (&total)[K], uint32_t *lds,
                                      uint32_t (&base)[K]) {
   uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0)
-    for (int k = 0; k < K; k++) lds[4 * k + wv] = total[k];
+    for (int k = 0; k < K; k++) lds[WAVES * k + wv] = total[k];
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t got[K];
     for (int k = 0; k < K; k++) {  // all K atomics in flight before any result is used
-      uint32_t t = lds[4 * k] + lds[4 * k + 1] + lds[4 * k + 2] + lds[4 * k + 3];
+      uint32_t t = 0;
+      for (int v = 0; v < WAVES; v++) t += lds[WAVES * k + v];
       got[k] = t ? atomicAdd(counter[k], t) : 0u;
     }
     for (int k = 0; k < K; k++) {
-      uint32_t a = lds[4 * k], b = lds[4 * k + 1], c = lds[4 * k + 2];
-      lds[4 * k] = got[k];
-      lds[4 * k + 1] = got[k] + a;
-      lds[4 * k + 2] = got[k] + a + b;
-      lds[4 * k + 3] = got[k] + a + b + c;
+      uint32_t acc = got[k];
+      for (int v = 0; v < WAVES; v++) {
+        uint32_t x = lds[WAVES * k + v];
+        lds[WAVES * k + v] = acc;
+        acc += x;
+      }
     }
   }
   __syncthreads();
-  for (int k = 0; k < K; k++) base[k] = lds[4 * k + wv];
+  for (int k = 0; k < K; k++) base[k] = lds[WAVES * k + wv];
   __syncthreads();  // the words are reused by the next call
 }
 
 // block_append: one entry for each thread with `want`, on each of K counters; returns positions.
-template <int K>
+template <int K, int WAVES = 4>
 __device__ inline void block_append(uint32_t *const (&counter)[K], const bool (&want)[K], uint32_t *lds,
                                     uint32_t (&pos)[K]) {
   uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
@@ -69,7 +71,7 @@ __device__ inline void block_append(uint32_t *const (&counter)[K], const bool (&
     m[k] = __ballot(want[k]);
     total[k] = (uint32_t)__popcll(m[k]);
   }
-  block_reserve<K>(counter, total, lds, pos);
+  block_reserve<K, WAVES>(counter, total, lds, pos);
   for (int k = 0; k < K; k++) pos[k] += (uint32_t)__popcll(m[k] & below);
 }
 
@@ -1595,8 +1597,11 @@ MCR_INLINE void resolve_slot(const ViewDev &V, const Wavefront &w, const Control
   }
 }
 
+// 1,024 threads per block: three same-address atomics per block (the sample counter, the free
+// list and the next queue) for four times the slots of a 256-thread block.
+#define RESOLVE_THREADS 1024
 template <bool MIX>
-__global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ views, Wavefront w, int q) {
+__global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(const ViewDev *__restrict__ views, Wavefront w, int q) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = w.counts[cw(q)];
   if (blockIdx.x * blockDim.x >= n) return;
@@ -1617,13 +1622,14 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
   // A finished slot starts the call's next sample at once, so the slot keeps its place in the
   // queue and consecutive lanes keep reading consecutive slots. With no sample left it goes to
   // the free list, from which a refill restarts it when the job list grows.
-  __shared__ uint32_t lds[8];
+  constexpr int WAVES = RESOLVE_THREADS / 64;
+  __shared__ uint32_t lds[2 * WAVES];
   uint32_t next;
   {
     uint32_t *c[1] = {&w.counts[cw(C_NEXT)]};
     bool want[1] = {finished};
     uint32_t pos[1];
-    block_append<1>(c, want, lds, pos);
+    block_append<1, WAVES>(c, want, lds, pos);
     next = pos[0];
   }
   bool idle = false;
@@ -1638,7 +1644,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ViewDev *__restrict__ vie
   uint32_t *c[2] = {&w.counts[cw(C_FREE)], &w.counts[cw(q ^ 1)]};
   bool want[2] = {idle, requeue};
   uint32_t pos[2];
-  block_append<2>(c, want, lds, pos);
+  block_append<2, WAVES>(c, want, lds, pos);
   if (idle) w.free_list[pos[0]] = slot;
   if (requeue) w.queue[q ^ 1][pos[1]] = slot;
 }
@@ -1914,7 +1920,9 @@ void launch_shadow_deep(hipStream_t s, bool mix, bool tex, const ViewDev *views,
 }
 void launch_resolve(hipStream_t s, bool mix, const ViewDev *views, const Wavefront &w, int q,
                     uint32_t max_live) {
-  MCR_LAUNCH_MIX(k_resolve, blocks_for(max_live ? max_live : 1), s, views, w, q);
+  const uint32_t grid = ((max_live ? max_live : 1) + RESOLVE_THREADS - 1) / RESOLVE_THREADS;
+  if (mix) hipLaunchKernelGGL(k_resolve<true>, dim3(grid), dim3(RESOLVE_THREADS), 0, s, views, w, q);
+  else hipLaunchKernelGGL(k_resolve<false>, dim3(grid), dim3(RESOLVE_THREADS), 0, s, views, w, q);
 }
 void launch_probe_closest(hipStream_t s, const DevScene &sc, const double *rays, uint32_t n,
                           double tmin, double tmax, int cull, int32_t *tri, double *t, double *u,
