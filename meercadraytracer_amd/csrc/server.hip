@@ -80,6 +80,7 @@ struct SrvRequest {
   // finished: the encode launches (by lane) they wait for, set when the host staged the last write.
   bool preview_staged = false, done_staged = false;
   uint64_t preview_fence[2] = {0, 0}, done_fence[2] = {0, 0};
+  uint64_t staged_seq[2] = {0, 0};  // per lane, the fence of its last encode call holding an item of this request
 };
 
 struct SrvJob {
@@ -226,6 +227,9 @@ struct mcr_server {
   uint64_t arena_floats = 0;
   std::vector<ViewDev> views;
   std::vector<int> view_free;
+  // The most light slots of a scene the server has held: the captured iterations' shadow grids are
+  // sized by it, so they are captured once, not again each time a request with more lights comes.
+  uint32_t scene_slots = 0;
   std::unordered_map<uint64_t, std::unique_ptr<SrvRequest>> reqs;
   std::deque<uint64_t> admit_queue;
   std::vector<uint64_t> running;   // admitted, in admission order
@@ -547,6 +551,7 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
   const uint32_t cap = s->enc_cap / 2;
   EncodeItem *enc_host = L.enc_host + (size_t)L.half * cap, *enc_dev = L.enc_dev + (size_t)L.half * cap;
   std::vector<TileRef> cur, again;
+  std::vector<SrvRequest *> touched;  // requests with an item staged by this call
   cur.swap(L.dirty);
   for (;;) {
     uint32_t first = nenc;
@@ -592,6 +597,7 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
         continue;
       }
       bool wide = r.r.samples > WIDE_SUM_SAMPLES;
+      touched.push_back(&r);
       enc_host[nenc++] = EncodeItem{r.image, preview || wide ? nullptr : r.sum,
                                       preview || !wide ? nullptr : reinterpret_cast<double *>(r.sum),
                                       s->arena_dev + data, r.r.width,
@@ -641,22 +647,26 @@ int deliver(mcr_server *s, Lane &L, bool &progress) {
     L.fences.emplace_back(++L.enc_seq, ev);
     L.half_fence[L.half] = L.enc_seq;
     L.half ^= 1;
+    const int li = __builtin_ctz(L.bit);
+    for (SrvRequest *r : touched) r->staged_seq[li] = L.enc_seq;
   }
   // The ranges go back to the allocator once this call's fence has completed (fences_poll): a step
   // of any lane may reuse them then without waiting for the encodes on the device.
   for (auto &f : to_free) L.deferred.push_back(Lane::Deferred{L.enc_seq, f.first, f.second});
-  // A request whose last preview tile or last pass has been staged waits for the encodes launched
-  // so far on every lane; stamp() marks it once they have finished.
+  // A request whose last preview tile or last pass has been staged waits for the encodes that hold
+  // its items on each lane (the lane's last encode call with one of them; stream order on the
+  // lane's encode stream covers the earlier ones); stamp() marks it once they have finished. An
+  // encode of another request does not hold it back.
   for (uint64_t id : s->running) {
     SrvRequest &r = *s->reqs[id];
     if (r.state != MCR_REQ_RUNNING) continue;
     if (!r.preview_staged && r.covered == r.tiles.size()) {
       r.preview_staged = true;
-      for (size_t k = 0; k < s->lanes.size(); k++) r.preview_fence[k] = s->lanes[k]->enc_seq;
+      for (size_t k = 0; k < s->lanes.size(); k++) r.preview_fence[k] = r.staged_seq[k];
     }
     if (!r.done_staged && r.finished_tiles == r.tiles.size() && r.pending_pass.empty() && r.pending_preview.empty()) {
       r.done_staged = true;
-      for (size_t k = 0; k < s->lanes.size(); k++) r.done_fence[k] = s->lanes[k]->enc_seq;
+      for (size_t k = 0; k < s->lanes.size(); k++) r.done_fence[k] = r.staged_seq[k];
     }
   }
   return MCR_OK;
@@ -786,7 +796,7 @@ int lane_launch(mcr_server *s, Lane &L, bool &progress) {
   // path-state accesses). Steps without a refill keep the queue's own order: waves requeue in
   // the order they finish, which groups paths of similar cost.
   if (k) MCR_TRY(engine_sort_queue(e, L.q, live));
-  uint32_t max_lights = 0;
+  uint32_t max_lights = s->scene_slots;
   e.tex = false;
   for (const ViewDev &v : s->views) {
     max_lights = std::max(max_lights, v.shadow_slots);
@@ -1089,6 +1099,7 @@ int mcr_server_add_scene(mcr_server *s, mcr_scene *scene) {
   uint64_t b = mcr_scene_bytes(scene);
   if (s->bytes_used + b > s->cfg.hbm_budget) return mcr_fail(MCR_E_OVERFLOW, "the scene does not fit the HBM budget");
   s->bytes_used += b;
+  s->scene_slots = std::max(s->scene_slots, slots_needed);
   s->scenes.push_back(scene);
   s->scene_bytes.push_back(b);
   scene_hold(scene);
